@@ -1,0 +1,225 @@
+"""Benchmark: time-to-solution of the IDR(s) hot path on one MI355X.
+
+Workload (default): 3-D convection-diffusion-reaction, 7-point central
+differences on a 200^3 grid (N = 8e6, nnz = 55.76e6), Pe = 50, reaction 1,
+CRS storage, FP64; IDR(4) with a degree-8 Chebyshev polynomial
+preconditioner (foci 0.15, 1.85) and diagonal scaling; b ~ U[-1,1] (seed 0);
+Gaussian shadow space (seed 0); ||r||/||b|| <= 1e-8.  A "step" is one full
+solve with b, x and the matrix already resident in HBM.
+
+Prints one JSON line.  roofline: the CRS SpMV kernel (the dominant kernel),
+achieved = algorithmic bytes of the timed SpMV launches / their summed HIP
+event time, against the 8 TB/s HBM peak.  cpu_baseline: the oracle (the
+project's scalar C restatement of the reference) on a bounded sample of the
+same workload, extrapolated to the GPU's iteration count.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "idrs-package_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def spmv_bytes(n, nnz, with_z, epi_vectors):
+    """Algorithmic bytes of one FP64 CRS SpMV launch with int32 indices:
+    values + column indices (12 nnz), row pointers (4(N+1)), x read once and
+    y written once (16 N), the Horner vector z (8 N) and epilogue vectors."""
+    return 12 * nnz + 4 * (n + 1) + 16 * n + (8 * n if with_z else 0) + 8 * n * epi_vectors
+
+
+def solve_spmv_bytes(n, nnz, s, d, iterations, prow):
+    """Sum of algorithmic bytes of the SpMV launches of one solve with
+    'iterations' iterations (inner steps + omega steps), plus the final
+    true-residual product.  Each operator application: d launches with z,
+    one final launch with the epilogue (P^T G reads s vectors; the omega
+    epilogue reads r)."""
+    cycles_full, rem = divmod(iterations, s + 1)
+    inner = cycles_full * s + min(rem, s)
+    omega = cycles_full + (1 if rem > s else 0)
+    per_op_z = d * spmv_bytes(n, nnz, True, 0)
+    b_inner = inner * (per_op_z + spmv_bytes(n, nnz, False, prow))
+    b_omega = omega * (per_op_z + spmv_bytes(n, nnz, False, 1))
+    b_final = per_op_z + spmv_bytes(n, nnz, False, 0)
+    launches = (inner + omega + 1) * (d + 1)
+    return b_inner + b_omega + b_final, launches
+
+
+def iteration_bytes(n, nnz, s, d):
+    """Algorithmic bytes per IDR(s) iteration: (d+1) SpMVs, d Horner vector
+    reads and the fused vector traffic 8N(3s^2+11s+3)/(s+1)."""
+    b_spmv = 12 * nnz + 4 * (n + 1) + 16 * n
+    b_op = (d + 1) * b_spmv + d * 8 * n
+    return b_op + 8 * n * (3 * s * s + 11 * s + 3) / (s + 1)
+
+
+def cpu_baseline(crs, b, PT, s, degree, foci, sample_iters, gpu_iters):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import orc
+
+    alpha = orc.chebyshev(degree, np.asarray(foci))
+    Ao = orc.OracleMatrix(orc.CRS, crs.n, crs=crs, which_precon=1, degree=degree, alpha=alpha)
+    t0 = time.perf_counter()
+    orc.idrs(Ao, b, s, tolerance=1e-8, maximum_iterations=sample_iters, PT_raw=PT)
+    dt = time.perf_counter() - t0
+    # the sample holds setup + sample_iters iterations + the final residual;
+    # one operator application costs about one iteration
+    per_iter = dt / (sample_iters + 2)
+    est_ms = per_iter * (gpu_iters + 2) * 1e3
+    return {
+        "value": round(est_ms, 1),
+        "unit": "ms (time-to-solution, extrapolated)",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle (scalar C restatement) IDR({s})+Chebyshev({degree}) on the same {crs.n}-row system, "
+                  f"{sample_iters} iterations in {dt:.1f} s, scaled to the GPU's {gpu_iters} iterations",
+    }
+
+
+def read_traffic():
+    """HBM bytes per SpMV launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), from the committed profile summary, or None."""
+    path = os.path.join(ROOT, "profiles", "spmv_crs_200cube_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip().split()[0])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=200)
+    ap.add_argument("--s", type=int, default=4)
+    ap.add_argument("--degree", type=int, default=8)
+    ap.add_argument("--format", choices=["crs", "dia"], default="crs")
+    ap.add_argument("--cpu-sample-iters", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
+              "described in DESIGN.md and not measured.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import idrs_amd
+    from idrs_amd import problems as P
+
+    t0 = time.time()
+    m, s, d = args.grid, args.s, args.degree
+    foci = (0.15, 1.85)
+    crs, dia = P.convection_diffusion_3d(m, 50.0, reaction=1.0)
+    n, nnz = crs.n, crs.nnz
+    rng = np.random.default_rng(0)
+    b_h = rng.uniform(-1.0, 1.0, n)
+    PT_h = P.gaussian_shadow(n, s, seed=0)
+    print(f"# system built: n={n} nnz={nnz} ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    if args.format == "crs":
+        K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    else:
+        K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
+    del dia
+    A = idrs_amd.Matrix(K)
+    A.set_precon(1)
+    idrs_amd.real_preconditioner(A, 2, d, foci=foci)
+    dev = torch.device("cuda:0")
+    b = torch.from_numpy(b_h).to(dev)
+    PT = torch.from_numpy(np.ascontiguousarray(PT_h)).to(dev)
+    x = torch.empty_like(b)
+    torch.cuda.synchronize()
+    print(f"# uploaded ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    def step(timed):
+        return idrs_amd.idrs(A, b, s, tolerance=1e-8, maximum_iterations=20000, PT_raw=PT, out=x,
+                             time_spmv=timed)
+
+    for i in range(args.warmup):
+        r = step(False)
+        print(f"# warmup {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} "
+              f"solve={r.solve_ms:.1f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    results = []
+    for i in range(args.steps):
+        r = step(True)
+        results.append(r)
+        print(f"# step {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} solve={r.solve_ms:.1f} ms "
+              f"spmv={r.spmv_ms:.1f} ms / {r.spmv_launches} launches", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall * 1e3 / args.steps
+    last = results[-1]
+    assert last.flag == 0, f"solve did not converge: flag {last.flag}"
+
+    # roofline of the dominant kernel (CRS SpMV) from the timed launches
+    tot_bytes, launches = solve_spmv_bytes(n, nnz, s, d, last.iterations, s)
+    spmv_ms = sum(r.spmv_ms for r in results) / len(results)
+    n_launch = results[-1].spmv_launches
+    if args.format == "dia":
+        tot_bytes = tot_bytes - launches * (12 * nnz + 4 * (n + 1)) + launches * 8 * 7 * n
+    achieved = tot_bytes / (spmv_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    per_launch = tot_bytes / max(n_launch, 1)
+    # whole-solve algorithmic bandwidth (model of all kernels)
+    solve_bytes = last.iterations * iteration_bytes(n, nnz, s, d)
+    solve_gbs = solve_bytes / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline sample ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(crs, b_h, PT_h, s, d, foci, args.cpu_sample_iters, last.iterations)
+
+    out = {
+        "metric": "time-to-solution (ms) at rel. residual 1e-8 + SpMV/vector HBM GB/s vs peak",
+        "value": round(ms_per_step, 2),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{m}^3 convection-diffusion-reaction (Pe 50, reaction 1), {args.format.upper()} FP64, "
+                        f"IDR({s}) + Chebyshev({d}) with diagonal scaling, tol 1e-8",
+            "n": n, "nnz": nnz, "s": s, "degree": d, "format": args.format,
+            "iterations": last.iterations, "relres": last.relres,
+            "solve_gbs_model": round(solve_gbs, 1),
+            "solve_frac_hbm_model": round(solve_gbs / HBM_PEAK_GBS, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": f"k_spmv_{args.format}",
+            "bytes_per_launch": round(per_launch),
+            "avg_launch_us": round(spmv_ms * 1e3 / max(n_launch, 1), 2),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,409 @@
+// extern "C" entry points declared in include/idrs_capi.h.
+#include "idrs_capi.h"
+
+#include "idrs.hpp"
+
+#include <cstdarg>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+struct idrs_user_matrix_s {
+    idrs::DevMatrix *M;
+};
+struct idrs_matrix_s {
+    idrs::Operator op;
+};
+
+namespace idrs {
+
+static thread_local std::string g_err;
+
+void set_last_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+template <class F>
+static int guarded(F &&f) {
+    try {
+        g_err.clear();
+        f();
+        return IDRS_OK;
+    } catch (const HipError &) {
+        return IDRS_ERR_HIP;
+    } catch (const std::invalid_argument &e) {
+        g_err = e.what();
+        return IDRS_ERR_ARG;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return IDRS_ERR_INTERNAL;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return IDRS_ERR_INTERNAL;
+    }
+}
+
+// device copies of host vectors for the host-pointer entry points
+template <class V>
+struct DVec {
+    V *p = nullptr;
+    size_t n = 0;
+    DVec() = default;
+    explicit DVec(size_t cnt) : n(cnt) { HIP_TRY(hipMalloc(&p, sizeof(V) * (cnt ? cnt : 1))); }
+    DVec(const V *h, size_t cnt, hipStream_t st) : DVec(cnt) {
+        if (h && cnt) HIP_TRY(hipMemcpyAsync(p, h, sizeof(V) * cnt, hipMemcpyHostToDevice, st));
+    }
+    ~DVec() { (void)hipFree(p); }
+    DVec(const DVec &) = delete;
+    DVec &operator=(const DVec &) = delete;
+    DVec &operator=(DVec &&o) noexcept {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        return *this;
+    }
+    void download(V *h, hipStream_t st) const {
+        HIP_TRY(hipMemcpyAsync(h, p, sizeof(V) * n, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+};
+
+static void need(bool c, const char *msg) {
+    if (!c) throw std::invalid_argument(msg);
+}
+
+template <class T>
+static int idrs_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                      idrs_stats_t *stats) {
+    return guarded([&] {
+        need(A && b && x, "null argument");
+        hipStream_t st = lib_stream();
+        idrs_opts_t o{};
+        if (opts) o = *opts;
+        SolveOpts<T> so;
+        so.has_m1 = o.has_M1 != 0;
+        so.M1 = (o.has_M1 && o.M1) ? &o.M1->op : nullptr;
+        if (o.has_tolerance) so.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            so.has_maxit = true;
+            so.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) so.variant = o.variant;
+        so.omega = (const T *)o.omega;
+        so.n_omega = o.omega ? o.n_omega : 0;
+        so.resvec = o.resvec;
+        so.resvec_len = o.resvec_len;
+        so.p_seed = o.p_seed ? o.p_seed : 1;
+        const int prow = (so.variant == 1) ? s : 1;
+        SolveStats ss;
+        SpmvTimer &tm = spmv_timer();
+        tm.enabled = o.time_spmv != 0;
+        if (o.device_pointers) {
+            so.x0 = (const T *)o.x0;
+            so.U0 = (const T *)o.U0;
+            so.PT_raw = (const T *)o.PT_raw;
+            idrs_solve<T>(A->op, n, s, (const T *)b, (T *)x, so, ss, st);
+        } else {
+            DVec<T> db((const T *)b, n, st), dx(n);
+            DVec<T> dx0, dU0, dP;
+            if (o.x0) {
+                dx0 = DVec<T>((const T *)o.x0, n, st);
+                so.x0 = dx0.p;
+            }
+            if (o.U0) {
+                dU0 = DVec<T>((const T *)o.U0, (size_t)n * s, st);
+                so.U0 = dU0.p;
+            }
+            if (o.PT_raw) {
+                dP = DVec<T>((const T *)o.PT_raw, (size_t)n * prow, st);
+                so.PT_raw = dP.p;
+            }
+            idrs_solve<T>(A->op, n, s, db.p, dx.p, so, ss, st);
+            dx.download((T *)x, st);
+        }
+        tm.enabled = false;
+        if (stats) {
+            stats->flag = ss.flag;
+            stats->iterations = ss.iterations;
+            stats->relres = ss.relres;
+            stats->solve_ms = ss.solve_ms;
+            stats->spmv_ms = ss.spmv_ms;
+            stats->spmv_launches = ss.spmv_launches;
+            stats->reserved = 0;
+        }
+    });
+}
+
+template <class T>
+static int matvec_entry(idrs_matrix_t A, const double *v, double *w, int what) {
+    return guarded([&] {
+        need(A && v && w, "null argument");
+        hipStream_t st = lib_stream();
+        long n = A->op.K->nrows;
+        DVec<T> dv((const T *)v, n, st), dw(n), w1(n), w2(n), w3(n);
+        EpiArgs<T> none{};
+        if (what == 0) op_apply<T>(A->op, dv.p, dw.p, w1.p, w2.p, EPI_NONE, none, nullptr, st);
+        else if (what == 1) op_precon<T>(A->op, dv.p, dw.p, st);
+        else op_scaleback<T>(A->op, dv.p, dw.p, w1.p, w2.p, w3.p, st);
+        dw.download((T *)w, st);
+    });
+}
+
+template <class T>
+static int user_mv_entry(idrs_user_matrix_t K, const double *v, double *w) {
+    return guarded([&] {
+        need(K && v && w, "null argument");
+        hipStream_t st = lib_stream();
+        DVec<T> dv((const T *)v, K->M->ncols, st), dw(K->M->nrows);
+        user_spmv<T>(*K->M, dv.p, dw.p, st);
+        dw.download((T *)w, st);
+    });
+}
+
+template <class T>
+static int make_p_entry(int n, int s, int method, const double *PT_raw, const double *r, uint64_t seed, double *PT) {
+    return guarded([&] {
+        need(n >= 1 && s >= 1 && s <= SMAX && PT, "bad arguments");
+        need(method >= 1 && method <= 4, "method must be 1..4");
+        need(method != 2 || r, "method 2 needs r");
+        hipStream_t st = lib_stream();
+        int rows = method == 1 ? s : 1;
+        DVec<T> draw, dr, dPT((size_t)n * rows);
+        if (PT_raw) draw = DVec<T>((const T *)PT_raw, (size_t)n * rows, st);
+        if (r) dr = DVec<T>((const T *)r, n, st);
+        make_p_device<T>(n, s, method, PT_raw ? draw.p : nullptr, dr.p, dPT.p, seed ? seed : 1, st);
+        dPT.download((T *)PT, st);
+    });
+}
+
+static ValKind kind_of(const void *vals, bool cplx) {
+    if (!vals) return ValKind::Pattern;
+    return cplx ? ValKind::Complex : ValKind::Real;
+}
+
+}  // namespace idrs
+
+using namespace idrs;
+
+extern "C" {
+
+const char *idrs_last_error(void) { return g_err.c_str(); }
+int idrs_version(void) { return 100; }
+
+int idrs_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+#define MAKE_UM(expr)                                             \
+    guarded([&] {                                                 \
+        need(A != nullptr, "null output handle");                 \
+        *A = nullptr;                                             \
+        DevMatrix *M = (expr);                                    \
+        *A = new idrs_user_matrix_s{M};                           \
+    })
+
+int idrs_crs_format_f64(int nrows, int ncols, const int *row_ptr, const int *col_ind, const double *rval,
+                        idrs_user_matrix_t *A) {
+    return MAKE_UM((need(row_ptr && (col_ind || nrows == 0), "null index array"),
+                    make_crs(nrows, ncols, row_ptr, col_ind, rval, kind_of(rval, false), 2)));
+}
+int idrs_crs_format_c128(int nrows, int ncols, const int *row_ptr, const int *col_ind, const double *cval,
+                         idrs_user_matrix_t *A) {
+    return MAKE_UM((need(row_ptr && cval, "null array"),
+                    make_crs(nrows, ncols, row_ptr, col_ind, cval, ValKind::Complex, 2)));
+}
+int idrs_coo_format_f64(int nrows, int ncols, int nnz, const int *indx, const int *jndx, const double *rval,
+                        idrs_user_matrix_t *A) {
+    return MAKE_UM((need((indx && jndx) || nnz == 0, "null index array"),
+                    make_coo(nrows, ncols, nnz, indx, jndx, rval, kind_of(rval, false))));
+}
+int idrs_coo_format_c128(int nrows, int ncols, int nnz, const int *indx, const int *jndx, const double *cval,
+                         idrs_user_matrix_t *A) {
+    return MAKE_UM((need(((indx && jndx) || nnz == 0) && cval, "null array"),
+                    make_coo(nrows, ncols, nnz, indx, jndx, cval, ValKind::Complex)));
+}
+int idrs_cds_format_f64(int nrows, int ncols, int half_band, const double *rband, idrs_user_matrix_t *A) {
+    return MAKE_UM(make_cds(nrows, ncols, half_band, rband, kind_of(rband, false)));
+}
+int idrs_cds_format_c128(int nrows, int ncols, int half_band, const double *cband, idrs_user_matrix_t *A) {
+    return MAKE_UM((need(cband != nullptr, "null band"), make_cds(nrows, ncols, half_band, cband, ValKind::Complex)));
+}
+int idrs_dia_format_f64(int nrows, int ncols, int ndiag, const int *offsets, const double *vals,
+                        idrs_user_matrix_t *A) {
+    return MAKE_UM((need(offsets != nullptr, "null offsets"),
+                    make_dia(nrows, ncols, ndiag, offsets, vals, kind_of(vals, false))));
+}
+int idrs_dia_format_c128(int nrows, int ncols, int ndiag, const int *offsets, const double *vals,
+                         idrs_user_matrix_t *A) {
+    return MAKE_UM((need(offsets && vals, "null array"), make_dia(nrows, ncols, ndiag, offsets, vals, ValKind::Complex)));
+}
+#undef MAKE_UM
+
+int idrs_user_matrix_destroy(idrs_user_matrix_t A) {
+    return guarded([&] {
+        if (!A) return;
+        delete A->M;
+        delete A;
+    });
+}
+
+int idrs_user_matrix_info(idrs_user_matrix_t A, int *nrows, int *ncols, int64_t *nnz, int *format) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        if (nrows) *nrows = A->M->nrows;
+        if (ncols) *ncols = A->M->ncols;
+        if (nnz) *nnz = A->M->nnz;
+        if (format) *format = A->M->orig_format;
+    });
+}
+
+int idrs_user_mv_f64(idrs_user_matrix_t K, const double *v, double *w) { return user_mv_entry<double>(K, v, w); }
+int idrs_user_mv_c128(idrs_user_matrix_t K, const double *v, double *w) { return user_mv_entry<c128>(K, v, w); }
+
+int idrs_real_diagonal(idrs_user_matrix_t K, double *D) {
+    return guarded([&] {
+        need(K && D, "null argument");
+        need(K->M->kind != ValKind::Complex, "real_diagonal of a complex matrix");
+        HIP_TRY(hipMemcpy(D, K->M->diag, sizeof(double) * K->M->nrows, hipMemcpyDeviceToHost));
+    });
+}
+int idrs_complex_diagonal(idrs_user_matrix_t K, double *D) {
+    return guarded([&] {
+        need(K && D, "null argument");
+        int n = K->M->nrows;
+        if (K->M->kind == ValKind::Complex) {
+            HIP_TRY(hipMemcpy(D, K->M->diag, sizeof(c128) * n, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<double> r(n);
+            HIP_TRY(hipMemcpy(r.data(), K->M->diag, sizeof(double) * n, hipMemcpyDeviceToHost));
+            for (int i = 0; i < n; i++) {
+                D[2 * i] = r[i];
+                D[2 * i + 1] = 0.0;
+            }
+        }
+    });
+}
+
+int idrs_matrix_create(idrs_user_matrix_t K, idrs_matrix_t *A) {
+    return guarded([&] {
+        need(K && A, "null argument");
+        need(K->M->nrows == K->M->ncols, "system matrix must be square");
+        *A = new idrs_matrix_s;
+        (*A)->op.K = K->M;
+        (*A)->op.alpha.assign(1, std::complex<double>(1.0));
+    });
+}
+int idrs_matrix_destroy(idrs_matrix_t A) {
+    return guarded([&] { delete A; });
+}
+int idrs_matrix_set_precon_f64(idrs_matrix_t A, int which_precon, const double *shift) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        need(which_precon == 0 || which_precon == 1,
+             "which_precon must be 0 (none) or 1 (diagonal); dense/band LU (2, 3) are not offered");
+        std::complex<double> sh = shift ? std::complex<double>(*shift, 0.0) : 0.0;
+        op_set_diag_precon(A->op, which_precon, shift ? &sh : nullptr);
+    });
+}
+int idrs_matrix_set_precon_c128(idrs_matrix_t A, int which_precon, const double *shift) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        need(which_precon == 0 || which_precon == 1,
+             "which_precon must be 0 (none) or 1 (diagonal); dense/band LU (2, 3) are not offered");
+        std::complex<double> sh = shift ? std::complex<double>(shift[0], shift[1]) : 0.0;
+        op_set_diag_precon(A->op, which_precon, shift ? &sh : nullptr);
+    });
+}
+int idrs_matrix_set_seed_f64(idrs_matrix_t A, double seed) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        A->op.seed = std::complex<double>(seed, 0.0);
+    });
+}
+int idrs_matrix_set_seed_c128(idrs_matrix_t A, const double *seed) {
+    return guarded([&] {
+        need(A && seed, "null argument");
+        A->op.seed = std::complex<double>(seed[0], seed[1]);
+    });
+}
+
+static int set_poly(idrs_matrix_t A, int preconditioner, int degree, const double *foci, const double *center,
+                    const double *H, int ldh, int ncol, bool cplx) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        need(preconditioner >= 0 && preconditioner <= 3, "preconditioner must be 0..3");
+        std::complex<double> f[2], c;
+        std::vector<std::complex<double>> h;
+        if (foci) {
+            if (cplx) {
+                f[0] = {foci[0], foci[1]};
+                f[1] = {foci[2], foci[3]};
+            } else {
+                f[0] = foci[0];
+                f[1] = foci[1];
+            }
+        }
+        if (center) c = cplx ? std::complex<double>(center[0], center[1]) : std::complex<double>(center[0], 0.0);
+        if (H) {
+            need(ldh > 0 && ncol > 0, "bad H dimensions");
+            h.resize((size_t)ldh * ncol);
+            for (size_t q = 0; q < h.size(); q++)
+                h[q] = cplx ? std::complex<double>(H[2 * q], H[2 * q + 1]) : std::complex<double>(H[q], 0.0);
+        }
+        if (preconditioner == 3) need(H != nullptr, "Hessenberg preconditioner needs H");
+        op_set_poly(A->op, preconditioner, preconditioner ? degree : 0, foci ? f : nullptr,
+                    center ? &c : nullptr, H ? h.data() : nullptr, ldh, ncol, cplx);
+    });
+}
+
+int idrs_real_preconditioner(idrs_matrix_t A, int preconditioner, int degree, const double *foci,
+                             const double *center, const double *H, int ldh, int ncol) {
+    return set_poly(A, preconditioner, degree, foci, center, H, ldh, ncol, false);
+}
+int idrs_complex_preconditioner(idrs_matrix_t A, int preconditioner, int degree, const double *foci,
+                                const double *center, const double *H, int ldh, int ncol) {
+    return set_poly(A, preconditioner, degree, foci, center, H, ldh, ncol, true);
+}
+int idrs_matrix_get_alpha(idrs_matrix_t A, double *alpha, int maxlen, int *degree) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        if (degree) *degree = A->op.degree;
+        if (alpha)
+            for (int i = 0; i <= A->op.degree && i < maxlen; i++) {
+                alpha[2 * i] = A->op.alpha[i].real();
+                alpha[2 * i + 1] = A->op.alpha[i].imag();
+            }
+    });
+}
+
+int idrs_matvec_f64(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<double>(A, v, w, 0); }
+int idrs_matvec_c128(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<c128>(A, v, w, 0); }
+int idrs_precon_f64(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<double>(A, v, w, 1); }
+int idrs_precon_c128(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<c128>(A, v, w, 1); }
+int idrs_scaleback_f64(idrs_matrix_t A, const double *y, double *x) { return matvec_entry<double>(A, y, x, 2); }
+int idrs_scaleback_c128(idrs_matrix_t A, const double *y, double *x) { return matvec_entry<c128>(A, y, x, 2); }
+
+int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                  idrs_stats_t *stats) {
+    return idrs_entry<double>(A, n, b, s, opts, x, stats);
+}
+int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                   idrs_stats_t *stats) {
+    return idrs_entry<c128>(A, n, b, s, opts, x, stats);
+}
+
+int idrs_make_p_f64(int n, int s, int method, const double *PT_raw, const double *r, uint64_t seed, double *PT) {
+    return make_p_entry<double>(n, s, method, PT_raw, r, seed, PT);
+}
+int idrs_make_p_c128(int n, int s, int method, const double *PT_raw, const double *r, uint64_t seed, double *PT) {
+    return make_p_entry<c128>(n, s, method, PT_raw, r, seed, PT);
+}
+
+}  // extern "C"

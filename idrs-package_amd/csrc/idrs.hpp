@@ -1,0 +1,49 @@
+// IDR(s) solver (RIDRS/CIDRS) on the device.
+#pragma once
+
+#include "matrix.hpp"
+
+namespace idrs {
+
+constexpr int SMAX = 16;             // largest shadow-space dimension s
+constexpr int MAX_PARTS = 4096;      // upper bound of blocks in any reduction grid
+constexpr int MAX_NV = SMAX + 3;     // values reduced per block
+
+struct SolveStats {
+    int flag = -1;
+    double relres = 1.0;
+    int iterations = 0;
+    double solve_ms = 0.0;       // hipEvent time of the whole solve
+    double spmv_ms = 0.0;        // summed SpMV launch time (when timing enabled)
+    int spmv_launches = 0;
+};
+
+// Solver options; pointers are DEVICE pointers unless noted.
+template <class T>
+struct SolveOpts {
+    bool has_m1 = false;         // present(M1)
+    const Operator *M1 = nullptr;  // v/M1 uses M1's diagonal if it has one
+    double tol = (double)1e-6f;  // reference default is the single-precision literal 1e-6
+    bool has_maxit = false;      // present(maximum_iterations); default min(2n, 1000)
+    int maxit = 0;
+    int variant = 1;
+    const T *x0 = nullptr;       // device
+    const T *U0 = nullptr;       // device, n*s column-major
+    const T *omega = nullptr;    // HOST array of user omegas
+    int n_omega = 0;
+    double *resvec = nullptr;    // HOST output, length >= maxit+1
+    int resvec_len = 0;
+    const T *PT_raw = nullptr;   // device, raw shadow space before orthonormalisation
+    unsigned long long p_seed = 1;   // default stream seed (reference: this_image() = 1)
+};
+
+template <class T>
+int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOpts<T> &o, SolveStats &st,
+               hipStream_t stream);
+
+// Standalone P generation (MAKE_P) for tests: PT(s,n) orthonormalised, device.
+template <class T>
+void make_p_device(long n, int s, int method, const T *PT_raw, const T *r, T *PT, unsigned long long seed,
+                   hipStream_t stream);
+
+}  // namespace idrs

@@ -1,0 +1,894 @@
+// Sparse formats, SpMV kernels and the preconditioned operator.
+//
+// Reference behaviour restated here (paths relative to the reference's IDRS/):
+//   CRS SpMV      MODULES/PP_MATRIX/user_module.f90:376-450 (rcrs_mv), :285-374 (ccrs_mv)
+//   COO SpMV      user_module.f90:525-585 / :452-523
+//   CDS SpMV      user_module.f90:676-827
+//   diagonal      user_module.f90:960-1161, REAL_PRECON(1) :1327-1381
+//   operator(*)   MODULES/PP_MATRIX/matrix_module.f90:135-264 (rmatvec, real_mv)
+//   polpre        matrix_module.f90:266-304;  operator(/) :306-400
+//   chebyshev / neumann / compute_alpha  matrix_module.f90:851-1057
+//
+// Design: one thread per row; each 256-row block stages its contiguous
+// slice of values and column indices in LDS with coalesced loads, then each
+// thread runs the row's fused multiply-add chain in stored order (the
+// reference's summation order).  Consecutive rows make the per-position x
+// gathers of banded/stencil matrices contiguous across the wavefront.
+// Workgroups take contiguous chunks of row blocks and are remapped so that
+// each XCD streams one contiguous slab of rows (L2 reuse of x).
+#include "matrix.hpp"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace idrs {
+
+hipStream_t lib_stream() {
+    static hipStream_t s = [] {
+        hipStream_t t;
+        HIP_TRY(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        return t;
+    }();
+    return s;
+}
+
+DevMatrix::~DevMatrix() {
+    (void)hipFree(row_ptr);
+    (void)hipFree(col);
+    (void)hipFree(offsets);
+    (void)hipFree(val);
+    (void)hipFree(val32);
+    (void)hipFree(diag);
+}
+
+Operator::~Operator() {
+    (void)hipFree(pdiag);
+    (void)hipFree(sval);
+    (void)hipFree(sval32);
+}
+
+SpmvTimer &spmv_timer() {
+    static SpmvTimer t;
+    return t;
+}
+SpmvTimer::~SpmvTimer() {}
+double SpmvTimer::total_ms() const {
+    double tot = 0.0;
+    for (int i = 0; i + 1 < used; i += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) tot += ms;
+    }
+    return tot;
+}
+
+static void timer_mark(hipStream_t st) {
+    SpmvTimer &t = spmv_timer();
+    if (!t.enabled) return;
+    if (t.used >= (int)t.ev.size()) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        t.ev.push_back(e);
+    }
+    HIP_TRY(hipEventRecord(t.ev[t.used++], st));
+}
+
+int vec_grid(long n) {
+    long b = (n + 255) / 256;
+    if (b > 2048) b = 2048;
+    if (b < 1) b = 1;
+    if (b >= 64) b = (b / NUM_XCD) * NUM_XCD;
+    return (int)b;
+}
+
+int spmv_grid(const DevMatrix &K) {
+    long units = (K.fmt == DevFmt::CRS) ? K.nrb : (K.nrows + SPMV_THREADS - 1) / SPMV_THREADS;
+    long b = units;
+    if (b > 4096) b = 4096;
+    if (b < 1) b = 1;
+    if (b >= 64) b = (b / NUM_XCD) * NUM_XCD;
+    return (int)b;
+}
+
+template <class V>
+static V *dev_upload(const V *h, size_t count) {
+    V *d = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc(&d, sizeof(V) * count));
+    if (h) HIP_TRY(hipMemcpy(d, h, sizeof(V) * count, hipMemcpyHostToDevice));
+    return d;
+}
+
+// ---------------------------------------------------------------------------
+// uploads
+// ---------------------------------------------------------------------------
+static void finish_crs(DevMatrix *M, const std::vector<int> &rp0, const std::vector<int> &col0,
+                       const std::vector<double> &rv, const std::vector<c128> &cv) {
+    long nnz = M->nnz;
+    M->row_ptr = dev_upload(rp0.data(), rp0.size());
+    M->col = dev_upload(col0.data(), (size_t)nnz);
+    if (M->kind == ValKind::Complex) {
+        M->val = dev_upload(cv.data(), (size_t)nnz);
+    } else {
+        M->val = dev_upload(rv.data(), (size_t)nnz);
+    }
+    // rows per block: 256, fewer if the average row does not fit the LDS slab
+    double avg = M->nrows > 0 ? (double)nnz / M->nrows : 1.0;
+    int rpb = SPMV_THREADS;
+    while (rpb > 64 && avg * rpb * 1.15 > LDS_NNZ) rpb /= 2;
+    M->rpb = rpb;
+    M->nrb = (M->nrows + rpb - 1) / rpb;
+    // diagonal (real_diagonal / complex_diagonal), zeros -> 1
+    if (M->kind == ValKind::Complex) {
+        std::vector<c128> d(M->nrows, c128(0.0));
+        for (int i = 0; i < M->nrows; i++)
+            for (int j = rp0[i]; j < rp0[i + 1]; j++)
+                if (col0[j] == i) d[i] = d[i] + cv[j];
+        for (auto &v : d)
+            if (v.re == 0.0 && v.im == 0.0) v = c128(1.0);
+        M->diag = dev_upload(d.data(), d.size());
+    } else {
+        std::vector<double> d(M->nrows, 0.0);
+        for (int i = 0; i < M->nrows; i++)
+            for (int j = rp0[i]; j < rp0[i + 1]; j++)
+                if (col0[j] == i) d[i] = d[i] + rv[j];
+        for (auto &v : d)
+            if (v == 0.0) v = 1.0;
+        M->diag = dev_upload(d.data(), d.size());
+    }
+}
+
+DevMatrix *make_crs(int nrows, int ncols, const int *row_ptr1, const int *col_ind1, const void *val,
+                    ValKind kind, int orig_format) {
+    if (nrows < 0 || ncols < 0) throw std::invalid_argument("negative dimension");
+    if (row_ptr1[0] != 1) throw std::invalid_argument("row_ptr(1) must be 1 (1-based CRS)");
+    long nnz = (long)row_ptr1[nrows] - 1;
+    if (nnz < 0) throw std::invalid_argument("row_ptr(nrows+1) < 1");
+    std::unique_ptr<DevMatrix> M(new DevMatrix);
+    M->orig_format = orig_format;
+    M->fmt = DevFmt::CRS;
+    M->kind = kind == ValKind::Pattern ? ValKind::Real : kind;
+    M->nrows = nrows;
+    M->ncols = ncols;
+    M->nnz = nnz;
+    std::vector<int> rp0(nrows + 1), col0((size_t)nnz);
+    for (int i = 0; i <= nrows; i++) {
+        rp0[i] = row_ptr1[i] - 1;
+        if (i > 0 && rp0[i] < rp0[i - 1]) throw std::invalid_argument("row_ptr not non-decreasing");
+    }
+    for (long j = 0; j < nnz; j++) {
+        int c = col_ind1[j] - 1;
+        if (c < 0 || c >= ncols) throw std::invalid_argument("col_ind out of range 1..ncols");
+        col0[j] = c;
+    }
+    std::vector<double> rv;
+    std::vector<c128> cv;
+    if (M->kind == ValKind::Complex) {
+        const c128 *v = (const c128 *)val;
+        cv.assign(v, v + nnz);
+    } else if (kind == ValKind::Pattern || !val) {
+        rv.assign((size_t)nnz, 1.0);
+    } else {
+        const double *v = (const double *)val;
+        rv.assign(v, v + nnz);
+    }
+    finish_crs(M.get(), rp0, col0, rv, cv);
+    return M.release();
+}
+
+// COO -> CRS by a stable sort on the row index: within each row the entries
+// keep their list order, so every row is summed in exactly the order of the
+// reference's sequential scatter loop (user_module.f90:542-558).
+DevMatrix *make_coo(int nrows, int ncols, long nnz, const int *indx1, const int *jndx1, const void *val,
+                    ValKind kind) {
+    if (nrows < 0 || ncols < 0 || nnz < 0) throw std::invalid_argument("negative dimension");
+    std::unique_ptr<DevMatrix> M(new DevMatrix);
+    M->orig_format = 3;
+    M->fmt = DevFmt::CRS;
+    M->kind = kind == ValKind::Pattern ? ValKind::Real : kind;
+    M->nrows = nrows;
+    M->ncols = ncols;
+    M->nnz = nnz;
+    std::vector<int> cnt(nrows + 1, 0);
+    for (long k = 0; k < nnz; k++) {
+        int i = indx1[k] - 1, j = jndx1[k] - 1;
+        if (i < 0 || i >= nrows) throw std::invalid_argument("indx out of range 1..nrows");
+        if (j < 0 || j >= ncols) throw std::invalid_argument("jndx out of range 1..ncols");
+        cnt[i + 1]++;
+    }
+    std::vector<int> rp0(nrows + 1, 0);
+    for (int i = 0; i < nrows; i++) rp0[i + 1] = rp0[i] + cnt[i + 1];
+    std::vector<int> pos(rp0.begin(), rp0.end() - 1), col0((size_t)nnz);
+    std::vector<double> rv;
+    std::vector<c128> cv;
+    if (M->kind == ValKind::Complex) cv.resize((size_t)nnz);
+    else rv.resize((size_t)nnz);
+    for (long k = 0; k < nnz; k++) {
+        int i = indx1[k] - 1;
+        int p = pos[i]++;
+        col0[p] = jndx1[k] - 1;
+        if (M->kind == ValKind::Complex) cv[p] = ((const c128 *)val)[k];
+        else rv[p] = (kind == ValKind::Pattern || !val) ? 1.0 : ((const double *)val)[k];
+    }
+    finish_crs(M.get(), rp0, col0, rv, cv);
+    return M.release();
+}
+
+static DevMatrix *make_dia_impl(int nrows, int ncols, int ndiag, const int *offsets, const void *vals,
+                                ValKind kind, int orig_format, const int *diag_index_in_band) {
+    if (ndiag < 1 || ndiag > MAX_DIAG) throw std::invalid_argument("ndiag out of range 1..1024");
+    std::unique_ptr<DevMatrix> M(new DevMatrix);
+    M->orig_format = orig_format;
+    M->fmt = DevFmt::DIA;
+    M->kind = kind == ValKind::Pattern ? ValKind::Real : kind;
+    M->nrows = nrows;
+    M->ncols = ncols;
+    M->ndiag = ndiag;
+    M->nnz = (long)ndiag * nrows;
+    M->h_offsets.assign(offsets, offsets + ndiag);
+    for (int d = 1; d < ndiag; d++)
+        if (offsets[d] <= offsets[d - 1]) throw std::invalid_argument("DIA offsets must be strictly ascending");
+    M->offsets = dev_upload(offsets, (size_t)ndiag);
+    size_t cnt = (size_t)ndiag * nrows;
+    int d0 = -1;
+    for (int d = 0; d < ndiag; d++)
+        if (offsets[d] == 0) d0 = d;
+    (void)diag_index_in_band;
+    if (M->kind == ValKind::Complex) {
+        M->val = dev_upload((const c128 *)vals, cnt);
+        std::vector<c128> dg(nrows, c128(1.0));
+        if (d0 >= 0)
+            for (int i = 0; i < nrows; i++) {
+                c128 v = ((const c128 *)vals)[(size_t)d0 * nrows + i];
+                dg[i] = (v.re == 0.0 && v.im == 0.0) ? c128(1.0) : v;
+            }
+        M->diag = dev_upload(dg.data(), dg.size());
+    } else {
+        std::vector<double> tmp;
+        const double *src = (const double *)vals;
+        if (kind == ValKind::Pattern || !vals) {
+            tmp.assign(cnt, 1.0);
+            src = tmp.data();
+        }
+        M->val = dev_upload(src, cnt);
+        std::vector<double> dg(nrows, 1.0);
+        if (d0 >= 0)
+            for (int i = 0; i < nrows; i++) {
+                double v = src[(size_t)d0 * nrows + i];
+                dg[i] = v == 0.0 ? 1.0 : v;
+            }
+        M->diag = dev_upload(dg.data(), dg.size());
+    }
+    return M.release();
+}
+
+// cds_format: dense band (nrows, -hb:hb) column-major = DIA with all 2hb+1
+// offsets in ascending order, the reference's diagonal loop order
+// (user_module.f90:727-733).
+DevMatrix *make_cds(int nrows, int ncols, int half_band, const void *band, ValKind kind) {
+    if (half_band < 0 || 2 * half_band + 1 > MAX_DIAG) throw std::invalid_argument("half_band out of range");
+    std::vector<int> off(2 * half_band + 1);
+    for (int d = 0; d < 2 * half_band + 1; d++) off[d] = d - half_band;
+    return make_dia_impl(nrows, ncols, 2 * half_band + 1, off.data(), band, kind, 4, nullptr);
+}
+
+DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const void *vals, ValKind kind) {
+    return make_dia_impl(nrows, ncols, ndiag, offsets, vals, kind, 5, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// elementwise helpers for the value/vector type combinations
+// ---------------------------------------------------------------------------
+__device__ inline double divv(double x, double d) { return x / d; }
+__device__ inline float divv(float x, float d) { return x / d; }
+template <class R>
+__device__ inline cplx<R> divv(cplx<R> x, R d) { return {x.re / d, x.im / d}; }
+template <class R>
+__device__ inline cplx<R> divv(cplx<R> x, cplx<R> d) { return x / d; }
+
+// ---------------------------------------------------------------------------
+// SpMV epilogues
+// ---------------------------------------------------------------------------
+template <class T, int EPI, int NV>
+__device__ inline void epi_accum(T (&e)[NV], const EpiArgs<T> &E, long q, T yv) {
+    using TT = Tr<T>;
+    if constexpr (EPI == EPI_PDOT) {
+        const T *p = E.P + q * E.prow;
+#pragma unroll
+        for (int i = 0; i < NV; i++)
+            if (i < E.prow) {
+                T pv = p[i];
+                if (E.conjP) pv = TT::conj(pv);
+                e[i] = TT::fma(pv, yv, e[i]);
+            }
+    } else if constexpr (EPI == EPI_OMEGA) {
+        T rr = E.r[q];
+        e[0] = TT::fma(TT::conj(rr), rr, e[0]);
+        e[1] = TT::fma(TT::conj(yv), yv, e[1]);
+        e[2] = TT::fma(TT::conj(yv), rr, e[2]);
+    } else if constexpr (EPI == EPI_NORM) {
+        e[0] = e[0] + T(TT::abs2(yv));
+    }
+}
+
+template <class T, int EPI, int NV>
+__device__ inline void epi_store(T (&e)[NV], const EpiArgs<T> &E, T *lds) {
+    if constexpr (EPI != EPI_NONE) {
+        block_sum<T, NV>(e, lds);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int i = 0; i < NV; i++) E.partials[(long)i * E.nparts + blockIdx.x] = e[i];
+    }
+}
+
+template <class VT, class T>
+struct SpmvArgs {
+    int nrows, ncols;
+    const int *rp, *col;
+    const VT *val;
+    int rpb, nrb;
+    int ndiag;
+    const int *offsets;
+    const T *x;
+    T xs;
+    const T *z;
+    T a;
+    int has_seed;      // seed term: y -= seed * x/sdiag (sdiag nullptr: x)
+    const VT *sdiag;
+    T seed;
+    T *y;
+};
+
+template <class VT, class T>
+__device__ inline T finish_row(const SpmvArgs<VT, T> &A, long row, T acc) {
+    if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
+    T yv = A.xs * acc;
+    if (A.z) yv = Tr<T>::fma(A.a, A.z[row], yv);
+    A.y[row] = yv;
+    return yv;
+}
+
+template <class VT, class T, int EPI, int NV>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, EpiArgs<T> E, const int *info) {
+    if (info && *info >= 0) return;
+    __shared__ VT s_val[LDS_NNZ];
+    __shared__ int s_col[LDS_NNZ];
+    __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
+    T e[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) e[i] = T(0);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const long rb0 = (long)A.nrb * lb / gridDim.x, rb1 = (long)A.nrb * (lb + 1) / gridDim.x;
+    const T *__restrict__ x = A.x;
+    for (long rb = rb0; rb < rb1; rb++) {
+        const long r0 = rb * A.rpb;
+        const long r1 = min(r0 + (long)A.rpb, (long)A.nrows);
+        const int p0 = A.rp[r0], p1 = A.rp[r1];
+        const int nz = p1 - p0;
+        const long row = r0 + threadIdx.x;
+        const bool active = (threadIdx.x < A.rpb) && row < r1;
+        T acc = T(0);
+        if (nz <= LDS_NNZ) {
+            for (int q = threadIdx.x; q < nz; q += SPMV_THREADS) {
+                s_val[q] = A.val[p0 + q];
+                s_col[q] = A.col[p0 + q];
+            }
+            __syncthreads();
+            if (active) {
+                const int a = A.rp[row] - p0, b = A.rp[row + 1] - p0;
+                for (int j = a; j < b; j++) acc = fmav(s_val[j], x[s_col[j]], acc);
+            }
+            __syncthreads();
+        } else if (active) {
+            const int a = A.rp[row], b = A.rp[row + 1];
+            for (int j = a; j < b; j++) acc = fmav(A.val[j], x[A.col[j]], acc);
+        }
+        if (active) {
+            T yv = finish_row(A, row, acc);
+            epi_accum<T, EPI, NV>(e, E, row, yv);
+        }
+    }
+    epi_store<T, EPI, NV>(e, E, s_red);
+}
+
+template <class VT, class T, int EPI, int NV>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, EpiArgs<T> E, const int *info) {
+    if (info && *info >= 0) return;
+    __shared__ int s_off[MAX_DIAG];
+    __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
+    for (int d = threadIdx.x; d < A.ndiag; d += SPMV_THREADS) s_off[d] = A.offsets[d];
+    __syncthreads();
+    T e[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) e[i] = T(0);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const long nchunk = (A.nrows + SPMV_THREADS - 1) / SPMV_THREADS;
+    const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
+    const T *__restrict__ x = A.x;
+    const long n = A.nrows;
+    for (long c = c0; c < c1; c++) {
+        const long row = c * SPMV_THREADS + threadIdx.x;
+        if (row >= n) continue;
+        T acc = T(0);
+        for (int d = 0; d < A.ndiag; d++) {
+            const long cc = row + s_off[d];
+            if (cc >= 0 && cc < A.ncols) acc = fmav(A.val[(long)d * n + row], x[cc], acc);
+        }
+        T yv = finish_row(A, row, acc);
+        epi_accum<T, EPI, NV>(e, E, row, yv);
+    }
+    epi_store<T, EPI, NV>(e, E, s_red);
+}
+
+template <class VT, class T, int EPI, int NV>
+static void launch_spmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T> E, const int *info,
+                        hipStream_t st) {
+    int grid = spmv_grid(K);
+    E.nparts = grid;
+    if (K.fmt == DevFmt::CRS)
+        hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV>), dim3(grid), dim3(SPMV_THREADS), 0, st, A, E, info);
+    else
+        hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV>), dim3(grid), dim3(SPMV_THREADS), 0, st, A, E, info);
+    LAUNCH_CHECK();
+}
+
+template <class VT, class T>
+static void dispatch_epi(const DevMatrix &K, const SpmvArgs<VT, T> &A, const EpiArgs<T> &E, int epi,
+                         const int *info, hipStream_t st) {
+    switch (epi) {
+    case EPI_NONE: launch_spmv<VT, T, EPI_NONE, 1>(K, A, E, info, st); break;
+    case EPI_NORM: launch_spmv<VT, T, EPI_NORM, 1>(K, A, E, info, st); break;
+    case EPI_OMEGA: launch_spmv<VT, T, EPI_OMEGA, 3>(K, A, E, info, st); break;
+    case EPI_PDOT:
+        if (E.prow <= 1) launch_spmv<VT, T, EPI_PDOT, 1>(K, A, E, info, st);
+        else if (E.prow <= 2) launch_spmv<VT, T, EPI_PDOT, 2>(K, A, E, info, st);
+        else if (E.prow <= 4) launch_spmv<VT, T, EPI_PDOT, 4>(K, A, E, info, st);
+        else if (E.prow <= 8) launch_spmv<VT, T, EPI_PDOT, 8>(K, A, E, info, st);
+        else launch_spmv<VT, T, EPI_PDOT, 16>(K, A, E, info, st);
+        break;
+    default: throw std::invalid_argument("bad epilogue");
+    }
+}
+
+template <class T> static T to_T(std::complex<double> c);
+template <> double to_T<double>(std::complex<double> c) { return c.real(); }
+template <> float to_T<float>(std::complex<double> c) { return (float)c.real(); }
+template <> c128 to_T<c128>(std::complex<double> c) { return c128(c.real(), c.imag()); }
+template <> c64 to_T<c64>(std::complex<double> c) { return c64((float)c.real(), (float)c.imag()); }
+
+template <class VT, class T>
+static SpmvArgs<VT, T> base_args(const DevMatrix &K, const VT *val) {
+    SpmvArgs<VT, T> A{};
+    A.nrows = K.nrows;
+    A.ncols = K.ncols;
+    A.rp = K.row_ptr;
+    A.col = K.col;
+    A.val = val;
+    A.rpb = K.rpb;
+    A.nrb = K.nrb;
+    A.ndiag = K.ndiag;
+    A.offsets = K.offsets;
+    return A;
+}
+
+template <class T>
+void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int epi, const EpiArgs<T> &E,
+             const int *info, hipStream_t st) {
+    const DevMatrix &K = *op.K;
+    const bool cv = op.values_complex();
+    const bool has_seed = op.seed != std::complex<double>(0.0, 0.0);
+    using R = typename Tr<T>::real;
+    constexpr bool single = std::is_same<R, float>::value;
+    if (has_seed && single) throw std::invalid_argument("seed shift in single precision is not supported");
+    if (has_seed && !Tr<T>::is_complex && op.seed.imag() != 0.0)
+        throw std::invalid_argument("complex seed on a real system");
+    if (cv) {
+        if constexpr (Tr<T>::is_complex && !single) {
+            auto A = base_args<c128, T>(K, (const c128 *)op.values());
+            A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
+            A.has_seed = has_seed;
+            A.sdiag = op.which_precon == 1 ? (const c128 *)op.pdiag : nullptr;
+            A.seed = to_T<T>(op.seed);
+            dispatch_epi<c128, T>(K, A, E, epi, info, st);
+        } else {
+            throw std::invalid_argument("complex matrix values need complex128 vectors");
+        }
+    } else {
+        const R *vals = single ? (const R *)(op.sval ? op.sval32 : K.val32) : (const R *)op.values();
+        if (!vals) throw std::invalid_argument("single-precision values not prepared");
+        auto A = base_args<R, T>(K, vals);
+        A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
+        A.has_seed = has_seed;
+        A.sdiag = op.which_precon == 1 ? (const R *)op.pdiag : nullptr;
+        A.seed = to_T<T>(op.seed);
+        dispatch_epi<R, T>(K, A, E, epi, info, st);
+    }
+}
+
+// polynomial coefficient as the vector type
+template <class T>
+static T coef(const Operator &op, int i) {
+    return to_T<T>(op.alpha[i]);
+}
+
+template <class T>
+void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const EpiArgs<T> &E,
+              const int *info, hipStream_t st) {
+    const int d = op.degree;
+    EpiArgs<T> none{};
+    if (d == 0) {
+        timer_mark(st);
+        op_spmv<T>(op, v, T(1), nullptr, T(0), w, epi, E, info, st);
+        timer_mark(st);
+        return;
+    }
+    // Horner: w = alpha_d v; for i = d-1..0: w = K'(w) + alpha_i v; then K'(w)
+    T *cur = w1, *nxt = w2;
+    timer_mark(st);
+    op_spmv<T>(op, v, coef<T>(op, d), v, coef<T>(op, d - 1), cur, EPI_NONE, none, info, st);
+    timer_mark(st);
+    for (int i = d - 2; i >= 0; i--) {
+        timer_mark(st);
+        op_spmv<T>(op, cur, T(1), v, coef<T>(op, i), nxt, EPI_NONE, none, info, st);
+        timer_mark(st);
+        std::swap(cur, nxt);
+    }
+    timer_mark(st);
+    op_spmv<T>(op, cur, T(1), nullptr, T(0), w, epi, E, info, st);
+    timer_mark(st);
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels: w = v / D  (rprecon), w = c*v
+// ---------------------------------------------------------------------------
+template <class T, class DT>
+__global__ void k_divide(long n, const T *__restrict__ v, const DT *__restrict__ d, T *__restrict__ w) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        w[i] = divv(v[i], d[i]);
+}
+template <class T>
+__global__ void k_scale_copy(long n, T c, const T *__restrict__ v, T *__restrict__ w) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        w[i] = c * v[i];
+}
+
+template <class T>
+void op_precon(const Operator &op, const T *v, T *w, hipStream_t st) {
+    long n = op.K->nrows;
+    int g = vec_grid(n);
+    if (op.which_precon == 1) {
+        if constexpr (Tr<T>::is_complex) {
+            using R = typename Tr<T>::real;
+            if (op.pdiag_kind == ValKind::Complex) {
+                if constexpr (std::is_same<R, double>::value)
+                    hipLaunchKernelGGL((k_divide<T, c128>), dim3(g), dim3(256), 0, st, n, v, (const c128 *)op.pdiag, w);
+                else
+                    throw std::invalid_argument("complex diagonal in single precision");
+            } else {
+                if constexpr (std::is_same<R, double>::value)
+                    hipLaunchKernelGGL((k_divide<T, double>), dim3(g), dim3(256), 0, st, n, v, (const double *)op.pdiag, w);
+                else
+                    throw std::invalid_argument("single-precision precon");
+            }
+        } else {
+            if (op.pdiag_kind == ValKind::Complex) throw std::invalid_argument("complex diagonal on real vector");
+            if constexpr (std::is_same<T, double>::value)
+                hipLaunchKernelGGL((k_divide<double, double>), dim3(g), dim3(256), 0, st, n, v, (const double *)op.pdiag, w);
+            else
+                throw std::invalid_argument("single-precision precon");
+        }
+    } else {
+        if (v != w) HIP_TRY(hipMemcpyAsync(w, v, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+        return;
+    }
+    LAUNCH_CHECK();
+}
+
+template <class T>
+void op_scaleback(const Operator &op, const T *y, T *x, T *w1, T *w2, T *w3, hipStream_t st) {
+    const int d = op.degree;
+    long n = op.K->nrows;
+    EpiArgs<T> none{};
+    const T *p = y;
+    if (d > 0) {
+        // polpre: w = alpha_d y; for i = d-1..0: w = K'(w) + alpha_i y
+        int g = vec_grid(n);
+        T *cur = w1, *nxt = w2;
+        hipLaunchKernelGGL((k_scale_copy<T>), dim3(g), dim3(256), 0, st, n, coef<T>(op, d), y, cur);
+        LAUNCH_CHECK();
+        for (int i = d - 1; i >= 0; i--) {
+            op_spmv<T>(op, cur, T(1), y, coef<T>(op, i), nxt, EPI_NONE, none, nullptr, st);
+            std::swap(cur, nxt);
+        }
+        p = cur;
+    }
+    (void)w3;
+    op_precon<T>(op, p, x, st);
+}
+
+template <class T>
+void user_spmv(const DevMatrix &K, const T *v, T *w, hipStream_t st) {
+    Operator tmp;
+    tmp.K = const_cast<DevMatrix *>(&K);
+    EpiArgs<T> none{};
+    op_spmv<T>(tmp, v, T(1), nullptr, T(0), w, EPI_NONE, none, nullptr, st);
+    tmp.K = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// operator construction
+// ---------------------------------------------------------------------------
+template <class VT, class DT>
+__global__ void k_scale_cols_crs(long nnz, const int *__restrict__ col, const VT *__restrict__ val,
+                                 const DT *__restrict__ d, VT *__restrict__ out) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < nnz; j += (long)gridDim.x * blockDim.x)
+        out[j] = divv(val[j], d[col[j]]);
+}
+template <class VT, class DT>
+__global__ void k_scale_cols_dia(int nrows, int ncols, int ndiag, const int *__restrict__ off,
+                                 const VT *__restrict__ val, const DT *__restrict__ d, VT *__restrict__ out) {
+    long tot = (long)nrows * ndiag;
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (long)gridDim.x * blockDim.x) {
+        int dd = (int)(q / nrows);
+        long row = q - (long)dd * nrows;
+        long c = row + off[dd];
+        out[q] = (c >= 0 && c < ncols) ? divv(val[q], d[c]) : val[q];
+    }
+}
+template <class T>
+__global__ void k_add_scalar(long n, T *__restrict__ d, T c) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d[i] = d[i] + c;
+}
+template <class VT, class OT>
+__global__ void k_convert(long n, const VT *__restrict__ in, OT *__restrict__ out) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (OT)in[i];
+}
+__global__ void k_real_to_cplx(long n, const double *__restrict__ in, c128 *__restrict__ out) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = c128(in[i], 0.0);
+}
+
+// REAL_PRECON/COMPLEX_PRECON(1, K[, shift]) (user_module.f90:1327-1381):
+// D = diag(K) [- shift] with zeros already replaced by 1 in diag(K).  The
+// values of K D^-1 are precomputed so that real_mv's t = v/D is folded into
+// the product.
+void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shift) {
+    hipStream_t st = lib_stream();
+    DevMatrix &K = *op.K;
+    (void)hipFree(op.pdiag);
+    (void)hipFree(op.sval);
+    (void)hipFree(op.sval32);
+    op.pdiag = op.sval = op.sval32 = nullptr;
+    op.which_precon = which;
+    if (which != 1) return;
+    if (K.nrows != K.ncols) throw std::invalid_argument("diagonal scaling needs a square matrix");
+    long n = K.nrows;
+    int g = vec_grid(n);
+    bool cdiag = (K.kind == ValKind::Complex) || (shift && shift->imag() != 0.0);
+    op.pdiag_kind = cdiag ? ValKind::Complex : ValKind::Real;
+    if (cdiag) {
+        HIP_TRY(hipMalloc(&op.pdiag, sizeof(c128) * (n ? n : 1)));
+        if (K.kind == ValKind::Complex)
+            HIP_TRY(hipMemcpyAsync(op.pdiag, K.diag, sizeof(c128) * n, hipMemcpyDeviceToDevice, st));
+        else {
+            hipLaunchKernelGGL(k_real_to_cplx, dim3(g), dim3(256), 0, st, n, (const double *)K.diag, (c128 *)op.pdiag);
+            LAUNCH_CHECK();
+        }
+        if (shift) {
+            hipLaunchKernelGGL((k_add_scalar<c128>), dim3(g), dim3(256), 0, st, n, (c128 *)op.pdiag,
+                               c128(-shift->real(), -shift->imag()));
+            LAUNCH_CHECK();
+        }
+    } else {
+        HIP_TRY(hipMalloc(&op.pdiag, sizeof(double) * (n ? n : 1)));
+        HIP_TRY(hipMemcpyAsync(op.pdiag, K.diag, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        if (shift) {
+            hipLaunchKernelGGL((k_add_scalar<double>), dim3(g), dim3(256), 0, st, n, (double *)op.pdiag, -shift->real());
+            LAUNCH_CHECK();
+        }
+    }
+    // scaled values
+    long cnt = K.fmt == DevFmt::CRS ? K.nnz : (long)K.ndiag * K.nrows;
+    int gv = vec_grid(cnt);
+    bool cvals = (K.kind == ValKind::Complex) || cdiag;
+    HIP_TRY(hipMalloc(&op.sval, (cvals ? sizeof(c128) : sizeof(double)) * (cnt ? cnt : 1)));
+    void *src = K.val;
+    void *tmpc = nullptr;
+    if (cvals && K.kind != ValKind::Complex) {
+        HIP_TRY(hipMalloc(&tmpc, sizeof(c128) * (cnt ? cnt : 1)));
+        hipLaunchKernelGGL(k_real_to_cplx, dim3(gv), dim3(256), 0, st, cnt, (const double *)K.val, (c128 *)tmpc);
+        LAUNCH_CHECK();
+        src = tmpc;
+    }
+    if (K.fmt == DevFmt::CRS) {
+        if (cvals)
+            hipLaunchKernelGGL((k_scale_cols_crs<c128, c128>), dim3(gv), dim3(256), 0, st, cnt, K.col,
+                               (const c128 *)src, (const c128 *)op.pdiag, (c128 *)op.sval);
+        else
+            hipLaunchKernelGGL((k_scale_cols_crs<double, double>), dim3(gv), dim3(256), 0, st, cnt, K.col,
+                               (const double *)src, (const double *)op.pdiag, (double *)op.sval);
+    } else {
+        if (cvals)
+            hipLaunchKernelGGL((k_scale_cols_dia<c128, c128>), dim3(gv), dim3(256), 0, st, K.nrows, K.ncols,
+                               K.ndiag, K.offsets, (const c128 *)src, (const c128 *)op.pdiag, (c128 *)op.sval);
+        else
+            hipLaunchKernelGGL((k_scale_cols_dia<double, double>), dim3(gv), dim3(256), 0, st, K.nrows, K.ncols,
+                               K.ndiag, K.offsets, (const double *)src, (const double *)op.pdiag, (double *)op.sval);
+    }
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(st));
+    (void)hipFree(tmpc);
+}
+
+void op_make_lowprec(Operator &op) {
+    hipStream_t st = lib_stream();
+    DevMatrix &K = *op.K;
+    if (op.values_complex()) throw std::invalid_argument("single-precision copy of complex values");
+    long cnt = K.fmt == DevFmt::CRS ? K.nnz : (long)K.ndiag * K.nrows;
+    int gv = vec_grid(cnt);
+    if (op.sval) {
+        if (!op.sval32) {
+            HIP_TRY(hipMalloc(&op.sval32, sizeof(float) * (cnt ? cnt : 1)));
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, st, cnt, (const double *)op.sval,
+                               (float *)op.sval32);
+            LAUNCH_CHECK();
+        }
+    } else if (!K.val32) {
+        HIP_TRY(hipMalloc(&K.val32, sizeof(float) * (cnt ? cnt : 1)));
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, st, cnt, (const double *)K.val,
+                           (float *)K.val32);
+        LAUNCH_CHECK();
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+}
+
+// ---------------------------------------------------------------------------
+// polynomial coefficients (host), restating matrix_module.f90:851-1057 in
+// the scalar type of the caller (real arithmetic for real_preconditioner).
+// ---------------------------------------------------------------------------
+template <class S>
+static void compute_alpha_t(int degree, const S *H, int ldh, int ncol, S *alpha) {
+    if (degree == 0) {
+        alpha[0] = S(1.0);
+        return;
+    }
+    if (ldh < degree + 2 || ncol < degree + 1)
+        throw std::invalid_argument("Hessenberg matrix too small for polynomial degree");
+    const int m = degree + 2;
+    std::vector<S> r((size_t)m * m, S(0.0));
+    auto R = [&](int i, int j) -> S & { return r[(size_t)i + (size_t)m * j]; };
+    R(0, 0) = S(1.0);
+    for (int i = 1; i <= degree + 1; i++) {
+        for (int c = 1; c <= degree + 1; c++) R(i, c) = R(i - 1, c - 1);
+        for (int j = 0; j <= i - 1; j++) {
+            S h = H[j + (size_t)ldh * (i - 1)];
+            for (int c = 0; c <= degree + 1; c++) R(i, c) = R(i, c) - h * R(j, c);
+        }
+        S hd = H[i + (size_t)ldh * (i - 1)];
+        for (int c = 0; c <= degree + 1; c++) R(i, c) = R(i, c) / hd;
+    }
+    for (int c = 0; c <= degree; c++) alpha[c] = -R(degree + 1, c + 1);
+}
+
+template <class S>
+static void chebyshev_t(int degree, const S foci[2], S *alpha) {
+    const int ld = degree + 2, nc = degree + 1;
+    std::vector<S> J((size_t)ld * nc, S(0.0));
+    auto JJ = [&](int i, int j) -> S & { return J[(size_t)(i - 1) + (size_t)ld * (j - 1)]; };
+    S theta = (foci[0] + foci[1]) / S(2.0);
+    S delta = (foci[0] - foci[1]) / S(2.0);
+    S sigma = theta / delta;
+    JJ(1, 1) = theta;
+    JJ(2, 1) = -theta;
+    S rho = S(1.0) / sigma;
+    for (int i = 2; i <= degree + 1; i++) {
+        JJ(i - 1, i) = -delta * rho / S(2.0);
+        JJ(i, i) = delta * sigma;
+        JJ(i + 1, i) = -delta * sigma + delta * rho / S(2.0);
+        rho = S(1.0) / (S(2.0) * sigma - rho);
+    }
+    compute_alpha_t<S>(degree, J.data(), ld, nc, alpha);
+}
+
+template <class S>
+static void neumann_t(int degree, S center, S *alpha) {
+    const int ld = degree + 2, nc = degree + 1;
+    std::vector<S> J((size_t)ld * nc, S(0.0));
+    for (int i = 1; i <= degree + 1; i++) {
+        J[(size_t)(i - 1) + (size_t)ld * (i - 1)] = center;
+        J[(size_t)i + (size_t)ld * (i - 1)] = -center;
+    }
+    compute_alpha_t<S>(degree, J.data(), ld, nc, alpha);
+}
+
+void compute_alpha(int degree, const std::complex<double> *H, int ldh, int ncol, std::complex<double> *alpha) {
+    compute_alpha_t<std::complex<double>>(degree, H, ldh, ncol, alpha);
+}
+void chebyshev(int degree, const std::complex<double> foci[2], std::complex<double> *alpha) {
+    chebyshev_t<std::complex<double>>(degree, foci, alpha);
+}
+void neumann(int degree, std::complex<double> center, std::complex<double> *alpha) {
+    neumann_t<std::complex<double>>(degree, center, alpha);
+}
+
+// real_preconditioner / complex_preconditioner (matrix_module.f90:551-722)
+void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> *foci,
+                 const std::complex<double> *center, const std::complex<double> *H, int ldh, int ncol,
+                 bool complex_poly) {
+    if (degree < 0) throw std::invalid_argument("negative degree");
+    op.poly_kind = kind;
+    op.degree = degree;
+    op.complex_poly = complex_poly;
+    op.alpha.assign((size_t)degree + 1, std::complex<double>(0.0));
+    if (kind == POLY_NONE) {
+        // the reference leaves alpha_p unallocated; polpre then only looks
+        // at degree, which the caller sets.  A degree without coefficients
+        // would dereference nothing in the reference: treat as degree 0.
+        op.degree = 0;
+        op.alpha.assign(1, std::complex<double>(1.0));
+        return;
+    }
+    const std::complex<double> sd = op.seed;
+    if (!complex_poly) {
+        std::vector<double> a((size_t)degree + 1);
+        if (kind == POLY_NEUMANN) {
+            double c = center ? center->real() - sd.real() : 1.0;
+            neumann_t<double>(degree, c, a.data());
+        } else if (kind == POLY_CHEBYSHEV) {
+            double f[2] = {0.25, 1.75};
+            if (foci) {
+                f[0] = foci[0].real() - sd.real();
+                f[1] = foci[1].real() - sd.real();
+            }
+            chebyshev_t<double>(degree, f, a.data());
+        } else if (kind == POLY_HESSENBERG) {
+            if (!H) throw std::invalid_argument("Hessenberg preconditioner needs H");
+            std::vector<double> Hr((size_t)ldh * ncol);
+            for (size_t q = 0; q < Hr.size(); q++) Hr[q] = H[q].real();
+            compute_alpha_t<double>(degree, Hr.data(), ldh, ncol, a.data());
+        } else {
+            throw std::invalid_argument("unknown polynomial kind");
+        }
+        for (int i = 0; i <= degree; i++) op.alpha[i] = a[i];
+    } else {
+        std::vector<std::complex<double>> a((size_t)degree + 1);
+        if (kind == POLY_NEUMANN) {
+            std::complex<double> c = center ? *center - sd : std::complex<double>(1.0);
+            neumann_t<std::complex<double>>(degree, c, a.data());
+        } else if (kind == POLY_CHEBYSHEV) {
+            std::complex<double> f[2] = {0.25, 1.75};
+            if (foci) {
+                f[0] = foci[0] - sd;
+                f[1] = foci[1] - sd;
+            }
+            chebyshev_t<std::complex<double>>(degree, f, a.data());
+        } else if (kind == POLY_HESSENBERG) {
+            if (!H) throw std::invalid_argument("Hessenberg preconditioner needs H");
+            compute_alpha_t<std::complex<double>>(degree, H, ldh, ncol, a.data());
+        } else {
+            throw std::invalid_argument("unknown polynomial kind");
+        }
+        op.alpha = a;
+    }
+}
+
+// explicit instantiations
+#define INST(T)                                                                                              \
+    template void op_spmv<T>(const Operator &, const T *, T, const T *, T, T *, int, const EpiArgs<T> &,     \
+                             const int *, hipStream_t);                                                      \
+    template void op_apply<T>(const Operator &, const T *, T *, T *, T *, int, const EpiArgs<T> &,           \
+                              const int *, hipStream_t);                                                     \
+    template void op_precon<T>(const Operator &, const T *, T *, hipStream_t);                               \
+    template void op_scaleback<T>(const Operator &, const T *, T *, T *, T *, T *, hipStream_t);             \
+    template void user_spmv<T>(const DevMatrix &, const T *, T *, hipStream_t);
+INST(double)
+INST(c128)
+INST(float)
+INST(c64)
+#undef INST
+
+}  // namespace idrs

@@ -1,0 +1,147 @@
+// Device-resident user matrices (CRS / COO / CDS / DIA) and the solver
+// operator A = K D^-1 p(K D^-1) (diagonal scaling + polynomial
+// preconditioner), the objects behind the C-ABI handles.
+#pragma once
+
+#include "common.hpp"
+
+#include <complex>
+#include <memory>
+#include <vector>
+
+namespace idrs {
+
+enum class ValKind { Pattern = 0, Real = 1, Complex = 2 };
+
+// Storage after upload: CRS (CRS and COO inputs) or DIA (CDS and DIA inputs).
+enum class DevFmt { CRS = 0, DIA = 1 };
+
+constexpr int SPMV_THREADS = 256;
+constexpr int LDS_NNZ = 2048;        // nonzeros staged in LDS per row block
+constexpr int MAX_DIAG = 1024;       // offsets held in LDS by the DIA kernel
+
+struct DevMatrix {
+    int orig_format = 0;     // user_module matrix_format: 2 CRS, 3 COO, 4 CDS, 5 DIA
+    DevFmt fmt = DevFmt::CRS;
+    ValKind kind = ValKind::Real;
+    int nrows = 0, ncols = 0;
+    long nnz = 0;            // stored entries (CRS) or ndiag*nrows (DIA)
+    // CRS (0-based on device)
+    int *row_ptr = nullptr, *col = nullptr;
+    int rpb = SPMV_THREADS;  // rows per row block
+    int nrb = 0;             // row blocks
+    // DIA
+    int ndiag = 0;
+    int *offsets = nullptr;
+    std::vector<int> h_offsets;
+    // values: double (Real/Pattern) or c128 (Complex); CRS: nnz, DIA: ndiag*nrows
+    void *val = nullptr;
+    // optional float copy (mixed precision), created on demand
+    void *val32 = nullptr;
+    // main diagonal (real_diagonal / complex_diagonal, zeros -> 1), device, as c128
+    // or double depending on kind
+    void *diag = nullptr;
+    ~DevMatrix();
+};
+
+// Polynomial preconditioner kinds (matrix_module real_preconditioner)
+enum { POLY_NONE = 0, POLY_NEUMANN = 1, POLY_CHEBYSHEV = 2, POLY_HESSENBERG = 3 };
+
+struct Operator {
+    DevMatrix *K = nullptr;       // not owned
+    int which_precon = 0;         // 0 identity, 1 diagonal scaling (A%P)
+    // diagonal of the preconditioner (D = diag(K) [+ shift terms]), device
+    void *pdiag = nullptr;        // double or c128 like K
+    ValKind pdiag_kind = ValKind::Real;
+    // values of K D^-1 (column scaled) when which_precon == 1, same layout as K
+    void *sval = nullptr;
+    void *sval32 = nullptr;       // float copy of the scaled values
+    int degree = 0;
+    int poly_kind = POLY_NONE;
+    std::vector<std::complex<double>> alpha;   // alpha_p(0:degree)
+    std::complex<double> seed{0.0, 0.0};       // A%rseed / A%cseed
+    bool complex_poly = false;                 // coefficients set by complex_preconditioner
+    ~Operator();
+    const void *values() const { return sval ? sval : K->val; }
+    bool values_complex() const {
+        return (K->kind == ValKind::Complex) || (sval && pdiag_kind == ValKind::Complex);
+    }
+};
+
+// epilogue kinds of the SpMV kernels
+enum { EPI_NONE = 0, EPI_PDOT = 1, EPI_OMEGA = 2, EPI_NORM = 3 };
+
+template <class T>
+struct EpiArgs {
+    const T *P = nullptr;       // shadow space PT(s,n): row q at P[q*s .. q*s+prow)
+    int s = 0, prow = 0, conjP = 0;
+    const T *r = nullptr;       // residual for EPI_OMEGA
+    T *partials = nullptr;      // [v * nparts + block]
+    int nparts = 0;
+};
+
+// Kernel timing (events around the dominant SpMV launches) for bench.
+struct SpmvTimer {
+    bool enabled = false;
+    std::vector<hipEvent_t> ev;
+    int used = 0;
+    void reset() { used = 0; }
+    double total_ms() const;
+    int launches() const { return used / 2; }
+    ~SpmvTimer();
+};
+SpmvTimer &spmv_timer();
+
+// y = xs * (K' x) [+ a z] with epilogue; K' = scaled values when the operator
+// has diagonal scaling.  T is the vector type.  'info' (device) makes the
+// kernel a no-op once a solver has finished (nullptr: always run).
+template <class T>
+void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int epi, const EpiArgs<T> &E,
+             const int *info, hipStream_t st);
+
+// w = A*v (rmatvec: Horner polynomial then system matvec), with the epilogue
+// applied to the last SpMV.  w1, w2: scratch vectors of n.
+template <class T>
+void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const EpiArgs<T> &E,
+              const int *info, hipStream_t st);
+
+// w = v / op (rprecon): divide by the preconditioner diagonal, or copy.
+template <class T>
+void op_precon(const Operator &op, const T *v, T *w, hipStream_t st);
+
+// x = scaleback(op, y) = polpre(y) / D
+template <class T>
+void op_scaleback(const Operator &op, const T *y, T *x, T *w1, T *w2, T *w3, hipStream_t st);
+
+// plain user-matrix product w = K v (no scaling, no polynomial)
+template <class T>
+void user_spmv(const DevMatrix &K, const T *v, T *w, hipStream_t st);
+
+// grid size for the streaming vector kernels
+int vec_grid(long n);
+int spmv_grid(const DevMatrix &K);
+
+// uploads
+DevMatrix *make_crs(int nrows, int ncols, const int *row_ptr1, const int *col_ind1, const void *val,
+                    ValKind kind, int orig_format);
+DevMatrix *make_coo(int nrows, int ncols, long nnz, const int *indx1, const int *jndx1, const void *val,
+                    ValKind kind);
+DevMatrix *make_cds(int nrows, int ncols, int half_band, const void *band, ValKind kind);
+DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const void *vals, ValKind kind);
+
+// operator construction
+void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shift);
+void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> *foci,
+                 const std::complex<double> *center, const std::complex<double> *H, int ldh, int ncol,
+                 bool complex_poly);
+void op_make_lowprec(Operator &op);
+
+// polynomial coefficient computation (host, restating matrix_module)
+void compute_alpha(int degree, const std::complex<double> *H, int ldh, int ncol,
+                   std::complex<double> *alpha);
+void chebyshev(int degree, const std::complex<double> foci[2], std::complex<double> *alpha);
+void neumann(int degree, std::complex<double> center, std::complex<double> *alpha);
+
+hipStream_t lib_stream();
+
+}  // namespace idrs

@@ -1,0 +1,23 @@
+"""MI355X-native IDR(s) solver hot path (HIP kernels behind a C ABI).
+
+The public API mirrors the reference IDRS package (user_module converters,
+type(matrix) with operator(*) and operator(/), polynomial preconditioners,
+the IDRS solver); see api.py.
+"""
+from ._lib import IdrsError, load as load_library  # noqa: F401
+from .api import (  # noqa: F401
+    IdrsResult,
+    Matrix,
+    UserMatrix,
+    cds_format,
+    complex_diagonal,
+    complex_preconditioner,
+    coo_format,
+    crs_format,
+    dia_format,
+    idrs,
+    make_p,
+    real_diagonal,
+    real_preconditioner,
+    scaleback,
+)

@@ -1,0 +1,353 @@
+"""Python mirror of the reference IDRS package interface for the hot path.
+
+Names, argument meaning and error behaviour follow the Fortran modules
+(paths relative to the reference's IDRS/ directory):
+
+  user_module   crs_format / coo_format / cds_format, real_diagonal,
+                complex_diagonal, operator(*)       MODULES/PP_MATRIX/user_module.f90
+  matrix_module type(matrix), real_preconditioner, complex_preconditioner,
+                operator(*) / operator(/), scaleback  MODULES/PP_MATRIX/matrix_module.f90
+  idrs_module   IDRS (RIDRS / CIDRS)                 MODULES/IDRS/idrs_module.f90
+
+Every computation runs in libidrs_hip.so on the GPU; there is no CPU path.
+Index arrays are 1-based like the reference's.  Vectors may be numpy arrays
+(host) or torch CUDA tensors (device resident; no host round trip).
+"""
+from collections import namedtuple
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, idrs_opts_t, idrs_stats_t
+
+IdrsResult = namedtuple("IdrsResult", "x flag relres iterations resvec solve_ms spmv_ms spmv_launches")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+class UserMatrix:
+    """type(user_matrix) (user_module.f90:134-167), stored on the device."""
+
+    __array_ufunc__ = None  # numpy defers v*K, v/K to this class
+
+    def __init__(self, handle, is_complex):
+        self._h = ctypes.c_void_p(handle)
+        self.is_complex = is_complex
+        n_r, n_c, nnz, fmt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64(), ctypes.c_int()
+        check(_lib.load().idrs_user_matrix_info(self._h, ctypes.byref(n_r), ctypes.byref(n_c), ctypes.byref(nnz), ctypes.byref(fmt)))
+        self.nrows, self.ncols, self.nnz, self.matrix_format = n_r.value, n_c.value, nnz.value, fmt.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().idrs_user_matrix_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def __mul__(self, v):
+        """w = K*v (ruser_mv / cuser_mv, user_module.f90:197-241)."""
+        v = np.asarray(v)
+        cplx = self.is_complex or np.iscomplexobj(v)
+        dt = np.complex128 if cplx else np.float64
+        v = np.ascontiguousarray(v, dtype=dt)
+        w = np.empty(self.nrows, dtype=dt)
+        f = _lib.load().idrs_user_mv_c128 if cplx else _lib.load().idrs_user_mv_f64
+        check(f(self._h, _ptr(v), _ptr(w)))
+        return w
+
+
+def _new_um(fn, *args, is_complex=False):
+    h = ctypes.c_void_p()
+    check(fn(*args, ctypes.byref(h)))
+    return UserMatrix(h.value, is_complex)
+
+
+def crs_format(nrows, ncols, row_ptr, col_ind, rval=None, cval=None):
+    """A = crs_format(nrows, ncols, row_ptr, col_ind, rval|cval); 1-based
+    (user_module.f90:829-863)."""
+    lib = _lib.load()
+    rp, ci = _i32(row_ptr), _i32(col_ind)
+    if cval is not None:
+        cv = np.ascontiguousarray(cval, dtype=np.complex128)
+        return _new_um(lib.idrs_crs_format_c128, int(nrows), int(ncols), _ptr(rp), _ptr(ci), _ptr(cv), is_complex=True)
+    rv = None if rval is None else np.ascontiguousarray(rval, dtype=np.float64)
+    return _new_um(lib.idrs_crs_format_f64, int(nrows), int(ncols), _ptr(rp), _ptr(ci), _ptr(rv))
+
+
+def coo_format(nrows, ncols, nnz, indx, jndx, rval=None, cval=None):
+    """A = coo_format(nrows, ncols, nnz, indx, jndx, rval|cval); 1-based
+    (user_module.f90:865-898)."""
+    lib = _lib.load()
+    ix, jx = _i32(indx), _i32(jndx)
+    if cval is not None:
+        cv = np.ascontiguousarray(cval, dtype=np.complex128)
+        return _new_um(lib.idrs_coo_format_c128, int(nrows), int(ncols), int(nnz), _ptr(ix), _ptr(jx), _ptr(cv), is_complex=True)
+    rv = None if rval is None else np.ascontiguousarray(rval, dtype=np.float64)
+    return _new_um(lib.idrs_coo_format_f64, int(nrows), int(ncols), int(nnz), _ptr(ix), _ptr(jx), _ptr(rv))
+
+
+def cds_format(nrows, ncols, half_band, rband=None, cband=None):
+    """A = cds_format(nrows, ncols, half_band, rband|cband); band(nrows,
+    -hb:hb) given as a numpy array of shape (nrows, 2*hb+1)
+    (user_module.f90:932-958)."""
+    lib = _lib.load()
+    if cband is not None:
+        b = np.asfortranarray(cband, dtype=np.complex128)
+        return _new_um(lib.idrs_cds_format_c128, int(nrows), int(ncols), int(half_band), _ptr(b), is_complex=True)
+    b = None if rband is None else np.asfortranarray(rband, dtype=np.float64)
+    return _new_um(lib.idrs_cds_format_f64, int(nrows), int(ncols), int(half_band), _ptr(b))
+
+
+def dia_format(nrows, ncols, offsets, rvals=None, cvals=None):
+    """Offset-list diagonal storage, values of shape (nrows, ndiag) for
+    strictly ascending offsets (extension of cds_format for wide bands)."""
+    lib = _lib.load()
+    off = _i32(offsets)
+    if cvals is not None:
+        v = np.asfortranarray(cvals, dtype=np.complex128)
+        return _new_um(lib.idrs_dia_format_c128, int(nrows), int(ncols), len(off), _ptr(off), _ptr(v), is_complex=True)
+    v = None if rvals is None else np.asfortranarray(rvals, dtype=np.float64)
+    return _new_um(lib.idrs_dia_format_f64, int(nrows), int(ncols), len(off), _ptr(off), _ptr(v))
+
+
+def real_diagonal(K):
+    """D = real_diagonal(K), zeros replaced by 1 (user_module.f90:960-1047)."""
+    d = np.empty(K.nrows)
+    check(_lib.load().idrs_real_diagonal(K.handle, _ptr(d)))
+    return d
+
+
+def complex_diagonal(K):
+    """D = complex_diagonal(K) (user_module.f90:1049-1161)."""
+    d = np.empty(K.nrows, dtype=np.complex128)
+    check(_lib.load().idrs_complex_diagonal(K.handle, _ptr(d)))
+    return d
+
+
+class Matrix:
+    """type(matrix) (matrix_module.f90:38-81): the operator the solvers use.
+
+    A = Matrix(K) sets A%K = K; set_precon(1) sets A%P = REAL_PRECON(1, K)
+    (diagonal scaling); real_preconditioner()/complex_preconditioner() add the
+    Chebyshev/Neumann polynomial.  A * v is rmatvec/cmatvec, v / A is
+    rprecon/cprecon.
+    """
+
+    __array_ufunc__ = None  # numpy defers v / A to __rtruediv__
+
+    def __init__(self, K, which_precon=0, shift=None):
+        self.K = K  # keeps the user matrix alive
+        h = ctypes.c_void_p()
+        check(_lib.load().idrs_matrix_create(K.handle, ctypes.byref(h)))
+        self._h = h
+        self.is_complex = K.is_complex
+        self.n = K.nrows
+        if which_precon:
+            self.set_precon(which_precon, shift)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().idrs_matrix_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def set_precon(self, which_precon, shift=None):
+        """A%P = REAL_PRECON/COMPLEX_PRECON(which_precon, K[, shift])."""
+        lib = _lib.load()
+        if shift is not None and np.iscomplexobj(shift):
+            s = np.array([np.real(shift), np.imag(shift)], dtype=np.float64)
+            check(lib.idrs_matrix_set_precon_c128(self._h, int(which_precon), _ptr(s)))
+        else:
+            s = None if shift is None else np.array([float(shift)])
+            check(lib.idrs_matrix_set_precon_f64(self._h, int(which_precon), _ptr(s)))
+
+    def set_seed(self, seed):
+        lib = _lib.load()
+        if np.iscomplexobj(seed):
+            s = np.array([np.real(seed), np.imag(seed)], dtype=np.float64)
+            check(lib.idrs_matrix_set_seed_c128(self._h, _ptr(s)))
+        else:
+            check(lib.idrs_matrix_set_seed_f64(self._h, float(seed)))
+
+    @property
+    def alpha(self):
+        deg = ctypes.c_int()
+        buf = np.zeros(2 * 64)
+        check(_lib.load().idrs_matrix_get_alpha(self._h, _ptr(buf), 64, ctypes.byref(deg)))
+        a = buf[: 2 * (deg.value + 1)].view(np.complex128)
+        return a.copy()
+
+    def _vec(self, v, cplx):
+        dt = np.complex128 if cplx else np.float64
+        return np.ascontiguousarray(v, dtype=dt)
+
+    def __mul__(self, v):
+        """w = A*v (rmatvec / cmatvec, matrix_module.f90:135-167)."""
+        cplx = self.is_complex or np.iscomplexobj(v)
+        v = self._vec(v, cplx)
+        w = np.empty_like(v)
+        f = _lib.load().idrs_matvec_c128 if cplx else _lib.load().idrs_matvec_f64
+        check(f(self._h, _ptr(v), _ptr(w)))
+        return w
+
+    def __rtruediv__(self, v):
+        """w = v/A (rprecon / cprecon, matrix_module.f90:306-400)."""
+        cplx = self.is_complex or np.iscomplexobj(v)
+        v = self._vec(v, cplx)
+        w = np.empty_like(v)
+        f = _lib.load().idrs_precon_c128 if cplx else _lib.load().idrs_precon_f64
+        check(f(self._h, _ptr(v), _ptr(w)))
+        return w
+
+
+def real_preconditioner(A, preconditioner=0, degree=0, foci=None, center=None, H=None):
+    """call real_preconditioner(A, preconditioner, degree, foci, center, H)
+    (matrix_module.f90:551-635): 0 none, 1 Neumann, 2 Chebyshev, 3 Hessenberg."""
+    f = None if foci is None else np.ascontiguousarray(foci, dtype=np.float64)
+    c = None if center is None else np.array([float(center)])
+    h, ldh, nc = None, 0, 0
+    if H is not None:
+        h = np.asfortranarray(H, dtype=np.float64)
+        ldh, nc = h.shape
+    check(_lib.load().idrs_real_preconditioner(A.handle, int(preconditioner), int(degree), _ptr(f), _ptr(c), _ptr(h), ldh, nc))
+
+
+def complex_preconditioner(A, preconditioner=0, degree=0, foci=None, center=None, H=None):
+    """call complex_preconditioner(...) (matrix_module.f90:637-722)."""
+    f = None if foci is None else np.ascontiguousarray(foci, dtype=np.complex128)
+    c = None if center is None else np.array([complex(center)], dtype=np.complex128)
+    h, ldh, nc = None, 0, 0
+    if H is not None:
+        h = np.asfortranarray(H, dtype=np.complex128)
+        ldh, nc = h.shape
+    check(_lib.load().idrs_complex_preconditioner(A.handle, int(preconditioner), int(degree), _ptr(f), _ptr(c), _ptr(h), ldh, nc))
+
+
+def scaleback(A, b, y):
+    """x = scaleback(A, b, y) = polpre(y) / D (matrix_module.f90:808-849)."""
+    cplx = A.is_complex or np.iscomplexobj(y)
+    dt = np.complex128 if cplx else np.float64
+    y = np.ascontiguousarray(y, dtype=dt)
+    x = np.empty_like(y)
+    f = _lib.load().idrs_scaleback_c128 if cplx else _lib.load().idrs_scaleback_f64
+    check(f(A.handle, _ptr(y), _ptr(x)))
+    return x
+
+
+def make_p(n, s, method=1, PT_raw=None, r=None, seed=1, complex=False):
+    """PT(s,n) from MAKE_P (idrs_module.f90:839-937), returned as an (n, rows)
+    array (row q holds PT(:,q))."""
+    rows = s if method == 1 else 1
+    dt = np.complex128 if complex else np.float64
+    raw = None if PT_raw is None else np.ascontiguousarray(PT_raw, dtype=dt)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=dt)
+    out = np.empty((n, rows), dtype=dt)
+    f = _lib.load().idrs_make_p_c128 if complex else _lib.load().idrs_make_p_f64
+    check(f(int(n), int(s), int(method), _ptr(raw), _ptr(rr), int(seed), _ptr(out)))
+    return out
+
+
+def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, U0=None,
+         omega=None, resvec=False, PT_raw=None, p_seed=None, time_spmv=False, out=None):
+    """x = IDRS(A, b, s, M1, tolerance, maximum_iterations, variant, flag,
+    relres, iterations, x0, U0, omega, resvec)  (idrs_module.f90:107-251).
+
+    Returns IdrsResult(x, flag, relres, iterations, resvec, solve_ms, ...).
+    b may be a numpy array (host) or a torch CUDA tensor (device-resident
+    solve; x0, U0, PT_raw and out are then CUDA tensors too).
+    PT_raw: raw shadow space, shape (n, s) (row q = PT(:,q)), orthonormalised
+    by the library like MAKE_P does with its random numbers.
+    """
+    lib = _lib.load()
+    device = _is_torch(b)
+    n = int(b.shape[0])
+    o = idrs_opts_t()
+    keep = []
+    if M1 is not None:
+        o.has_M1 = 1
+        o.M1 = M1.handle if isinstance(M1, Matrix) else None
+    if tolerance is not None:
+        if tolerance < 0:
+            raise ValueError("Illegal value for parameter tolerance")
+        o.has_tolerance, o.tolerance = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    maxit = int(maximum_iterations) if maximum_iterations is not None else min(2 * n, 1000)
+    if device:
+        import torch
+
+        cplx = b.is_complex()
+        dt = torch.complex128 if cplx else torch.float64
+        if b.dtype != dt or not b.is_contiguous():
+            raise TypeError("device b must be a contiguous float64/complex128 CUDA tensor")
+        x = out if out is not None else torch.empty_like(b)
+        o.device_pointers = 1
+        for name, arr in (("x0", x0), ("U0", U0), ("PT_raw", PT_raw)):
+            if arr is not None:
+                if not (_is_torch(arr) and arr.is_cuda and arr.dtype == dt and arr.is_contiguous()):
+                    raise TypeError(f"{name} must be a contiguous CUDA tensor of dtype {dt}")
+                setattr(o, name, arr.data_ptr())
+        bp, xp = ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr())
+        npdt = np.complex128 if cplx else np.float64
+        # the library runs on its own stream: finish torch's producers first
+        torch.cuda.current_stream().synchronize()
+    else:
+        cplx = A.is_complex or np.iscomplexobj(b)
+        npdt = np.complex128 if cplx else np.float64
+        b = np.ascontiguousarray(b, dtype=npdt)
+        x = np.empty_like(b)
+        if x0 is not None:
+            a = np.ascontiguousarray(x0, dtype=npdt)
+            keep.append(a)
+            o.x0 = a.ctypes.data
+        if U0 is not None:
+            a = np.asfortranarray(U0, dtype=npdt)  # U0(n,s) column-major
+            keep.append(a)
+            o.U0 = a.ctypes.data
+        if PT_raw is not None:
+            a = np.ascontiguousarray(PT_raw, dtype=npdt)  # (n, rows) C order == PT(rows,n) column-major
+            keep.append(a)
+            o.PT_raw = a.ctypes.data
+        bp, xp = _ptr(b), _ptr(x)
+    if omega is not None:
+        om = np.ascontiguousarray(np.atleast_1d(omega), dtype=npdt)
+        keep.append(om)
+        o.omega, o.n_omega = om.ctypes.data, len(om)
+    rv = None
+    if resvec:
+        rv = np.zeros(maxit + 1)
+        o.resvec, o.resvec_len = rv.ctypes.data, len(rv)
+    o.p_seed = int(p_seed) if p_seed is not None else 0
+    o.time_spmv = 1 if time_spmv else 0
+    st = idrs_stats_t()
+    f = lib.idrs_idrs_c128 if cplx else lib.idrs_idrs_f64
+    check(f(A.handle, n, bp, int(s), ctypes.byref(o), xp, ctypes.byref(st)))
+    if rv is not None:
+        rv = rv[: st.iterations + 1]
+    return IdrsResult(x, st.flag, st.relres, st.iterations, rv, st.solve_ms, st.spmv_ms, st.spmv_launches)

@@ -1,0 +1,195 @@
+/*
+ * idrs_capi.h - C ABI of the MI355X-native IDR(s) library (libidrs_hip.so).
+ *
+ * This is the boundary the Fortran drop-in modules (idrs-package_amd/fortran)
+ * bind with iso_c_binding, and that Python binds with ctypes.  Each entry
+ * point names the reference interface it replaces (paths relative to the
+ * reference's IDRS/ directory).
+ *
+ * Conventions
+ *   - Index arrays are 1-based, as the reference's converters take them.
+ *   - Complex arrays are interleaved (re, im) doubles, i.e. Fortran
+ *     complex(kind=real64) / C double _Complex.
+ *   - 2-D arrays are Fortran column-major.
+ *   - Vectors are HOST pointers unless idrs_opts_t.device_pointers is set
+ *     (then b, x, x0, U0 and PT_raw are device pointers on the current HIP
+ *     device, and the solve runs with inputs already resident in HBM).
+ *   - Every function returns IDRS_OK or a negative status; idrs_last_error()
+ *     gives the message.  The reference's "error stop" conditions (negative
+ *     tolerance, short resvec, bad dimensions) return IDRS_ERR_ARG.
+ *   - One host thread per device; calls are ordered on the library stream.
+ */
+#ifndef IDRS_CAPI_H
+#define IDRS_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IDRS_OK 0
+#define IDRS_ERR_ARG (-1)
+#define IDRS_ERR_HIP (-2)
+#define IDRS_ERR_UNSUPPORTED (-3)
+#define IDRS_ERR_INTERNAL (-4)
+
+typedef struct idrs_user_matrix_s *idrs_user_matrix_t; /* type(user_matrix) */
+typedef struct idrs_matrix_s *idrs_matrix_t;           /* type(matrix)      */
+
+/* Message of the last failed call on this thread (empty if none). */
+const char *idrs_last_error(void);
+/* Library version (major*10000 + minor*100 + patch). */
+int idrs_version(void);
+
+/* ---------------------------------------------------------------------
+ * type(user_matrix) converters
+ * ------------------------------------------------------------------- */
+
+/* A = crs_format(nrows, ncols, row_ptr, col_ind, rval)
+   replaces PP_MATRIX/user_module.f90:829-863 (crs_format); rval NULL gives
+   the pattern matrix (all values 1). */
+int idrs_crs_format_f64(int nrows, int ncols, const int *row_ptr, const int *col_ind, const double *rval,
+                        idrs_user_matrix_t *A);
+/* crs_format(..., cval=) - complex values, user_module.f90:829-863 */
+int idrs_crs_format_c128(int nrows, int ncols, const int *row_ptr, const int *col_ind, const double *cval,
+                         idrs_user_matrix_t *A);
+/* A = coo_format(nrows, ncols, nnz, indx, jndx, rval)
+   replaces user_module.f90:865-898 (coo_format) */
+int idrs_coo_format_f64(int nrows, int ncols, int nnz, const int *indx, const int *jndx, const double *rval,
+                        idrs_user_matrix_t *A);
+/* coo_format(..., cval=), user_module.f90:865-898 */
+int idrs_coo_format_c128(int nrows, int ncols, int nnz, const int *indx, const int *jndx, const double *cval,
+                         idrs_user_matrix_t *A);
+/* A = cds_format(nrows, ncols, half_band, rband) with rband(nrows,-hb:hb)
+   replaces user_module.f90:932-958 (cds_format) */
+int idrs_cds_format_f64(int nrows, int ncols, int half_band, const double *rband, idrs_user_matrix_t *A);
+/* cds_format(..., cband=), user_module.f90:932-958 */
+int idrs_cds_format_c128(int nrows, int ncols, int half_band, const double *cband, idrs_user_matrix_t *A);
+/* Offset-list diagonal storage: values(nrows, ndiag) for strictly ascending
+   offsets.  Extension of cds_format (user_module.f90:932-958) for wide
+   sparse bands: the same per-row arithmetic over the stored offsets only. */
+int idrs_dia_format_f64(int nrows, int ncols, int ndiag, const int *offsets, const double *vals,
+                        idrs_user_matrix_t *A);
+/* idrs_dia_format_f64 with complex values */
+int idrs_dia_format_c128(int nrows, int ncols, int ndiag, const int *offsets, const double *vals,
+                         idrs_user_matrix_t *A);
+/* Frees the device storage of a user matrix (the reference's allocatable
+   components are released when the Fortran object goes out of scope). */
+int idrs_user_matrix_destroy(idrs_user_matrix_t A);
+/* nrows, ncols, stored entries, matrix_format (2 CRS, 3 COO, 4 CDS, 5 DIA) */
+int idrs_user_matrix_info(idrs_user_matrix_t A, int *nrows, int *ncols, int64_t *nnz, int *format);
+
+/* w = K*v: ruser_mv, user_module.f90:197-218 */
+int idrs_user_mv_f64(idrs_user_matrix_t K, const double *v, double *w);
+/* w = K*v: cuser_mv, user_module.f90:220-241 */
+int idrs_user_mv_c128(idrs_user_matrix_t K, const double *v, double *w);
+/* D = real_diagonal(K), user_module.f90:960-1047 (zeros replaced by 1) */
+int idrs_real_diagonal(idrs_user_matrix_t K, double *D);
+/* D = complex_diagonal(K), user_module.f90:1049-1161 */
+int idrs_complex_diagonal(idrs_user_matrix_t K, double *D);
+
+/* ---------------------------------------------------------------------
+ * type(matrix): operator A = p(K D^-1) K D^-1 applied by operator(*)
+ * ------------------------------------------------------------------- */
+
+/* A%K = K (PP_IDRS/real_matrix.inc:9); K must outlive A */
+int idrs_matrix_create(idrs_user_matrix_t K, idrs_matrix_t *A);
+int idrs_matrix_destroy(idrs_matrix_t A);
+/* A%P = REAL_PRECON(which_precon, K[, shift]): which_precon 0 (identity) or
+   1 (diagonal scaling), user_module.f90:1327-1381; shift may be NULL. */
+int idrs_matrix_set_precon_f64(idrs_matrix_t A, int which_precon, const double *shift);
+/* A%P = COMPLEX_PRECON(which_precon, K[, shift]), user_module.f90:1465-1518;
+   shift is one complex (re, im) or NULL. */
+int idrs_matrix_set_precon_c128(idrs_matrix_t A, int which_precon, const double *shift);
+/* A%rseed = seed (PP_IDRS/real_matrix.inc:41-45) */
+int idrs_matrix_set_seed_f64(idrs_matrix_t A, double seed);
+/* A%cseed = seed (complex_matrix.inc) */
+int idrs_matrix_set_seed_c128(idrs_matrix_t A, const double *seed);
+/* call real_preconditioner(A, preconditioner, degree, foci, center, H),
+   PP_MATRIX/matrix_module.f90:551-635.  preconditioner: 0 none, 1 Neumann,
+   2 Chebyshev, 3 Hessenberg.  foci (2 values), center (1 value) and
+   H(ldh, ncol) may be NULL (absent). */
+int idrs_real_preconditioner(idrs_matrix_t A, int preconditioner, int degree, const double *foci,
+                             const double *center, const double *H, int ldh, int ncol);
+/* call complex_preconditioner(...), matrix_module.f90:637-722; complex
+   arguments interleaved. */
+int idrs_complex_preconditioner(idrs_matrix_t A, int preconditioner, int degree, const double *foci,
+                                const double *center, const double *H, int ldh, int ncol);
+/* A%ralpha_p / A%calpha_p(0:degree) as (re, im) pairs; *degree out */
+int idrs_matrix_get_alpha(idrs_matrix_t A, double *alpha, int maxlen, int *degree);
+/* w = A*v: rmatvec, matrix_module.f90:152-167 */
+int idrs_matvec_f64(idrs_matrix_t A, const double *v, double *w);
+/* w = A*v: cmatvec, matrix_module.f90:135-150 */
+int idrs_matvec_c128(idrs_matrix_t A, const double *v, double *w);
+/* w = v/A: rprecon, matrix_module.f90:306-352 */
+int idrs_precon_f64(idrs_matrix_t A, const double *v, double *w);
+/* w = v/A: cprecon, matrix_module.f90:354-400 */
+int idrs_precon_c128(idrs_matrix_t A, const double *v, double *w);
+/* x = scaleback(A, b, y): rscaleback, matrix_module.f90:831-849 */
+int idrs_scaleback_f64(idrs_matrix_t A, const double *y, double *x);
+/* x = scaleback(A, b, y): cscaleback, matrix_module.f90:808-829 */
+int idrs_scaleback_c128(idrs_matrix_t A, const double *y, double *x);
+
+/* ---------------------------------------------------------------------
+ * solvers
+ * ------------------------------------------------------------------- */
+
+/* Optional arguments of IDRS (MODULES/IDRS/idrs_module.f90:180-251).  has_*
+   flags play the role of Fortran's present(). */
+typedef struct {
+    int32_t has_M1;             /* present(M1)                            */
+    int32_t has_tolerance;
+    int32_t has_maximum_iterations;
+    int32_t has_variant;
+    idrs_matrix_t M1;           /* v/M1 divides by M1's diagonal if set   */
+    double tolerance;
+    int32_t maximum_iterations;
+    int32_t variant;            /* 1 IDR(s), 2 BiCGSTAB, 3/4 one-image minsync */
+    const void *x0;             /* x0(n)                                  */
+    const void *U0;             /* U0(n,s)                                */
+    const void *omega;          /* omega(n_omega), host                   */
+    int32_t n_omega;
+    int32_t resvec_len;
+    double *resvec;             /* resvec(maxit+1), host, output          */
+    const void *PT_raw;         /* raw shadow space PT(s,n) before Gram-Schmidt;
+                                   NULL: the library's uniform stream      */
+    uint64_t p_seed;            /* seed of that stream (0 -> 1, the reference's this_image()) */
+    int32_t device_pointers;    /* b, x, x0, U0, PT_raw are device pointers */
+    int32_t time_spmv;          /* time each SpMV launch with HIP events  */
+} idrs_opts_t;
+
+typedef struct {
+    int32_t flag;               /* 0 converged, 1 maxit, 2 true residual above tol, 3 breakdown */
+    int32_t iterations;
+    double relres;              /* ||b - A x|| / ||b|| (true residual)   */
+    double solve_ms;            /* device time of the solve (HIP events)  */
+    double spmv_ms;             /* summed SpMV launch time (time_spmv)    */
+    int32_t spmv_launches;
+    int32_t reserved;
+} idrs_stats_t;
+
+/* x = IDRS(A, b, s, [M1, tolerance, maximum_iterations, variant, flag,
+   relres, iterations, x0, U0, omega, resvec]): RIDRS,
+   MODULES/IDRS/idrs_module.f90:180-251 with MODULES/IDRS/idrs.inc:1-311.
+   (The H output, used only by the out-of-scope IDRS_RITZ, is not offered.) */
+int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                  idrs_stats_t *stats);
+/* CIDRS, idrs_module.f90:107-178 */
+int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                   idrs_stats_t *stats);
+
+/* PT = MAKE_P(...) for inspection: the orthonormalised shadow space PT(s,n)
+   (RMAKE_P / CMAKE_P, idrs_module.f90:839-937).  method 1 Gram-Schmidt of
+   PT_raw (or the library stream when NULL); method 2 PT = conj(r). Host arrays. */
+int idrs_make_p_f64(int n, int s, int method, const double *PT_raw, const double *r, uint64_t seed, double *PT);
+int idrs_make_p_c128(int n, int s, int method, const double *PT_raw, const double *r, uint64_t seed, double *PT);
+
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+int idrs_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IDRS_CAPI_H */

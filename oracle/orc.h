@@ -1,0 +1,125 @@
+/*
+ * Oracle: a plain-C, single-threaded restatement of the reference IDRS
+ * package's hot path (IDR(s) solvers, operator, polynomial preconditioner,
+ * sparse formats).  TEST INFRASTRUCTURE ONLY: it is the checker that the
+ * HIP library is compared against, and the CPU baseline timed by bench.py.
+ * Nothing in the product links, loads or calls it.
+ *
+ * Every routine names the reference file:line it restates (paths relative to
+ * the reference's IDRS/ directory).  Arithmetic follows the reference's
+ * statement order: one vector statement at a time, sequential sums, the same
+ * single-precision literals (kappa = 0.7 -> 0.699999988079071, sqrt(2.) in
+ * single precision).  The build uses -ffp-contract=off so every multiply and
+ * add rounds separately, as the reference's array statements do.
+ *
+ * Indexing: all index arrays are 1-based (Fortran convention), vectors are
+ * 0-based C arrays of length n.  Column-major (Fortran) layout for 2-D arrays:
+ * PT(s,n) -> PT[i + s*j]; G(n,s) -> G[j + n*i]; band(n,-hb:hb) ->
+ * band[i + n*(d+hb)].
+ */
+#ifndef ORC_H
+#define ORC_H
+
+#include <complex.h>
+#include <stdint.h>
+
+typedef double _Complex orc_cplx;
+
+/* user_matrix formats, PP_MATRIX/user_module.f90:134-167 */
+enum { ORC_CRS = 2, ORC_COO = 3, ORC_CDS = 4, ORC_DIA = 5 };
+
+typedef struct {
+    int format;
+    int nrows, ncols;
+    /* values: exactly one of rval / cval set (or neither: pattern matrix) */
+    const double *rval;
+    const orc_cplx *cval;
+    /* CRS (user_module.f90:829-863) */
+    const int *row_ptr, *col_ind;
+    /* COO (user_module.f90:865-898) */
+    int nnz;
+    const int *indx, *jndx;
+    /* CDS dense band (user_module.f90:932-958): values in rval/cval,
+       laid out (nrows, -half_band:half_band) column-major */
+    int half_band;
+    /* DIA offset list (extension; same arithmetic as CDS restricted to the
+       stored offsets, in ascending offset order): values (nrows, ndiag) */
+    int ndiag;
+    const int *offsets;
+} orc_user_matrix;
+
+/* type(matrix), PP_MATRIX/matrix_module.f90:38-81, restricted to what the hot
+   path uses: K, diagonal preconditioner P (which_precon 0/1), seed shift,
+   polynomial coefficients. */
+typedef struct {
+    orc_user_matrix K;
+    int which_precon;          /* 0 identity, 1 diagonal scaling            */
+    const double *rdiag;       /* P%rband(:,0) when which_precon == 1 (real) */
+    const orc_cplx *cdiag;     /* P%cband(:,0) (complex)                     */
+    double rseed;              /* A%rseed                                   */
+    orc_cplx cseed;            /* A%cseed                                   */
+    int degree;                /* A%degree                                  */
+    const double *ralpha_p;    /* A%ralpha_p(0:degree)                      */
+    const orc_cplx *calpha_p;  /* A%calpha_p(0:degree)                      */
+} orc_matrix;
+
+/* optional arguments of IDRS (idrs_module.f90:180-251).  has_* flags mirror
+   Fortran's present(). */
+typedef struct {
+    int has_M1;  const orc_matrix *M1;
+    int has_tol; double tol;
+    int has_maxit; int maxit;
+    int has_variant; int variant;
+    const void *x0;             /* T[n] or NULL                     */
+    const void *U0;             /* T[n*s] column-major or NULL      */
+    const void *omega; int n_omega;  /* T[n_omega] or NULL          */
+    double *resvec; int resvec_len;  /* out, or NULL                */
+    /* Shadow space: raw random PT(s,n) before orthonormalisation.  NULL ->
+       the oracle's default uniform [0,1) stream (orc_uniform01). */
+    const void *PT_raw;
+} orc_idrs_opts;
+
+typedef struct {
+    int flag;
+    double relres;
+    int iterations;
+} orc_stats;
+
+/* ---- default shadow-space stream (uniform [0,1), counter based) ---- */
+double orc_uniform01(uint64_t seed, uint64_t index);
+
+/* ---- user_module restatement ---- */
+void orc_r_user_mv(const orc_user_matrix *A, const double *v, double *w);
+void orc_c_user_mv(const orc_user_matrix *A, const orc_cplx *v, orc_cplx *w);
+void orc_r_diagonal(const orc_user_matrix *A, double *D);
+void orc_c_diagonal(const orc_user_matrix *A, orc_cplx *D);
+
+/* ---- matrix_module restatement ---- */
+void orc_r_matvec(const orc_matrix *A, const double *v, double *w);
+void orc_c_matvec(const orc_matrix *A, const orc_cplx *v, orc_cplx *w);
+void orc_r_precon(const orc_matrix *A, const double *v, double *w);
+void orc_c_precon(const orc_matrix *A, const orc_cplx *v, orc_cplx *w);
+void orc_r_scaleback(const orc_matrix *A, const double *y, double *x);
+void orc_c_scaleback(const orc_matrix *A, const orc_cplx *y, orc_cplx *x);
+int  orc_r_compute_alpha(int degree, const double *H, int ldh, int ncol, double *alpha);
+int  orc_c_compute_alpha(int degree, const orc_cplx *H, int ldh, int ncol, orc_cplx *alpha);
+void orc_r_chebyshev(int degree, const double foci[2], double *alpha);
+void orc_c_chebyshev(int degree, const orc_cplx foci[2], orc_cplx *alpha);
+void orc_r_neumann(int degree, double center, double *alpha);
+void orc_c_neumann(int degree, orc_cplx center, orc_cplx *alpha);
+void orc_r_ms_parameters(const orc_matrix *A, int n_sigma, const double *sigma,
+                         double *sigma_p, double *alpha_s);
+void orc_c_ms_parameters(const orc_matrix *A, int n_sigma, const orc_cplx *sigma,
+                         orc_cplx *sigma_p, orc_cplx *alpha_s);
+
+/* ---- idrs_module restatement ---- */
+double orc_r_norm(int n, const double *w);
+double orc_c_norm(int n, const orc_cplx *w);
+void orc_r_make_p(int n, int s, int method, const double *PT_raw, const double *r, double *PT);
+void orc_c_make_p(int n, int s, int method, const orc_cplx *PT_raw, const orc_cplx *r, orc_cplx *PT);
+int orc_r_idrs(const orc_matrix *A, int n, const double *b, int s,
+               const orc_idrs_opts *o, double *x, orc_stats *st);
+int orc_c_idrs(const orc_matrix *A, int n, const orc_cplx *b, int s,
+               const orc_idrs_opts *o, orc_cplx *x, orc_stats *st);
+
+#endif

@@ -1,0 +1,267 @@
+"""ctypes front end of the oracle (oracle/liborc.so, built from orc.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker of the HIP library.  The
+functions mirror the reference routines restated in orc_generic.inc.
+"""
+import ctypes
+import os
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liborc.so")
+
+CRS, COO, CDS, DIA = 2, 3, 4, 5
+
+
+class UserMatrix(ctypes.Structure):
+    _fields_ = [
+        ("format", c_int), ("nrows", c_int), ("ncols", c_int),
+        ("rval", c_void_p), ("cval", c_void_p),
+        ("row_ptr", c_void_p), ("col_ind", c_void_p),
+        ("nnz", c_int), ("indx", c_void_p), ("jndx", c_void_p),
+        ("half_band", c_int),
+        ("ndiag", c_int), ("offsets", c_void_p),
+    ]
+
+
+class Matrix(ctypes.Structure):
+    _fields_ = [
+        ("K", UserMatrix),
+        ("which_precon", c_int),
+        ("rdiag", c_void_p), ("cdiag", c_void_p),
+        ("rseed", c_double), ("cseed", c_double * 2),
+        ("degree", c_int),
+        ("ralpha_p", c_void_p), ("calpha_p", c_void_p),
+    ]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [
+        ("has_M1", c_int), ("M1", c_void_p),
+        ("has_tol", c_int), ("tol", c_double),
+        ("has_maxit", c_int), ("maxit", c_int),
+        ("has_variant", c_int), ("variant", c_int),
+        ("x0", c_void_p), ("U0", c_void_p),
+        ("omega", c_void_p), ("n_omega", c_int),
+        ("resvec", c_void_p), ("resvec_len", c_int),
+        ("PT_raw", c_void_p),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("flag", c_int), ("relres", c_double), ("iterations", c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liborc.so"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.orc_uniform01.restype = c_double
+        _lib.orc_uniform01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        _lib.orc_r_norm.restype = c_double
+        _lib.orc_c_norm.restype = c_double
+        for p in ("r", "c"):
+            getattr(_lib, f"orc_{p}_idrs").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, POINTER(Opts), c_void_p, POINTER(Stats)]
+            getattr(_lib, f"orc_{p}_compute_alpha").restype = c_int
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class OracleMatrix:
+    """Holds numpy arrays alive behind an orc_matrix."""
+
+    def __init__(self, fmt, n, ncols=None, *, crs=None, coo=None, cds=None, dia=None, which_precon=0,
+                 degree=0, alpha=None, seed=0.0, complex_vectors=False):
+        self.keep = []
+        self.complex = complex_vectors
+        um = UserMatrix()
+        um.format, um.nrows, um.ncols = fmt, n, ncols if ncols is not None else n
+
+        def hold(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt) if dt != "F" else a
+            self.keep.append(a)
+            return a.ctypes.data
+
+        def setvals(v):
+            if np.iscomplexobj(v):
+                um.cval = hold(v, np.complex128)
+            else:
+                um.rval = hold(v, np.float64)
+
+        if fmt == CRS:
+            um.row_ptr, um.col_ind = hold(crs.row_ptr, np.int32), hold(crs.col_ind, np.int32)
+            setvals(crs.val)
+        elif fmt == COO:
+            indx, jndx, v = coo
+            um.nnz = len(indx)
+            um.indx, um.jndx = hold(indx, np.int32), hold(jndx, np.int32)
+            setvals(v)
+        elif fmt == CDS:
+            hb, band = cds
+            um.half_band = hb
+            setvals(np.asfortranarray(band).ravel(order="F"))
+        elif fmt == DIA:
+            off, vals = dia
+            um.ndiag = len(off)
+            um.offsets = hold(off, np.int32)
+            setvals(np.asfortranarray(vals).ravel(order="F"))
+        self.um = um
+        m = Matrix()
+        m.K = um
+        m.which_precon = which_precon
+        lib = load()
+        if which_precon == 1:
+            if complex_vectors:
+                d = np.empty(n, dtype=np.complex128)
+                lib.orc_c_diagonal(ctypes.byref(um), c_void_p(d.ctypes.data))
+                m.cdiag = hold(d, np.complex128)
+            else:
+                d = np.empty(n)
+                lib.orc_r_diagonal(ctypes.byref(um), c_void_p(d.ctypes.data))
+                m.rdiag = hold(d, np.float64)
+        if complex_vectors:
+            m.cseed[0], m.cseed[1] = np.real(seed), np.imag(seed)
+        else:
+            m.rseed = float(np.real(seed))
+        m.degree = degree
+        if degree > 0:
+            if complex_vectors:
+                m.calpha_p = hold(np.asarray(alpha, dtype=np.complex128), np.complex128)
+            else:
+                m.ralpha_p = hold(np.asarray(alpha, dtype=np.float64), np.float64)
+        self.m = m
+        self.n = n
+
+    def matvec(self, v):
+        dt = np.complex128 if self.complex else np.float64
+        v = np.ascontiguousarray(v, dtype=dt)
+        w = np.empty_like(v)
+        f = load().orc_c_matvec if self.complex else load().orc_r_matvec
+        f(ctypes.byref(self.m), c_void_p(v.ctypes.data), c_void_p(w.ctypes.data))
+        return w
+
+    def user_mv(self, v):
+        dt = np.complex128 if self.complex else np.float64
+        v = np.ascontiguousarray(v, dtype=dt)
+        w = np.empty(self.um.nrows, dtype=dt)
+        f = load().orc_c_user_mv if self.complex else load().orc_r_user_mv
+        f(ctypes.byref(self.um), c_void_p(v.ctypes.data), c_void_p(w.ctypes.data))
+        return w
+
+    def diagonal(self):
+        dt = np.complex128 if self.complex else np.float64
+        d = np.empty(self.um.nrows, dtype=dt)
+        f = load().orc_c_diagonal if self.complex else load().orc_r_diagonal
+        f(ctypes.byref(self.um), c_void_p(d.ctypes.data))
+        return d
+
+    def scaleback(self, y):
+        dt = np.complex128 if self.complex else np.float64
+        y = np.ascontiguousarray(y, dtype=dt)
+        x = np.empty_like(y)
+        f = load().orc_c_scaleback if self.complex else load().orc_r_scaleback
+        f(ctypes.byref(self.m), c_void_p(y.ctypes.data), c_void_p(x.ctypes.data))
+        return x
+
+
+def chebyshev(degree, foci):
+    lib = load()
+    if np.iscomplexobj(foci):
+        f = np.ascontiguousarray(foci, dtype=np.complex128)
+        a = np.empty(degree + 1, dtype=np.complex128)
+        lib.orc_c_chebyshev(c_int(degree), c_void_p(f.ctypes.data), c_void_p(a.ctypes.data))
+    else:
+        f = np.ascontiguousarray(foci, dtype=np.float64)
+        a = np.empty(degree + 1)
+        lib.orc_r_chebyshev(c_int(degree), c_void_p(f.ctypes.data), c_void_p(a.ctypes.data))
+    return a
+
+
+def neumann(degree, center):
+    lib = load()
+    a = np.empty(degree + 1)
+    lib.orc_r_neumann.argtypes = [c_int, c_double, c_void_p]
+    lib.orc_r_neumann(degree, float(center), c_void_p(a.ctypes.data))
+    return a
+
+
+def make_p(n, s, method, PT_raw=None, r=None, complex=False):
+    lib = load()
+    rows = s if method == 1 else 1
+    dt = np.complex128 if complex else np.float64
+    raw = None if PT_raw is None else np.ascontiguousarray(PT_raw, dtype=dt)
+    rr = None if r is None else np.ascontiguousarray(r, dtype=dt)
+    out = np.empty((n, rows), dtype=dt)
+    f = lib.orc_c_make_p if complex else lib.orc_r_make_p
+    f(c_int(n), c_int(s), c_int(method), c_void_p(_p(raw)), c_void_p(_p(rr)), c_void_p(out.ctypes.data))
+    return out
+
+
+def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, U0=None, omega=None,
+         resvec=False, PT_raw=None):
+    """Oracle IDRS with the reference's optional arguments; returns
+    (x, flag, relres, iterations, resvec)."""
+    lib = load()
+    cplx = A.complex
+    dt = np.complex128 if cplx else np.float64
+    n = len(b)
+    b = np.ascontiguousarray(b, dtype=dt)
+    x = np.empty_like(b)
+    o = Opts()
+    keep = []
+
+    def hold(a, order="C"):
+        a = np.asarray(a, dtype=dt)
+        a = np.asfortranarray(a) if order == "F" else np.ascontiguousarray(a)
+        keep.append(a)
+        return a.ctypes.data
+
+    if M1 is not None:
+        o.has_M1, o.M1 = 1, ctypes.addressof(M1.m)
+    if tolerance is not None:
+        o.has_tol, o.tol = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maxit, o.maxit = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    if x0 is not None:
+        o.x0 = hold(x0)
+    if U0 is not None:
+        o.U0 = hold(U0, "F")
+    if omega is not None:
+        o.omega, o.n_omega = hold(np.atleast_1d(omega)), len(np.atleast_1d(omega))
+    maxit = maximum_iterations if maximum_iterations is not None else min(2 * n, 1000)
+    rv = None
+    if resvec:
+        rv = np.zeros(maxit + 1)
+        o.resvec, o.resvec_len = rv.ctypes.data, len(rv)
+    if PT_raw is not None:
+        o.PT_raw = hold(PT_raw)
+    st = Stats()
+    f = lib.orc_c_idrs if cplx else lib.orc_r_idrs
+    rc = f(ctypes.byref(A.m), n, c_void_p(b.ctypes.data), s, ctypes.byref(o), c_void_p(x.ctypes.data), ctypes.byref(st))
+    if rc != 0:
+        raise ValueError("oracle idrs rejected its arguments")
+    if rv is not None:
+        rv = rv[: st.iterations + 1]
+    return x, st.flag, st.relres, st.iterations, rv
+
+
+def uniform01(seed, index):
+    return load().orc_uniform01(seed, index)

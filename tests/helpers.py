@@ -1,0 +1,37 @@
+"""Shared helpers of the parity tests: build the same system for the HIP
+library (idrs_amd) and for the oracle."""
+import numpy as np
+
+import idrs_amd
+import orc
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def gpu_crs(crs, diag_precon=False, poly=None, complex_poly=False):
+    K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind,
+                            **({"cval": crs.val} if np.iscomplexobj(crs.val) else {"rval": crs.val}))
+    A = idrs_amd.Matrix(K)
+    if diag_precon:
+        A.set_precon(1)
+    if poly is not None:
+        kind, degree, foci = poly
+        if complex_poly:
+            idrs_amd.complex_preconditioner(A, kind, degree, foci=foci)
+        else:
+            idrs_amd.real_preconditioner(A, kind, degree, foci=foci)
+    return A
+
+
+def oracle_crs(crs, diag_precon=False, poly=None, complex_vectors=False):
+    alpha, degree = None, 0
+    if poly is not None:
+        kind, degree, foci = poly
+        assert kind == 2
+        alpha = orc.chebyshev(degree, np.asarray(foci))
+    return orc.OracleMatrix(orc.CRS, crs.n, crs=crs, which_precon=1 if diag_precon else 0, degree=degree,
+                            alpha=alpha, complex_vectors=complex_vectors or np.iscomplexobj(crs.val))

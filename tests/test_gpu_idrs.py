@@ -1,0 +1,191 @@
+"""GPU parity of the IDR(s) solver (RIDRS / CIDRS, MODULES/IDRS/idrs.inc)
+against the oracle on the same inputs and the same raw shadow space.
+
+Bar (FP64): iteration count within +-2 of the oracle at the convergence
+tolerance; at an equal, fixed iteration count the solutions agree within
+1e-10 relative; flag semantics identical.
+"""
+import numpy as np
+import pytest
+
+import idrs_amd
+import orc
+from idrs_amd import problems as P
+
+from helpers import gpu_crs, oracle_crs, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(crs, **kw):
+    return gpu_crs(crs, **kw), oracle_crs(crs, **kw)
+
+
+@pytest.mark.parametrize("s", [1, 2, 4, 8])
+def test_finite_termination_toeplitz(gpu, s):
+    """EXAMPLES/TOEPLITZ/toeplitz.f90:111-165: IDR(s) terminates within N+N/s."""
+    crs, _, b = P.toeplitz_1d()
+    A, Ao = _pair(crs)
+    M = idrs_amd.Matrix(A.K)
+    Mo = oracle_crs(crs)
+    r = idrs_amd.idrs(A, b, s, M1=M, tolerance=1e-8, maximum_iterations=4000)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, s, M1=Mo, tolerance=1e-8, maximum_iterations=4000)
+    print(f"s={s} gpu it={r.iterations} flag={r.flag} relres={r.relres:.3e}  oracle it={io} flag={fo}")
+    assert r.flag == 0 and fo == 0
+    assert r.iterations <= 60 + 60 // s
+    assert abs(r.iterations - io) <= 2
+    assert rel(r.x, np.ones(60)) < 1e-7
+
+
+def test_c1_convection_diffusion_idr4(gpu):
+    """3-D convection-diffusion 32^3, Pe=10, b=A*1, IDR(4), tol 1e-8,
+    Gaussian shadow space (seed 0)."""
+    crs, _ = P.convection_diffusion_3d(32, 10.0)
+    b = crs.matvec(np.ones(crs.n))
+    PT = P.gaussian_shadow(crs.n, 4, seed=0)
+    A, Ao = _pair(crs)
+    r = idrs_amd.idrs(A, b, 4, tolerance=1e-8, PT_raw=PT, resvec=True)
+    xo, fo, ro, io, rvo = orc.idrs(Ao, b, 4, tolerance=1e-8, PT_raw=PT, resvec=True)
+    print(f"gpu it={r.iterations} relres={r.relres:.3e} oracle it={io} relres={ro:.3e}")
+    assert r.flag == 0 and fo == 0
+    assert abs(r.iterations - io) <= 2
+    assert r.relres <= 1e-8
+    k = min(r.iterations, io)
+    # recursive residual histories agree until the rounding noise matters
+    assert rel(r.resvec[: k // 2], rvo[: k // 2]) < 1e-8
+    # equal iteration count: solutions agree to 1e-10
+    for it in (k // 2, k):
+        g = idrs_amd.idrs(A, b, 4, tolerance=0.0, maximum_iterations=it, PT_raw=PT)
+        xo2, fo2, _, io2, _ = orc.idrs(Ao, b, 4, tolerance=0.0, maximum_iterations=it, PT_raw=PT)
+        assert g.iterations == io2 == it and g.flag == fo2 == 1
+        assert rel(g.x, xo2) < 1e-10, it
+
+
+def test_chebyshev_diag_preconditioned(gpu):
+    """IDR(4) + degree-8 Chebyshev with diagonal scaling: the 200^3 config's
+    operator at 24^3 with the same cell Peclet number Pe*h/2 = 0.124
+    (Pe = 6.2), so that the foci (0.15, 1.85) enclose the spectrum of D^-1 K."""
+    crs, _ = P.convection_diffusion_3d(24, 6.2, reaction=1.0)
+    b = np.random.default_rng(0).uniform(-1, 1, crs.n)
+    poly = (2, 8, [0.15, 1.85])
+    A = gpu_crs(crs, diag_precon=True, poly=poly)
+    Ao = oracle_crs(crs, diag_precon=True, poly=poly)
+    PT = P.gaussian_shadow(crs.n, 4, seed=0)
+    r = idrs_amd.idrs(A, b, 4, tolerance=1e-8, maximum_iterations=2000, PT_raw=PT)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, 4, tolerance=1e-8, maximum_iterations=2000, PT_raw=PT)
+    print(f"gpu it={r.iterations} relres={r.relres:.3e} oracle it={io} relres={ro:.3e}")
+    assert r.flag == 0 and fo == 0 and abs(r.iterations - io) <= 2
+    g = idrs_amd.idrs(A, b, 4, tolerance=0.0, maximum_iterations=io, PT_raw=PT)
+    assert rel(g.x, xo if io == r.iterations else orc.idrs(Ao, b, 4, tolerance=0.0, maximum_iterations=io, PT_raw=PT)[0]) < 1e-10
+    # scale back to the unpreconditioned system: ||b - K x|| / ||b||
+    x = idrs_amd.scaleback(A, b, r.x)
+    assert rel(crs.matvec(x), b) < 2e-8
+
+
+def test_default_shadow_space_matches_oracle(gpu):
+    """Without PT_raw both sides draw the same uniform stream (seed 1)."""
+    crs, _ = P.convection_diffusion_3d(10, 5.0)
+    b = crs.matvec(np.ones(crs.n))
+    A, Ao = _pair(crs)
+    r = idrs_amd.idrs(A, b, 3, tolerance=1e-10)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, 3, tolerance=1e-10)
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= 2
+
+
+def test_bicgstab_variant(gpu):
+    crs, _, b = P.toeplitz_1d()
+    A, Ao = _pair(crs)
+    r = idrs_amd.idrs(A, b, 1, tolerance=1e-8, maximum_iterations=4000, variant=2, resvec=True)
+    xo, fo, ro, io, rvo = orc.idrs(Ao, b, 1, tolerance=1e-8, maximum_iterations=4000, variant=2, resvec=True)
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= 2
+    k = min(r.iterations, io) // 2
+    assert rel(r.resvec[:k], rvo[:k]) < 1e-6
+
+
+def test_flags_forced_from_host(gpu):
+    crs, _, b = P.toeplitz_1d()
+    A, Ao = _pair(crs)
+    # flag 1: maximum_iterations reached
+    r = idrs_amd.idrs(A, b, 2, maximum_iterations=1)
+    assert r.flag == 1 and r.iterations == 1
+    # flag 0 at iteration 0: x0 is the exact solution
+    r = idrs_amd.idrs(A, b, 2, x0=np.ones(60))
+    assert r.flag == 0 and r.iterations == 0
+    # flag 3: user omega = 0 triggers the |om| <= epsilon breakdown
+    r = idrs_amd.idrs(A, b, 2, omega=[0.0], maximum_iterations=100)
+    xo, fo, _, io, _ = orc.idrs(Ao, b, 2, omega=[0.0], maximum_iterations=100)
+    assert r.flag == fo == 3 and r.iterations == io
+    # user omegas cycle (idrs.inc:274-278)
+    r = idrs_amd.idrs(A, b, 2, omega=[0.5, 0.25], tolerance=1e-8, maximum_iterations=4000)
+    xo, fo, _, io, _ = orc.idrs(Ao, b, 2, omega=[0.5, 0.25], tolerance=1e-8, maximum_iterations=4000)
+    assert r.flag == fo and abs(r.iterations - io) <= 2
+    # negative tolerance is rejected (idrs.inc:31)
+    with pytest.raises(ValueError):
+        idrs_amd.idrs(A, b, 2, tolerance=-1.0)
+    # short resvec is rejected (idrs.inc:76-77)
+    o = idrs_amd._lib.idrs_opts_t()
+    o.has_maximum_iterations, o.maximum_iterations = 1, 10
+    rv = np.zeros(5)
+    o.resvec, o.resvec_len = rv.ctypes.data, 5
+    x = np.zeros(60)
+    st = idrs_amd._lib.idrs_stats_t()
+    import ctypes
+    rc = idrs_amd.load_library().idrs_idrs_f64(A.handle, 60, b.ctypes.data, 2, ctypes.byref(o), x.ctypes.data, ctypes.byref(st))
+    assert rc == idrs_amd._lib.IDRS_ERR_ARG
+
+
+def test_initial_search_space_and_x0(gpu):
+    crs, _ = P.convection_diffusion_3d(10, 5.0)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(crs.n)
+    A, Ao = _pair(crs)
+    U0 = rng.standard_normal((crs.n, 3))
+    x0 = rng.standard_normal(crs.n)
+    PT = P.gaussian_shadow(crs.n, 3, seed=1)
+    r = idrs_amd.idrs(A, b, 3, tolerance=1e-9, U0=U0, x0=x0, PT_raw=PT)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, 3, tolerance=1e-9, U0=U0, x0=x0, PT_raw=PT)
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= 2
+    g = idrs_amd.idrs(A, b, 3, tolerance=0.0, maximum_iterations=15, U0=U0, x0=x0, PT_raw=PT)
+    o = orc.idrs(Ao, b, 3, tolerance=0.0, maximum_iterations=15, U0=U0, x0=x0, PT_raw=PT)
+    assert rel(g.x, o[0]) < 1e-10
+
+
+def test_m1_diagonal_preconditioner(gpu):
+    """M1 present with a diagonal: t = om*(v/M1) branches (idrs.inc:169-190, 266-267)."""
+    crs, _ = P.convection_diffusion_3d(10, 20.0, reaction=2.0)
+    b = np.random.default_rng(4).standard_normal(crs.n)
+    A, Ao = _pair(crs)
+    M1 = gpu_crs(crs, diag_precon=True)
+    M1o = oracle_crs(crs, diag_precon=True)
+    PT = P.gaussian_shadow(crs.n, 2, seed=2)
+    r = idrs_amd.idrs(A, b, 2, M1=M1, tolerance=1e-9, PT_raw=PT)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, 2, M1=M1o, tolerance=1e-9, PT_raw=PT)
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= 2
+    g = idrs_amd.idrs(A, b, 2, M1=M1, tolerance=0.0, maximum_iterations=20, PT_raw=PT)
+    o = orc.idrs(Ao, b, 2, M1=M1o, tolerance=0.0, maximum_iterations=20, PT_raw=PT)
+    assert rel(g.x, o[0]) < 1e-10
+
+
+@pytest.mark.parametrize("s", [1, 4, 16])
+def test_complex_toeplitz_accuracy(gpu, s):
+    """Complex Toeplitz accuracy test, tol 1e-12 (toeplitz.f90:41-70), DIA
+    storage.  This problem stagnates for hundreds of iterations before it
+    reaches 1e-12, so rounding moves the stopping iteration by a few steps:
+    both sides must reach the accuracy, with iteration counts within 3%."""
+    n = 200
+    off = np.array([-1, 0, 2, 3], dtype=np.int32)
+    cv = np.array([3.6j, 4.0, 1.0, 0.7])
+    vals = np.tile(cv, (n, 1)).astype(np.complex128)
+    i = np.arange(n)
+    for d, o in enumerate(off):
+        vals[(i + o < 0) | (i + o >= n), d] = 0
+    K = idrs_amd.dia_format(n, n, off, cvals=vals)
+    A = idrs_amd.Matrix(K)
+    Ao = orc.OracleMatrix(orc.DIA, n, dia=(off, vals), complex_vectors=True)
+    b = np.full(n, 1j)
+    r = idrs_amd.idrs(A, b, s, M1=idrs_amd.Matrix(K), tolerance=1e-12, maximum_iterations=4000)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, s, M1=orc.OracleMatrix(orc.DIA, n, dia=(off, vals), complex_vectors=True),
+                                 tolerance=1e-12, maximum_iterations=4000)
+    print(f"s={s} gpu it={r.iterations} relres={r.relres:.2e} oracle it={io} relres={ro:.2e}")
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= max(2, 0.03 * io)
+    assert r.relres <= 1e-12 and ro <= 1e-12

@@ -1,0 +1,172 @@
+"""GPU parity of the sparse formats, the operator and MAKE_P against the oracle.
+
+Floating-point bar: the GPU sums each row in stored order with fused
+multiply-adds, the oracle rounds every product and sum separately, so each
+entry must agree within the rounding bound 2*(k+1)*eps*sum_j |a_ij x_j|
+(k = entries in the row); normwise checks use 1e-13.
+"""
+import numpy as np
+import pytest
+
+import idrs_amd
+import orc
+from idrs_amd import problems as P
+
+from helpers import gpu_crs, oracle_crs, rel
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(np.float64).eps
+
+
+def _bound(crs, x):
+    rows = np.repeat(np.arange(crs.n), np.diff(crs.row_ptr))
+    mag = np.zeros(crs.n)
+    np.add.at(mag, rows, np.abs(crs.val) * np.abs(x[crs.col_ind - 1]))
+    k = np.diff(crs.row_ptr)
+    return 2.0 * (k + 1) * EPS * mag + 1e-300
+
+
+def _ragged(n, seed, long_rows=()):
+    """Random CRS with empty rows, short rows and a few very long rows (more
+    nonzeros than one LDS slab holds)."""
+    rng = np.random.default_rng(seed)
+    lens = np.minimum(rng.integers(0, 12, size=n), n)
+    lens[rng.integers(0, n, size=max(1, n // 50))] = 0
+    for r in long_rows:
+        lens[r] = min(n, 3000)
+    cols = [np.sort(rng.choice(n, size=l, replace=False)) for l in lens]
+    rp = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int32)
+    ci = (np.concatenate(cols) + 1).astype(np.int32) if lens.sum() else np.zeros(0, np.int32)
+    val = rng.standard_normal(int(lens.sum()))
+    return P.Crs(n, rp, ci, val)
+
+
+@pytest.mark.parametrize("n,seed,long_rows", [(1, 0, ()), (257, 1, ()), (5000, 2, (7, 4000)), (70001, 3, ())])
+def test_crs_spmv_matches_oracle(gpu, n, seed, long_rows):
+    crs = _ragged(n, seed, long_rows)
+    x = np.random.default_rng(seed + 10).standard_normal(n)
+    K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    w = K * x
+    ref = oracle_crs(crs).user_mv(x)
+    assert np.all(np.abs(w - ref) <= _bound(crs, x))
+
+
+def test_crs_pattern_and_complex(gpu):
+    crs = _ragged(3000, 5)
+    x = np.random.default_rng(1).standard_normal(3000) + 1j * np.random.default_rng(2).standard_normal(3000)
+    # pattern matrix (rval absent): values 1 (user_module.f90:408-414)
+    K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind)
+    pat = P.Crs(crs.n, crs.row_ptr, crs.col_ind, np.ones(crs.nnz))
+    xr = x.real.copy()
+    assert np.all(np.abs(K * xr - pat.matvec(xr)) <= _bound(pat, xr))
+    # real matrix times complex vector (ccrs_mv rval branch, user_module.f90:318-324)
+    K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    ref = orc.OracleMatrix(orc.CRS, crs.n, crs=crs, complex_vectors=True).user_mv(x)
+    assert rel(K * x, ref) < 1e-13
+    # complex matrix
+    cv = crs.val * (1 + 0.5j)
+    ccrs = P.Crs(crs.n, crs.row_ptr, crs.col_ind, cv)
+    K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, cval=cv)
+    ref = orc.OracleMatrix(orc.CRS, crs.n, crs=ccrs, complex_vectors=True).user_mv(x)
+    assert rel(K * x, ref) < 1e-13
+
+
+def test_coo_order_and_duplicates(gpu):
+    """COO entries in arbitrary order with duplicates: each row is summed in
+    list order, like the sequential scatter loop (user_module.f90:542-558)."""
+    rng = np.random.default_rng(7)
+    n, nnz = 4000, 30000
+    indx = rng.integers(1, n + 1, size=nnz).astype(np.int32)
+    jndx = rng.integers(1, n + 1, size=nnz).astype(np.int32)
+    jndx[:100] = jndx[100:200]
+    indx[:100] = indx[100:200]                      # duplicate entries
+    val = rng.standard_normal(nnz)
+    x = rng.standard_normal(n)
+    K = idrs_amd.coo_format(n, n, nnz, indx, jndx, rval=val)
+    w = K * x
+    ref = orc.OracleMatrix(orc.COO, n, coo=(indx, jndx, val)).user_mv(x)
+    # exact order is kept, so the only difference is fused vs separate rounding
+    order = np.argsort(indx, kind="stable")
+    rp = np.concatenate([[1], 1 + np.cumsum(np.bincount(indx - 1, minlength=n))]).astype(np.int32)
+    crs = P.Crs(n, rp, jndx[order], val[order])
+    assert np.all(np.abs(w - ref) <= _bound(crs, x))
+
+
+def test_cds_band_and_dia(gpu):
+    rng = np.random.default_rng(11)
+    n, hb = 3001, 17
+    band = rng.standard_normal((n, 2 * hb + 1))
+    x = rng.standard_normal(n)
+    K = idrs_amd.cds_format(n, n, hb, rband=band)
+    ref = orc.OracleMatrix(orc.CDS, n, cds=(hb, band)).user_mv(x)
+    assert rel(K * x, ref) < 1e-14
+    # offset-list storage of a 7-point stencil equals its CRS product
+    crs, dia = P.convection_diffusion_3d(12, 10.0, reaction=1.0)
+    x = rng.standard_normal(crs.n)
+    Kd = idrs_amd.dia_format(crs.n, crs.n, dia.offsets, rvals=dia.vals)
+    ref = orc.OracleMatrix(orc.DIA, crs.n, dia=(dia.offsets, dia.vals)).user_mv(x)
+    assert rel(Kd * x, ref) < 1e-14
+    assert rel(Kd * x, crs.matvec(x)) < 1e-14
+
+
+def test_complex_toeplitz_dia(gpu):
+    """The reference's complex Toeplitz operator (EXAMPLES/TOEPLITZ/toeplitz.f90:41-42)."""
+    n = 200
+    off = np.array([-1, 0, 2, 3], dtype=np.int32)
+    cv = np.array([3.6j, 4.0, 1.0, 0.7])
+    vals = np.tile(cv, (n, 1)).astype(np.complex128)
+    for d, o in enumerate(off):
+        i = np.arange(n)
+        vals[(i + o < 0) | (i + o >= n), d] = 0
+    K = idrs_amd.dia_format(n, n, off, cvals=vals)
+    x = np.random.default_rng(0).standard_normal(n) + 1j
+    dense = np.zeros((n, n), dtype=np.complex128)
+    for d, o in enumerate(off):
+        dense += np.diag(np.full(n - abs(o), cv[d]), o)
+    assert rel(K * x, dense @ x) < 1e-14
+
+
+def test_diagonal_zero_rule(gpu):
+    crs = _ragged(500, 3)
+    K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    d = idrs_amd.real_diagonal(K)
+    ref = orc.OracleMatrix(orc.CRS, crs.n, crs=crs).diagonal()
+    assert np.array_equal(d, ref)
+    assert np.all(d != 0)
+
+
+@pytest.mark.parametrize("degree", [0, 1, 4, 8])
+def test_operator_chebyshev_diag(gpu, degree):
+    """A*v = K D^-1 p(K D^-1) v with diagonal scaling and the Chebyshev
+    polynomial (matrix_module.f90:135-304)."""
+    crs, _ = P.convection_diffusion_3d(16, 50.0, reaction=1.0)
+    foci = [0.15, 1.85]
+    poly = (2, degree, foci) if degree else None
+    A = gpu_crs(crs, diag_precon=True, poly=poly)
+    Ao = oracle_crs(crs, diag_precon=True, poly=poly)
+    v = np.random.default_rng(degree).standard_normal(crs.n)
+    assert rel(A * v, Ao.matvec(v)) < 1e-12
+    # operator(/) and scaleback
+    assert rel(v / A, v / idrs_amd.real_diagonal(A.K)) < 1e-15
+    assert rel(idrs_amd.scaleback(A, None, v), Ao.scaleback(v)) < 1e-12
+    if degree:
+        assert np.allclose(A.alpha.real, orc.chebyshev(degree, np.array(foci)), rtol=0, atol=0)
+
+
+def test_make_p_orthonormal(gpu):
+    n, s = 10007, 6
+    raw = P.gaussian_shadow(n, s, seed=0)
+    PTg = idrs_amd.make_p(n, s, 1, PT_raw=raw)
+    PTo = orc.make_p(n, s, 1, PT_raw=raw)
+    assert rel(PTg, PTo) < 1e-13
+    assert np.allclose(PTg.T @ PTg, np.eye(s), atol=1e-13)
+    # default stream: the library and the oracle draw the same raw numbers
+    PTg = idrs_amd.make_p(n, s, 1)
+    PTo = orc.make_p(n, s, 1)
+    assert rel(PTg, PTo) < 1e-13
+    # complex, and the BiCGSTAB shadow vector conj(r)
+    PTg = idrs_amd.make_p(n, s, 1, complex=True)
+    PTo = orc.make_p(n, s, 1, complex=True)
+    assert rel(PTg, PTo) < 1e-13
+    r = raw[:, 0] + 1j * raw[:, 1]
+    assert np.array_equal(idrs_amd.make_p(n, 1, 2, r=r, complex=True)[:, 0], np.conj(r))
