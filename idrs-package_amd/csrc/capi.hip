@@ -371,6 +371,53 @@ int idrs_complex_preconditioner(idrs_matrix_t A, int preconditioner, int degree,
                                 const double *center, const double *H, int ldh, int ncol) {
     return set_poly(A, preconditioner, degree, foci, center, H, ldh, ncol, true);
 }
+int idrs_matrix_set_alpha(idrs_matrix_t A, int degree, const double *alpha) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        need(degree >= 0 && (degree == 0 || alpha), "bad polynomial");
+        A->op.degree = degree;
+        A->op.poly_kind = degree ? POLY_HESSENBERG : POLY_NONE;
+        A->op.alpha.assign((size_t)degree + 1, std::complex<double>(1.0));
+        if (degree)
+            for (int i = 0; i <= degree; i++) A->op.alpha[i] = {alpha[2 * i], alpha[2 * i + 1]};
+    });
+}
+
+static int coeffs_entry(int kind, int degree, const double *foci, const double *center, const double *H, int ldh,
+                        int ncol, int is_complex, double *alpha) {
+    return guarded([&] {
+        need(degree >= 0 && alpha, "bad arguments");
+        std::complex<double> f[2], c;
+        std::vector<std::complex<double>> h, a((size_t)degree + 1);
+        if (foci) {
+            f[0] = {foci[0], foci[1]};
+            f[1] = {foci[2], foci[3]};
+        }
+        if (center) c = {center[0], center[1]};
+        if (H) {
+            need(ldh > 0 && ncol > 0, "bad H dimensions");
+            h.resize((size_t)ldh * ncol);
+            for (size_t q = 0; q < h.size(); q++) h[q] = {H[2 * q], H[2 * q + 1]};
+        }
+        if (degree == 0) a[0] = 1.0;
+        else poly_coeffs(kind, degree, foci ? f : nullptr, center ? &c : nullptr, H ? h.data() : nullptr, ldh, ncol,
+                         is_complex != 0, a.data());
+        for (int i = 0; i <= degree; i++) {
+            alpha[2 * i] = a[i].real();
+            alpha[2 * i + 1] = is_complex ? a[i].imag() : 0.0;
+        }
+    });
+}
+int idrs_chebyshev_coeffs(int degree, const double *foci, int is_complex, double *alpha) {
+    return coeffs_entry(POLY_CHEBYSHEV, degree, foci, nullptr, nullptr, 0, 0, is_complex, alpha);
+}
+int idrs_neumann_coeffs(int degree, const double *center, int is_complex, double *alpha) {
+    return coeffs_entry(POLY_NEUMANN, degree, nullptr, center, nullptr, 0, 0, is_complex, alpha);
+}
+int idrs_compute_alpha_coeffs(int degree, const double *H, int ldh, int ncol, int is_complex, double *alpha) {
+    return coeffs_entry(POLY_HESSENBERG, degree, nullptr, nullptr, H, ldh, ncol, is_complex, alpha);
+}
+
 int idrs_matrix_get_alpha(idrs_matrix_t A, double *alpha, int maxlen, int *degree) {
     return guarded([&] {
         need(A != nullptr, "null handle");

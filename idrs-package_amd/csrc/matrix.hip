@@ -815,34 +815,19 @@ void neumann(int degree, std::complex<double> center, std::complex<double> *alph
     neumann_t<std::complex<double>>(degree, center, alpha);
 }
 
-// real_preconditioner / complex_preconditioner (matrix_module.f90:551-722)
-void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> *foci,
-                 const std::complex<double> *center, const std::complex<double> *H, int ldh, int ncol,
-                 bool complex_poly) {
-    if (degree < 0) throw std::invalid_argument("negative degree");
-    op.poly_kind = kind;
-    op.degree = degree;
-    op.complex_poly = complex_poly;
-    op.alpha.assign((size_t)degree + 1, std::complex<double>(0.0));
-    if (kind == POLY_NONE) {
-        // the reference leaves alpha_p unallocated; polpre then only looks
-        // at degree, which the caller sets.  A degree without coefficients
-        // would dereference nothing in the reference: treat as degree 0.
-        op.degree = 0;
-        op.alpha.assign(1, std::complex<double>(1.0));
-        return;
-    }
-    const std::complex<double> sd = op.seed;
+// coefficient computation behind real_preconditioner / complex_preconditioner
+// (matrix_module.f90:551-722, 851-1057), in real arithmetic unless complex_poly
+void poly_coeffs(int kind, int degree, const std::complex<double> *foci, const std::complex<double> *center,
+                 const std::complex<double> *H, int ldh, int ncol, bool complex_poly, std::complex<double> *out) {
     if (!complex_poly) {
         std::vector<double> a((size_t)degree + 1);
         if (kind == POLY_NEUMANN) {
-            double c = center ? center->real() - sd.real() : 1.0;
-            neumann_t<double>(degree, c, a.data());
+            neumann_t<double>(degree, center ? center->real() : 1.0, a.data());
         } else if (kind == POLY_CHEBYSHEV) {
             double f[2] = {0.25, 1.75};
             if (foci) {
-                f[0] = foci[0].real() - sd.real();
-                f[1] = foci[1].real() - sd.real();
+                f[0] = foci[0].real();
+                f[1] = foci[1].real();
             }
             chebyshev_t<double>(degree, f, a.data());
         } else if (kind == POLY_HESSENBERG) {
@@ -853,27 +838,50 @@ void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> 
         } else {
             throw std::invalid_argument("unknown polynomial kind");
         }
-        for (int i = 0; i <= degree; i++) op.alpha[i] = a[i];
+        for (int i = 0; i <= degree; i++) out[i] = a[i];
     } else {
-        std::vector<std::complex<double>> a((size_t)degree + 1);
         if (kind == POLY_NEUMANN) {
-            std::complex<double> c = center ? *center - sd : std::complex<double>(1.0);
-            neumann_t<std::complex<double>>(degree, c, a.data());
+            neumann_t<std::complex<double>>(degree, center ? *center : std::complex<double>(1.0), out);
         } else if (kind == POLY_CHEBYSHEV) {
             std::complex<double> f[2] = {0.25, 1.75};
             if (foci) {
-                f[0] = foci[0] - sd;
-                f[1] = foci[1] - sd;
+                f[0] = foci[0];
+                f[1] = foci[1];
             }
-            chebyshev_t<std::complex<double>>(degree, f, a.data());
+            chebyshev_t<std::complex<double>>(degree, f, out);
         } else if (kind == POLY_HESSENBERG) {
             if (!H) throw std::invalid_argument("Hessenberg preconditioner needs H");
-            compute_alpha_t<std::complex<double>>(degree, H, ldh, ncol, a.data());
+            compute_alpha_t<std::complex<double>>(degree, H, ldh, ncol, out);
         } else {
             throw std::invalid_argument("unknown polynomial kind");
         }
-        op.alpha = a;
     }
+}
+
+// real_preconditioner / complex_preconditioner (matrix_module.f90:551-722):
+// foci and center are shifted by the seed before the coefficients are formed.
+void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> *foci,
+                 const std::complex<double> *center, const std::complex<double> *H, int ldh, int ncol,
+                 bool complex_poly) {
+    if (degree < 0) throw std::invalid_argument("negative degree");
+    op.poly_kind = kind;
+    op.complex_poly = complex_poly;
+    if (kind == POLY_NONE) {
+        op.degree = 0;
+        op.alpha.assign(1, std::complex<double>(1.0));
+        return;
+    }
+    const std::complex<double> sd = complex_poly ? op.seed : std::complex<double>(op.seed.real(), 0.0);
+    std::complex<double> f[2], c;
+    if (foci) {
+        f[0] = foci[0] - sd;
+        f[1] = foci[1] - sd;
+    }
+    if (center) c = *center - sd;
+    std::vector<std::complex<double>> a((size_t)degree + 1);
+    poly_coeffs(kind, degree, foci ? f : nullptr, center ? &c : nullptr, H, ldh, ncol, complex_poly, a.data());
+    op.degree = degree;
+    op.alpha = a;
 }
 
 // explicit instantiations

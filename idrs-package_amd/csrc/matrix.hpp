@@ -135,6 +135,8 @@ void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> 
                  const std::complex<double> *center, const std::complex<double> *H, int ldh, int ncol,
                  bool complex_poly);
 void op_make_lowprec(Operator &op);
+void poly_coeffs(int kind, int degree, const std::complex<double> *foci, const std::complex<double> *center,
+                 const std::complex<double> *H, int ldh, int ncol, bool complex_poly, std::complex<double> *alpha);
 
 // polynomial coefficient computation (host, restating matrix_module)
 void compute_alpha(int degree, const std::complex<double> *H, int ldh, int ncol,

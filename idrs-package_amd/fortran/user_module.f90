@@ -1,0 +1,303 @@
+!
+! user_module: drop-in for the reference's MODULES/PP_MATRIX/user_module.f90
+! public interface on the hot path.  type(user_matrix) keeps the reference's
+! scalar components and holds a handle to device storage in libidrs_hip.so;
+! the products run in the library's HIP kernels.
+!
+!   crs_format / coo_format / cds_format    user_module.f90:829-958
+!   operator(*)  (ruser_mv / cuser_mv)       user_module.f90:197-241
+!   operator(/)  (ruser_precon / cuser_precon, diagonal scaling)  :243-283
+!   real_diagonal / complex_diagonal         user_module.f90:960-1161
+!   REAL_PRECON / COMPLEX_PRECON(1, K)       user_module.f90:1327-1381, 1465-1518
+!
+! Storage formats: matrix_format 2 CRS, 3 COO, 4 CDS, 5 DIA (offset list,
+! an extension for wide sparse bands).  Dense storage and the dense/band LU
+! preconditioners (which_precon 2, 3) are not offered: they stop with a
+! message.  Device storage is released by user_matrix_free (the reference's
+! allocatable components are freed with the object; these handles are not).
+!
+module user_module
+
+   use precision_module
+   use idrs_capi
+   use iso_c_binding
+
+   implicit none
+
+   private
+   public :: user_matrix, operator(*), operator(/), crs_format, coo_format, cds_format, dia_format, &
+             dense_format, real_diagonal, complex_diagonal, REAL_PRECON, COMPLEX_PRECON, user_matrix_free
+
+   type user_matrix
+      integer                        :: matrix_format = 0
+      integer                        :: which_precon = 0
+      integer                        :: nrows = 0           ! Number of rows
+      integer                        :: ncols = 0           ! Number columns
+      integer                        :: nnz = 0             ! Number of stored nonzeros
+      integer                        :: half_band = 0       ! CDS half bandwidth
+      logical                        :: is_complex = .false.
+! Diagonal preconditioner (which_precon = 1): band(:,0) as in the reference
+      real(kind=rp), allocatable     :: rband(:,:)
+      complex(kind=cp), allocatable  :: cband(:,:)
+      logical                        :: has_shift = .false.
+      real(kind=rp)                  :: rshift = 0._rp
+      complex(kind=cp)               :: cshift = (0._cp,0._cp)
+! Device objects: h is the stored matrix, hp the diagonal preconditioner
+! operator built from h (which_precon = 1)
+      type(c_ptr)                    :: h = c_null_ptr
+      type(c_ptr)                    :: hp = c_null_ptr
+   end type user_matrix
+
+   INTERFACE OPERATOR(*)
+      module procedure ruser_mv, cuser_mv
+   END INTERFACE
+
+   INTERFACE OPERATOR(/)
+      module procedure ruser_precon, cuser_precon
+   END INTERFACE
+
+contains
+
+   function ruser_mv( A, v ) result(w)
+      type(user_matrix), intent(in)     :: A
+      real(kind=rp), intent(in)         :: v(:)
+      real(kind=rp)                     :: w(A%nrows)
+      if ( .not. c_associated(A%h) ) stop 'No matrix format specified.'
+      call idrs_check( idrs_user_mv_f64(A%h, v, w), 'user_mv' )
+   end function ruser_mv
+
+   function cuser_mv( A, v ) result(w)
+      type(user_matrix), intent(in)     :: A
+      complex(kind=cp), intent(in)      :: v(:)
+      complex(kind=cp)                  :: w(A%nrows)
+      if ( .not. c_associated(A%h) ) stop 'Illegal value for matrix format'
+      call idrs_check( idrs_user_mv_c128(A%h, v, w), 'user_mv' )
+   end function cuser_mv
+
+   function ruser_precon( v, A ) result(w)
+      type(user_matrix), intent(in)     :: A
+      real(kind=rp), intent(in)         :: v(:)
+      real(kind=rp)                     :: w(size(v))
+      if ( A%which_precon == 1 ) then
+         call idrs_check( idrs_precon_f64(A%hp, v, w), 'user_precon' )
+      else
+         w = v
+      end if
+   end function ruser_precon
+
+   function cuser_precon( v, A ) result(w)
+      type(user_matrix), intent(in)     :: A
+      complex(kind=cp), intent(in)      :: v(:)
+      complex(kind=cp)                  :: w(size(v))
+      if ( A%which_precon == 1 ) then
+         call idrs_check( idrs_precon_c128(A%hp, v, w), 'user_precon' )
+      else
+         w = v
+      end if
+   end function cuser_precon
+
+   function crs_format( nrows, ncols, row_ptr, col_ind, rval, ival, cval ) result(A)
+      type(user_matrix)                      :: A
+      integer                                :: nrows, ncols
+      integer                                :: row_ptr(:), col_ind(:)
+      real(kind=rp), optional, intent(in)    :: rval(:)
+      integer, optional, intent(in)          :: ival(:)
+      complex(kind=cp), optional, intent(in) :: cval(:)
+      real(kind=rp), allocatable             :: tmp(:)
+
+      A%matrix_format = 2
+      A%nrows = nrows
+      A%ncols = ncols
+      A%nnz = row_ptr(nrows+1) - 1
+      if ( present(cval) ) then
+         A%is_complex = .true.
+         call idrs_check( idrs_crs_format_c128(nrows, ncols, row_ptr, col_ind, cval, A%h), 'crs_format' )
+      elseif ( present(rval) ) then
+         call idrs_check( idrs_crs_format_f64(nrows, ncols, row_ptr, col_ind, rval, A%h), 'crs_format' )
+      elseif ( present(ival) ) then
+! integer values are stored as real values (user_module.f90:860)
+         tmp = real(ival, kind=rp)
+         call idrs_check( idrs_crs_format_f64(nrows, ncols, row_ptr, col_ind, tmp, A%h), 'crs_format' )
+      else
+         call idrs_check( idrs_crs_format_f64(nrows, ncols, row_ptr, col_ind, A=A%h), 'crs_format' )
+      end if
+   end function crs_format
+
+   function coo_format( nrows, ncols, nnz, indx, jndx, rval, ival, cval ) result(A)
+      type(user_matrix)                      :: A
+      integer                                :: nrows, ncols, nnz
+      integer                                :: indx(:), jndx(:)
+      real(kind=rp), optional, intent(in)    :: rval(:)
+      integer, optional, intent(in)          :: ival(:)
+      complex(kind=cp), optional, intent(in) :: cval(:)
+      real(kind=rp), allocatable             :: tmp(:)
+
+      A%matrix_format = 3
+      A%nrows = nrows
+      A%ncols = ncols
+      A%nnz   = nnz
+      if ( present(cval) ) then
+         A%is_complex = .true.
+         call idrs_check( idrs_coo_format_c128(nrows, ncols, nnz, indx, jndx, cval, A%h), 'coo_format' )
+      elseif ( present(rval) ) then
+         call idrs_check( idrs_coo_format_f64(nrows, ncols, nnz, indx, jndx, rval, A%h), 'coo_format' )
+      elseif ( present(ival) ) then
+         tmp = real(ival(1:nnz), kind=rp)
+         call idrs_check( idrs_coo_format_f64(nrows, ncols, nnz, indx, jndx, tmp, A%h), 'coo_format' )
+      else
+         call idrs_check( idrs_coo_format_f64(nrows, ncols, nnz, indx, jndx, A=A%h), 'coo_format' )
+      end if
+   end function coo_format
+
+   function cds_format( nrows, ncols, half_band, rband, iband, cband ) result(A)
+      type(user_matrix)                      :: A
+      integer                                :: nrows, ncols, half_band
+      real(kind=rp), optional                :: rband(nrows,-half_band:half_band)
+      integer, optional                      :: iband(nrows,-half_band:half_band)
+      complex(kind=cp), optional             :: cband(nrows,-half_band:half_band)
+      real(kind=rp), allocatable             :: tmp(:,:)
+
+      A%matrix_format = 4
+      A%nrows = nrows
+      A%ncols = ncols
+      A%half_band = half_band
+      A%nnz = nrows*(2*half_band+1)
+      if ( present(cband) ) then
+         A%is_complex = .true.
+         call idrs_check( idrs_cds_format_c128(nrows, ncols, half_band, cband, A%h), 'cds_format' )
+      elseif ( present(rband) ) then
+         call idrs_check( idrs_cds_format_f64(nrows, ncols, half_band, rband, A%h), 'cds_format' )
+      elseif ( present(iband) ) then
+         tmp = real(iband, kind=rp)
+         call idrs_check( idrs_cds_format_f64(nrows, ncols, half_band, tmp, A%h), 'cds_format' )
+      else
+         call idrs_check( idrs_cds_format_f64(nrows, ncols, half_band, A=A%h), 'cds_format' )
+      end if
+   end function cds_format
+
+! Offset-list diagonal storage: vals(nrows, ndiag) for strictly ascending
+! offsets; entry (i, d) multiplies v(i + offsets(d)).
+   function dia_format( nrows, ncols, offsets, rvals, cvals ) result(A)
+      type(user_matrix)                      :: A
+      integer                                :: nrows, ncols
+      integer                                :: offsets(:)
+      real(kind=rp), optional                :: rvals(:,:)
+      complex(kind=cp), optional             :: cvals(:,:)
+
+      A%matrix_format = 5
+      A%nrows = nrows
+      A%ncols = ncols
+      A%nnz = nrows*size(offsets)
+      if ( present(cvals) ) then
+         A%is_complex = .true.
+         call idrs_check( idrs_dia_format_c128(nrows, ncols, size(offsets), offsets, cvals, A%h), 'dia_format' )
+      elseif ( present(rvals) ) then
+         call idrs_check( idrs_dia_format_f64(nrows, ncols, size(offsets), offsets, rvals, A%h), 'dia_format' )
+      else
+         call idrs_check( idrs_dia_format_f64(nrows, ncols, size(offsets), offsets, A=A%h), 'dia_format' )
+      end if
+   end function dia_format
+
+   function dense_format( rmat, imat, cmat ) result(A)
+      type(user_matrix)                      :: A
+      real(kind=rp), optional                :: rmat(:,:)
+      integer, optional                      :: imat(:,:)
+      complex(kind=cp), optional             :: cmat(:,:)
+      error stop 'dense_format: dense storage is not offered by the GPU library, use crs_format'
+   end function dense_format
+
+   function real_diagonal( A ) result(D)
+      type(user_matrix), intent(in)         :: A
+      real(kind=rp)                         :: D(A%nrows)
+      call idrs_check( idrs_real_diagonal(A%h, D), 'real_diagonal' )
+   end function real_diagonal
+
+   function complex_diagonal( A ) result(D)
+      type(user_matrix), intent(in)         :: A
+      complex(kind=cp)                      :: D(A%nrows)
+      call idrs_check( idrs_complex_diagonal(A%h, D), 'complex_diagonal' )
+   end function complex_diagonal
+
+   function REAL_PRECON( which_precon, K, M, C, shift ) result(P)
+!
+! which_precon = 0 : no preconditioner
+! which_precon = 1 : diagonal preconditioner D = diag(K) [- shift]
+!
+      integer, intent(in)                              :: which_precon
+      type(user_matrix), intent(in)                    :: K
+      type(user_matrix), intent(in), optional          :: M
+      type(user_matrix), intent(in), optional          :: C
+      type(user_matrix)                                :: P
+      real(kind=rp), optional                          :: shift
+
+      P%matrix_format = 0
+      P%which_precon = which_precon
+      P%nrows = K%nrows
+      P%ncols = K%ncols
+      if ( present(M) .or. present(C) ) &
+         error stop 'REAL_PRECON: mass and damping matrices are not offered by the GPU library'
+      if ( which_precon == 1 ) then
+         P%matrix_format = 4
+         P%h = K%h
+         call idrs_check( idrs_matrix_create(K%h, P%hp), 'REAL_PRECON' )
+         if ( present(shift) ) then
+            P%has_shift = .true.
+            P%rshift = shift
+            call idrs_check( idrs_matrix_set_precon_f64(P%hp, 1, shift), 'REAL_PRECON' )
+         else
+            call idrs_check( idrs_matrix_set_precon_f64(P%hp, 1), 'REAL_PRECON' )
+         end if
+! rband(:,0) = the diagonal the preconditioner divides by
+         allocate( P%rband(P%nrows,0:0) )
+         P%rband(:,0) = real_diagonal( K )
+         if ( present(shift) ) P%rband(:,0) = P%rband(:,0) - shift
+      elseif ( which_precon /= 0 ) then
+         error stop 'REAL_PRECON: dense and band LU preconditioners are not offered by the GPU library'
+      end if
+   end function REAL_PRECON
+
+   function COMPLEX_PRECON( which_precon, K, M, C, shift ) result(P)
+      integer, intent(in)                              :: which_precon
+      type(user_matrix), intent(in)                    :: K
+      type(user_matrix), intent(in), optional          :: M
+      type(user_matrix), intent(in), optional          :: C
+      type(user_matrix)                                :: P
+      complex(kind=cp), optional                       :: shift
+
+      P%matrix_format = 0
+      P%which_precon = which_precon
+      P%nrows = K%nrows
+      P%ncols = K%ncols
+      P%is_complex = .true.
+      if ( present(M) .or. present(C) ) &
+         error stop 'COMPLEX_PRECON: mass and damping matrices are not offered by the GPU library'
+      if ( which_precon == 1 ) then
+         P%matrix_format = 4
+         P%h = K%h
+         call idrs_check( idrs_matrix_create(K%h, P%hp), 'COMPLEX_PRECON' )
+         if ( present(shift) ) then
+            P%has_shift = .true.
+            P%cshift = shift
+            call idrs_check( idrs_matrix_set_precon_c128(P%hp, 1, shift), 'COMPLEX_PRECON' )
+         else
+            call idrs_check( idrs_matrix_set_precon_c128(P%hp, 1), 'COMPLEX_PRECON' )
+         end if
+         allocate( P%cband(P%nrows,0:0) )
+         P%cband(:,0) = complex_diagonal( K )
+         if ( present(shift) ) P%cband(:,0) = P%cband(:,0) - shift
+      elseif ( which_precon /= 0 ) then
+         error stop 'COMPLEX_PRECON: dense and band LU preconditioners are not offered by the GPU library'
+      end if
+   end function COMPLEX_PRECON
+
+! Releases the device storage behind A (and a preconditioner built from it).
+   subroutine user_matrix_free( A )
+      type(user_matrix), intent(inout)   :: A
+      integer(c_int)                     :: status
+      if ( c_associated(A%hp) ) status = idrs_matrix_destroy(A%hp)
+      if ( c_associated(A%h) .and. A%which_precon == 0 ) status = idrs_user_matrix_destroy(A%h)
+      A%h = c_null_ptr
+      A%hp = c_null_ptr
+   end subroutine user_matrix_free
+
+end module user_module

@@ -87,6 +87,10 @@ _PROTOS = {
     "idrs_real_preconditioner": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "idrs_complex_preconditioner": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]),
     "idrs_matrix_get_alpha": (c_int, [c_void_p, c_void_p, c_int, POINTER(c_int)]),
+    "idrs_matrix_set_alpha": (c_int, [c_void_p, c_int, c_void_p]),
+    "idrs_chebyshev_coeffs": (c_int, [c_int, c_void_p, c_int, c_void_p]),
+    "idrs_neumann_coeffs": (c_int, [c_int, c_void_p, c_int, c_void_p]),
+    "idrs_compute_alpha_coeffs": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
     "idrs_matvec_f64": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_matvec_c128": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_precon_f64": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -116,6 +116,18 @@ int idrs_real_preconditioner(idrs_matrix_t A, int preconditioner, int degree, co
    arguments interleaved. */
 int idrs_complex_preconditioner(idrs_matrix_t A, int preconditioner, int degree, const double *foci,
                                 const double *center, const double *H, int ldh, int ncol);
+/* Sets A%degree and A%ralpha_p / A%calpha_p(0:degree) directly (values
+   as (re, im) pairs); the polynomial of rpolpre, matrix_module.f90:266-304. */
+int idrs_matrix_set_alpha(idrs_matrix_t A, int degree, const double *alpha);
+/* alpha = chebyshev(degree, foci): rchebyshev / cchebyshev,
+   matrix_module.f90:903-981; is_complex selects complex arithmetic, all
+   values as (re, im) pairs (imaginary parts ignored for real). */
+int idrs_chebyshev_coeffs(int degree, const double *foci, int is_complex, double *alpha);
+/* alpha = neumann(degree, center): rneumann / cneumann, matrix_module.f90:851-901 */
+int idrs_neumann_coeffs(int degree, const double *center, int is_complex, double *alpha);
+/* alpha = compute_alpha(degree, H): rcompute_alpha / ccompute_alpha,
+   matrix_module.f90:983-1057; H(ldh, ncol) as (re, im) pairs. */
+int idrs_compute_alpha_coeffs(int degree, const double *H, int ldh, int ncol, int is_complex, double *alpha);
 /* A%ralpha_p / A%calpha_p(0:degree) as (re, im) pairs; *degree out */
 int idrs_matrix_get_alpha(idrs_matrix_t A, double *alpha, int maxlen, int *degree);
 /* w = A*v: rmatvec, matrix_module.f90:152-167 */

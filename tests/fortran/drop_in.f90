@@ -1,0 +1,143 @@
+!
+! Drop-in check: reference-style calls of the IDRS package interface,
+! compiled against the GPU library's Fortran modules.  Prints one line per
+! case: "CASE <name> iter <n> flag <f> relres <r> err <e>".
+!
+program drop_in
+
+   use precision_module
+   use user_module
+   use matrix_module
+   use idrs_module
+
+   implicit none
+
+   call toeplitz_real()
+   call cdr_preconditioned()
+   call toeplitz_complex()
+
+contains
+
+! 1-D convection-diffusion, N=60 (EXAMPLES/TOEPLITZ/toeplitz.f90:111-165),
+! stored in CRS; positional call as in the example.
+   subroutine toeplitz_real()
+      integer, parameter            :: N = 60, maxit = 4000
+      real(kind=rp), parameter      :: Pe = 0.5
+      type(matrix)                  :: A, M
+      real(kind=rp)                 :: x(N), b(N), tol, relres
+      integer                       :: row_ptr(N+1), col_ind(3*N), flag, iter, s, i, k, variant
+      real(kind=rp)                 :: val(3*N)
+      character(len=16)             :: name
+
+      k = 0
+      row_ptr(1) = 1
+      do i = 1, N
+         if ( i > 1 ) then
+            k = k + 1; col_ind(k) = i-1; val(k) = -1._rp-Pe
+         end if
+         k = k + 1; col_ind(k) = i; val(k) = 2._rp
+         if ( i < N ) then
+            k = k + 1; col_ind(k) = i+1; val(k) = -1._rp+Pe
+         end if
+         row_ptr(i+1) = k + 1
+      end do
+      A%K = crs_format( N, N, row_ptr, col_ind(1:k), val(1:k) )
+      b = 0.
+      b(1) = 1._rp+Pe
+      b(N) = 1._rp-Pe
+      tol = 1.e-8_rp
+      variant = 1
+      do i = 1, 4
+         s = 2**(i-1)
+         x = idrs( A, b, s, M, tol, maxit, variant, flag, relres, iter )
+         write(name,'(a,i0)') 'toeplitz_s', s
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', relres, ' err', maxval(abs(x-1._rp))
+      end do
+      x = idrs( A, b, 1, M, tol, maxit, 2, flag, relres, iter )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'toeplitz_bicgstab', ' iter', iter, ' flag', flag, &
+         ' relres', relres, ' err', maxval(abs(x-1._rp))
+   end subroutine toeplitz_real
+
+! pp_idrs call sequence (PP_IDRS/real_matrix.inc, real_preconditioner.inc,
+! run_idrs.inc): diagonal scaling + Chebyshev(8), IDR(4), scale back.
+   subroutine cdr_preconditioned()
+      integer, parameter            :: m = 16, n = m*m*m
+      type(matrix)                  :: A, M1
+      type(user_matrix)             :: K
+      real(kind=rp)                 :: x(n), b(n), x0(n), relres, h, e2, c, foci(2)
+      real(kind=rp), allocatable    :: val(:)
+      integer, allocatable          :: row_ptr(:), col_ind(:)
+      integer                       :: flag, iter, i, j, l, q, nz
+      real(kind=rp)                 :: cw(7)
+      integer                       :: off(7, 3)
+
+      h = 1._rp/(m+1)
+      e2 = 1._rp/h**2
+      c = 6._rp*e2 + 1._rp
+      ! front, south, west, centre, east, north, back with Pe = 5
+      cw = [ -e2-5._rp/(2*h), -e2-5._rp/(2*h), -e2-5._rp/(2*h), c, -e2+5._rp/(2*h), -e2+5._rp/(2*h), -e2+5._rp/(2*h) ]
+      off(:,1) = [0, 0, -1, 0, 1, 0, 0]
+      off(:,2) = [0, -1, 0, 0, 0, 1, 0]
+      off(:,3) = [-1, 0, 0, 0, 0, 0, 1]
+      allocate( row_ptr(n+1), col_ind(7*n), val(7*n) )
+      nz = 0
+      row_ptr(1) = 1
+      do l = 0, m-1
+         do j = 0, m-1
+            do i = 0, m-1
+               do q = 1, 7
+                  if ( i+off(q,1) < 0 .or. i+off(q,1) >= m ) cycle
+                  if ( j+off(q,2) < 0 .or. j+off(q,2) >= m ) cycle
+                  if ( l+off(q,3) < 0 .or. l+off(q,3) >= m ) cycle
+                  nz = nz + 1
+                  col_ind(nz) = 1 + (i+off(q,1)) + m*(j+off(q,2)) + m*m*(l+off(q,3))
+                  val(nz) = cw(q)
+               end do
+               row_ptr(1 + i + m*j + m*m*l + 1) = nz + 1
+            end do
+         end do
+      end do
+      K = crs_format( n, n, row_ptr, col_ind(1:nz), val(1:nz) )
+      x = 1._rp
+      b = K*x
+      A%K = K
+      A%P = real_precon( 1, K )
+      foci = [0.15_rp, 1.85_rp]
+      call real_preconditioner( A, 2, 8, foci )
+      x0 = idrs( A, Pmul(A,b), 4, M1, 1.e-8_rp, 2000, 1, flag, relres, iter )
+      x = scaleback( A, b, x0 )
+      relres = norm( b - K*x )/norm( b )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'cdr_cheb8', ' iter', iter, ' flag', flag, &
+         ' relres', relres, ' err', maxval(abs(x-1._rp))
+      call user_matrix_free( A%P )
+      call user_matrix_free( K )
+   end subroutine cdr_preconditioned
+
+! Complex Toeplitz accuracy test (toeplitz.f90:41-70) in offset storage.
+   subroutine toeplitz_complex()
+      integer, parameter            :: N = 200
+      type(matrix)                  :: A, M
+      complex(kind=cp)              :: x(N), b(N), vals(N,4), cv(4)
+      integer                       :: offs(4), flag, iter, i, d
+      real(kind=rp)                 :: relres
+
+      offs = [-1, 0, 2, 3]
+      cv = [ (0._cp,3.6_cp), (4._cp,0._cp), (1._cp,0._cp), (0.7_cp,0._cp) ]
+      do d = 1, 4
+         do i = 1, N
+            if ( i+offs(d) >= 1 .and. i+offs(d) <= N ) then
+               vals(i,d) = cv(d)
+            else
+               vals(i,d) = (0._cp,0._cp)
+            end if
+         end do
+      end do
+      A%K = dia_format( N, N, offs, cvals=vals )
+      b = (0._cp,1._cp)
+      x = idrs( A, b, 4, M, 1.e-12_rp, 4000, 1, flag, relres, iter )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'toeplitz_complex_s4', ' iter', iter, ' flag', flag, &
+         ' relres', relres, ' err', norm( b - A%K*x )/norm( b )
+   end subroutine toeplitz_complex
+
+end program drop_in
