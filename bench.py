@@ -2,12 +2,13 @@
 
 Workload (default): 3-D convection-diffusion-reaction, 7-point central
 differences on a 200^3 grid (N = 8e6, nnz = 55.76e6), Pe = 50, reaction 1,
-CRS storage, FP64; IDR(4) with a degree-8 Chebyshev polynomial
+FP64, stored both as offset-list CDS (DIA, the headline value) and as CRS
+(reported beside it); IDR(4) with a degree-8 Chebyshev polynomial
 preconditioner (foci 0.15, 1.85) and diagonal scaling; b ~ U[-1,1] (seed 0);
 Gaussian shadow space (seed 0); ||r||/||b|| <= 1e-8.  A "step" is one full
 solve with b, x and the matrix already resident in HBM.
 
-Prints one JSON line.  roofline: the CRS SpMV kernel (the dominant kernel),
+Prints one JSON line.  roofline: the SpMV kernel (the dominant kernel),
 achieved = algorithmic bytes of the timed SpMV launches / their summed HIP
 event time, against the 8 TB/s HBM peak.  cpu_baseline: the oracle (the
 project's scalar C restatement of the reference) on a bounded sample of the
@@ -81,10 +82,10 @@ def cpu_baseline(crs, b, PT, s, degree, foci, sample_iters, gpu_iters):
     }
 
 
-def read_traffic():
+def read_traffic(fmt):
     """HBM bytes per SpMV launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), from the committed profile summary, or None."""
-    path = os.path.join(ROOT, "profiles", "spmv_crs_200cube_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"spmv_{fmt}_200cube_traffic.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -96,6 +97,21 @@ def read_traffic():
     return None
 
 
+def calibrate_copy():
+    """A plain 512 MiB device-to-device copy, 5 times: each dispatch reads and
+    writes exactly 536870912 bytes, which calibrates rocprofv3's FETCH_SIZE
+    and WRITE_SIZE counters (KiB per dispatch)."""
+    import torch
+
+    a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8, "dispatches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,10 +120,16 @@ def main():
     ap.add_argument("--grid", type=int, default=200)
     ap.add_argument("--s", type=int, default=4)
     ap.add_argument("--degree", type=int, default=8)
-    ap.add_argument("--format", choices=["crs", "dia"], default="crs")
+    ap.add_argument("--format", choices=["both", "crs", "dia"], default="both",
+                    help="storage: both (headline DIA, then CRS), or one of them")
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain device copy of known size (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and not measured.", file=sys.stderr)
@@ -128,94 +150,112 @@ def main():
     b_h = rng.uniform(-1.0, 1.0, n)
     PT_h = P.gaussian_shadow(n, s, seed=0)
     print(f"# system built: n={n} nnz={nnz} ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
-
-    if args.format == "crs":
-        K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
-    else:
-        K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
-    del dia
-    A = idrs_amd.Matrix(K)
-    A.set_precon(1)
-    idrs_amd.real_preconditioner(A, 2, d, foci=foci)
     dev = torch.device("cuda:0")
     b = torch.from_numpy(b_h).to(dev)
     PT = torch.from_numpy(np.ascontiguousarray(PT_h)).to(dev)
     x = torch.empty_like(b)
-    torch.cuda.synchronize()
-    print(f"# uploaded ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+    formats = ["dia", "crs"] if args.format == "both" else [args.format]
 
-    def step(timed):
-        return idrs_amd.idrs(A, b, s, tolerance=1e-8, maximum_iterations=20000, PT_raw=PT, out=x,
-                             time_spmv=timed)
+    def run(fmt):
+        if fmt == "crs":
+            K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+        else:
+            K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
+        A = idrs_amd.Matrix(K)
+        A.set_precon(1)
+        idrs_amd.real_preconditioner(A, 2, d, foci=foci)
+        torch.cuda.synchronize()
 
-    for i in range(args.warmup):
-        r = step(False)
-        print(f"# warmup {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} "
-              f"solve={r.solve_ms:.1f} ms", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    results = []
-    for i in range(args.steps):
-        r = step(True)
-        results.append(r)
-        print(f"# step {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} solve={r.solve_ms:.1f} ms "
-              f"spmv={r.spmv_ms:.1f} ms / {r.spmv_launches} launches", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    ms_per_step = wall * 1e3 / args.steps
-    last = results[-1]
-    assert last.flag == 0, f"solve did not converge: flag {last.flag}"
+        def step(timed):
+            return idrs_amd.idrs(A, b, s, tolerance=1e-8, maximum_iterations=20000, PT_raw=PT, out=x,
+                                 time_spmv=timed)
 
-    # roofline of the dominant kernel (CRS SpMV) from the timed launches
-    tot_bytes, launches = solve_spmv_bytes(n, nnz, s, d, last.iterations, s)
-    spmv_ms = sum(r.spmv_ms for r in results) / len(results)
-    n_launch = results[-1].spmv_launches
-    if args.format == "dia":
-        tot_bytes = tot_bytes - launches * (12 * nnz + 4 * (n + 1)) + launches * 8 * 7 * n
-    achieved = tot_bytes / (spmv_ms * 1e-3) / 1e9
-    traffic = read_traffic()
-    per_launch = tot_bytes / max(n_launch, 1)
-    # whole-solve algorithmic bandwidth (model of all kernels)
-    solve_bytes = last.iterations * iteration_bytes(n, nnz, s, d)
-    solve_gbs = solve_bytes / (ms_per_step * 1e-3) / 1e9
+        for i in range(args.warmup):
+            r = step(False)
+            print(f"# {fmt} warmup {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} "
+                  f"solve={r.solve_ms:.1f} ms", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        results = []
+        for i in range(args.steps):
+            r = step(True)
+            results.append(r)
+            print(f"# {fmt} step {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} "
+                  f"solve={r.solve_ms:.1f} ms spmv={r.spmv_ms:.1f} ms / {r.spmv_launches} launches",
+                  file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        ms_per_step = wall * 1e3 / args.steps
+        last = results[-1]
+        assert last.flag == 0, f"solve did not converge: flag {last.flag}"
+        # roofline of the SpMV kernel over the timed launches
+        tot_bytes, launches = solve_spmv_bytes(n, nnz, s, d, last.iterations, s)
+        if fmt == "dia":
+            nd = len(dia.offsets)
+            tot_bytes = tot_bytes - launches * (12 * nnz + 4 * (n + 1)) + launches * 8 * nd * n
+        spmv_ms = sum(r.spmv_ms for r in results) / len(results)
+        achieved = tot_bytes / (spmv_ms * 1e-3) / 1e9
+        it_bytes = iteration_bytes(n, nnz, s, d)
+        if fmt == "dia":
+            it_bytes = it_bytes - (d + 1) * (12 * nnz + 4 * (n + 1)) + (d + 1) * 8 * len(dia.offsets) * n
+        solve_gbs = last.iterations * it_bytes / (ms_per_step * 1e-3) / 1e9
+        del A, K
+        return {
+            "ms_per_step": ms_per_step, "iterations": last.iterations, "relres": last.relres,
+            "roofline": {
+                "bound": "hbm",
+                "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS,
+                "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(fmt),
+                "kernel": f"k_spmv_{fmt}",
+                "bytes_per_launch": round(tot_bytes / launches),
+                "avg_launch_us": round(spmv_ms * 1e3 / launches, 2),
+            },
+            "solve_gbs_model": round(solve_gbs, 1),
+            "solve_frac_hbm_model": round(solve_gbs / HBM_PEAK_GBS, 4),
+        }
+
+    res = {fmt: run(fmt) for fmt in formats}
+    head = res[formats[0]]
 
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline sample ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(crs, b_h, PT_h, s, d, foci, args.cpu_sample_iters, last.iterations)
+        cpu = cpu_baseline(crs, b_h, PT_h, s, d, foci, args.cpu_sample_iters, head["iterations"])
 
+    config = {
+        "workload": f"{m}^3 convection-diffusion-reaction (Pe 50, reaction 1), FP64, IDR({s}) + "
+                    f"Chebyshev({d}) with diagonal scaling, tol 1e-8; storage {formats[0].upper()} "
+                    "(offset-list CDS)" if formats[0] == "dia" else
+                    f"{m}^3 convection-diffusion-reaction (Pe 50, reaction 1), FP64, IDR({s}) + "
+                    f"Chebyshev({d}) with diagonal scaling, tol 1e-8; storage CRS",
+        "n": n, "nnz": nnz, "s": s, "degree": d, "format": formats[0],
+        "iterations": head["iterations"], "relres": head["relres"],
+        "solve_gbs_model": head["solve_gbs_model"],
+        "solve_frac_hbm_model": head["solve_frac_hbm_model"],
+    }
+    for fmt in formats[1:]:
+        r = res[fmt]
+        config[fmt] = {"ms_per_step": round(r["ms_per_step"], 2), "iterations": r["iterations"],
+                       "relres": r["relres"], "roofline": r["roofline"],
+                       "solve_frac_hbm_model": r["solve_frac_hbm_model"]}
     out = {
         "metric": "time-to-solution (ms) at rel. residual 1e-8 + SpMV/vector HBM GB/s vs peak",
-        "value": round(ms_per_step, 2),
+        "value": round(head["ms_per_step"], 2),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 2),
+        "ms_per_step": round(head["ms_per_step"], 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": f"{m}^3 convection-diffusion-reaction (Pe 50, reaction 1), {args.format.upper()} FP64, "
-                        f"IDR({s}) + Chebyshev({d}) with diagonal scaling, tol 1e-8",
-            "n": n, "nnz": nnz, "s": s, "degree": d, "format": args.format,
-            "iterations": last.iterations, "relres": last.relres,
-            "solve_gbs_model": round(solve_gbs, 1),
-            "solve_frac_hbm_model": round(solve_gbs / HBM_PEAK_GBS, 4),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": f"k_spmv_{args.format}",
-            "bytes_per_launch": round(per_launch),
-            "avg_launch_us": round(spmv_ms * 1e3 / max(n_launch, 1), 2),
-        },
+        "config": config,
+        "roofline": head["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

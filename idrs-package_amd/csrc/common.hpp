@@ -171,24 +171,39 @@ __device__ inline void block_sum(T (&v)[NV], T *lds) {
     }
 }
 
-// Reduce partials laid out [q * nparts + p] over p, for q < nv, with one block;
-// deterministic.  Result in out[q] (shared memory) for all threads after return.
+// Reduce partials laid out [q * nparts + p] over p, for q < nv <= MAX_RED,
+// with one block in one pass; deterministic.  Result in out[q] (shared
+// memory), valid for all threads after return.
+constexpr int MAX_RED = 20;
+constexpr int FIN_THREADS = 1024;   // block size of the one-block reduction kernels
 template <class T>
-__device__ inline void reduce_partials(const T *__restrict__ partials, int nparts, int nv, T *out, T *lds) {
+__device__ inline void reduce_partials(const T *__restrict__ partials, int nparts, int nv, T *out) {
+    __shared__ T lds[MAX_RED * (FIN_THREADS / WAVE)];
     const int nw = blockDim.x / WAVE, lane = threadIdx.x % WAVE, wid = threadIdx.x / WAVE;
     for (int q = 0; q < nv; q++) {
-        T acc = T(0);
-        for (int p = threadIdx.x; p < nparts; p += blockDim.x) acc = acc + partials[(long)q * nparts + p];
-        acc = wave_sum(acc);
-        if (lane == 0) lds[wid] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            T t = lds[0];
-            for (int w = 1; w < nw; w++) t = t + lds[w];
-            out[q] = t;
+        // four independent partial sums per thread keep several loads in flight
+        T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+        const T *pq = partials + (long)q * nparts;
+        int p = threadIdx.x;
+        const int step = blockDim.x;
+        for (; p + 3 * step < nparts; p += 4 * step) {
+            a0 = a0 + pq[p];
+            a1 = a1 + pq[p + step];
+            a2 = a2 + pq[p + 2 * step];
+            a3 = a3 + pq[p + 3 * step];
         }
-        __syncthreads();
+        for (; p < nparts; p += step) a0 = a0 + pq[p];
+        T v = wave_sum((a0 + a1) + (a2 + a3));
+        if (lane == 0) lds[q * (FIN_THREADS / WAVE) + wid] = v;
     }
+    __syncthreads();
+    if (threadIdx.x < nv) {
+        const int q = threadIdx.x;
+        T t = lds[q * (FIN_THREADS / WAVE)];
+        for (int w = 1; w < nw; w++) t = t + lds[q * (FIN_THREADS / WAVE) + w];
+        out[q] = t;
+    }
+    __syncthreads();
 }
 
 // XCD-aware workgroup index: gfx950 dispatches consecutive workgroups round

@@ -156,9 +156,9 @@ __global__ __launch_bounds__(256) void k_residual(long n, const T *__restrict__ 
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_normb(State<T> *st, const T *partials, int nparts) {
-    __shared__ T red[1], lds[4];
-    reduce_partials(partials, nparts, 1, red, lds);
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_normb(State<T> *st, const T *partials, int nparts) {
+    __shared__ T red[1];
+    reduce_partials(partials, nparts, 1, red);
     if (threadIdx.x == 0) {
         st->normb = sqrt(sre(red[0]));
         st->tolb = st->tol * st->normb;                                    // idrs.inc:94
@@ -166,9 +166,9 @@ __global__ __launch_bounds__(256) void k_fin_normb(State<T> *st, const T *partia
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_init(State<T> *st, const T *partials, int nparts, double *resvec) {
-    __shared__ T red[1], lds[4];
-    reduce_partials(partials, nparts, 1, red, lds);
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_init(State<T> *st, const T *partials, int nparts, double *resvec) {
+    __shared__ T red[1];
+    reduce_partials(partials, nparts, 1, red);
     if (threadIdx.x == 0) {
         double nr = sqrt(sre(red[0]));
         st->normr = nr;
@@ -178,9 +178,9 @@ __global__ __launch_bounds__(256) void k_fin_init(State<T> *st, const T *partial
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_final(State<T> *st, const T *partials, int nparts) {
-    __shared__ T red[1], lds[4];
-    reduce_partials(partials, nparts, 1, red, lds);
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_final(State<T> *st, const T *partials, int nparts) {
+    __shared__ T red[1];
+    reduce_partials(partials, nparts, 1, red);
     if (threadIdx.x == 0) {
         double nr = sqrt(sre(red[0]));                                     // idrs.inc:304-307
         st->normr = nr;
@@ -226,9 +226,9 @@ __global__ __launch_bounds__(256) void k_gs_dots(long n, int s, int j, const T *
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_gs(State<T> *st, const T *partials, int nparts, int nv) {
-    __shared__ T red[SMAX], lds[4];
-    reduce_partials(partials, nparts, nv, red, lds);
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_gs(State<T> *st, const T *partials, int nparts, int nv) {
+    __shared__ T red[SMAX];
+    reduce_partials(partials, nparts, nv, red);
     if (threadIdx.x == 0)
         for (int k = 0; k < nv; k++) st->gs[k] = red[k];
 }
@@ -252,9 +252,9 @@ __global__ __launch_bounds__(256) void k_gs_update_norm(long n, int s, int j, co
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_rho(State<T> *st, const T *partials, int nparts) {
-    __shared__ T red[1], lds[4];
-    reduce_partials(partials, nparts, 1, red, lds);
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_rho(State<T> *st, const T *partials, int nparts) {
+    __shared__ T red[1];
+    reduce_partials(partials, nparts, 1, red);
     if (threadIdx.x == 0) st->rho = sqrt(sre(red[0]));
 }
 
@@ -297,10 +297,10 @@ __global__ __launch_bounds__(256) void k_pdot(long n, const T *__restrict__ w, c
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_f(State<T> *st, const T *partials, int nparts) {
-    __shared__ T red[SMAX], lds[4];
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_f(State<T> *st, const T *partials, int nparts) {
+    __shared__ T red[SMAX];
     const int prow = st->prow;
-    reduce_partials(partials, nparts, prow, red, lds);
+    reduce_partials(partials, nparts, prow, red);
     if (threadIdx.x == 0)
         for (int i = 0; i < st->s; i++) st->f[i] = i < prow ? red[i] : T(0);
 }
@@ -350,19 +350,19 @@ __device__ void scalar_mu(State<T> *st, int k, const T *red) {
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_mu(int k, State<T> *st, const T *partials, int nparts) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_mu(int k, State<T> *st, const T *partials, int nparts) {
     if (st->info >= 0) return;
-    __shared__ T red[SMAX], lds[4];
-    reduce_partials(partials, nparts, st->prow, red, lds);
+    __shared__ T red[SMAX];
+    reduce_partials(partials, nparts, st->prow, red);
     if (threadIdx.x == 0) scalar_mu(st, k, red);
 }
 
 // normr, iteration count, convergence (idrs.inc:240-249)
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_norm(State<T> *st, const T *partials, int nparts, double *resvec) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_norm(State<T> *st, const T *partials, int nparts, double *resvec) {
     if (st->info >= 0) return;
-    __shared__ T red[1], lds[4];
-    reduce_partials(partials, nparts, 1, red, lds);
+    __shared__ T red[1];
+    reduce_partials(partials, nparts, 1, red);
     if (threadIdx.x != 0) return;
     double nr = sqrt(sre(red[0]));
     st->normr = nr;
@@ -403,21 +403,21 @@ __device__ void scalar_om(State<T> *st, const T *red, const T *omega) {
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_om(State<T> *st, const T *partials, int nparts, const T *omega) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_om(State<T> *st, const T *partials, int nparts, const T *omega) {
     if (st->info >= 0) return;
-    __shared__ T red[3], lds[4];
-    reduce_partials(partials, nparts, 3, red, lds);
+    __shared__ T red[3];
+    reduce_partials(partials, nparts, 3, red);
     if (threadIdx.x == 0) scalar_om(st, red, omega);
 }
 
 // end of the omega step (idrs.inc:292-299) and the next cycle's f = P^T r
 // and gamma (idrs.inc:147, 159-165)
 template <class T>
-__global__ __launch_bounds__(256) void k_fin_cycle(State<T> *st, const T *partials, int nparts, double *resvec) {
+__global__ __launch_bounds__(FIN_THREADS) void k_fin_cycle(State<T> *st, const T *partials, int nparts, double *resvec) {
     if (st->info >= 0) return;
-    __shared__ T red[SMAX + 1], lds[4];
+    __shared__ T red[SMAX + 1];
     const int prow = st->prow;
-    reduce_partials(partials, nparts, 1 + prow, red, lds);
+    reduce_partials(partials, nparts, 1 + prow, red);
     if (threadIdx.x != 0) return;
     double nr = sqrt(sre(red[0]));
     st->normr = nr;
@@ -611,12 +611,12 @@ static void make_p_impl(long n, int s, int method, const T *PT_raw, const T *r, 
     for (int j = 0; j < rows; j++) {
         if (j > 0) {
             launch_gs_dots<T>(n, rows, j, PT, partials, stream);
-            hipLaunchKernelGGL((k_fin_gs<T>), dim3(1), dim3(256), 0, stream, st, partials, g, j);
+            hipLaunchKernelGGL((k_fin_gs<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, partials, g, j);
             LAUNCH_CHECK();
         }
         hipLaunchKernelGGL((k_gs_update_norm<T>), dim3(g), dim3(256), 0, stream, n, rows, j, st, PT, partials);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_fin_rho<T>), dim3(1), dim3(256), 0, stream, st, partials, g);
+        hipLaunchKernelGGL((k_fin_rho<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, partials, g);
         LAUNCH_CHECK();
         hipLaunchKernelGGL((k_gs_scale<T>), dim3(g), dim3(256), 0, stream, n, rows, j, st, PT);
         LAUNCH_CHECK();
@@ -716,18 +716,18 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     // normb, tolb; r = b - A x; normr (idrs.inc:93-106)
     hipLaunchKernelGGL((k_norm_partials<T>), dim3(gv), dim3(256), 0, stream, n, b, part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_fin_normb<T>), dim3(1), dim3(256), 0, stream, st, part, gv);
+    hipLaunchKernelGGL((k_fin_normb<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv);
     LAUNCH_CHECK();
     EpiArgs<T> none{};
     if (o.x0) op_apply<T>(A, x, tmp, w1, w2, EPI_NONE, none, nullptr, stream);
     hipLaunchKernelGGL((k_residual<T>), dim3(gv), dim3(256), 0, stream, n, b, o.x0 ? tmp : nullptr, r, part);
     LAUNCH_CHECK();
-    hipLaunchKernelGGL((k_fin_init<T>), dim3(1), dim3(256), 0, stream, st, part, gv, resvec);
+    hipLaunchKernelGGL((k_fin_init<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv, resvec);
     LAUNCH_CHECK();
     // P (idrs.inc:109-126) and f = P^T r for the first cycle (idrs.inc:147)
     make_p_impl<T>(n, s, method, o.PT_raw, r, P, o.p_seed, st, part, stream);
     launch_pdot<T>(n, r, P, prow, conjP, part, stream);
-    hipLaunchKernelGGL((k_fin_f<T>), dim3(1), dim3(256), 0, stream, st, part, gv);
+    hipLaunchKernelGGL((k_fin_f<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv);
     LAUNCH_CHECK();
 
     EpiArgs<T> epd{};
@@ -747,11 +747,11 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         LAUNCH_CHECK();
         for (int k = 0; k < s; k++) {
             op_apply<T>(A, U + (long)k * n, G + (long)k * n, w1, w2, EPI_PDOT, epd, info, stream);
-            hipLaunchKernelGGL((k_fin_mu<T>), dim3(1), dim3(256), 0, stream, k, st, part, gs);
+            hipLaunchKernelGGL((k_fin_mu<T>), dim3(1), dim3(FIN_THREADS), 0, stream, k, st, part, gs);
             LAUNCH_CHECK();
             hipLaunchKernelGGL((k_orth_update<T>), dim3(gv), dim3(256), 0, stream, n, k, st, G, U, r, x, m1, part);
             LAUNCH_CHECK();
-            hipLaunchKernelGGL((k_fin_norm<T>), dim3(1), dim3(256), 0, stream, st, part, gv, resvec);
+            hipLaunchKernelGGL((k_fin_norm<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv, resvec);
             LAUNCH_CHECK();
         }
         if (m1) {
@@ -759,10 +759,10 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
             LAUNCH_CHECK();
         }
         op_apply<T>(A, v, t, w1, w2, EPI_OMEGA, eom, info, stream);
-        hipLaunchKernelGGL((k_fin_om<T>), dim3(1), dim3(256), 0, stream, st, part, gs, omega_d);
+        hipLaunchKernelGGL((k_fin_om<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gs, omega_d);
         LAUNCH_CHECK();
         launch_omega_update<T>(n, st, v, t, r, x, P, prow, conjP, part, stream);
-        hipLaunchKernelGGL((k_fin_cycle<T>), dim3(1), dim3(256), 0, stream, st, part, gv, resvec);
+        hipLaunchKernelGGL((k_fin_cycle<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv, resvec);
         LAUNCH_CHECK();
     };
 
@@ -788,7 +788,7 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         op_apply<T>(A, x, tmp, w1, w2, EPI_NONE, none, nullptr, stream);
         hipLaunchKernelGGL((k_residual<T>), dim3(gv), dim3(256), 0, stream, n, b, tmp, r, part);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_fin_final<T>), dim3(1), dim3(256), 0, stream, st, part, gv);
+        hipLaunchKernelGGL((k_fin_final<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv);
         LAUNCH_CHECK();
         HIP_TRY(hipEventRecord(e1, stream));
         State<T> hs;

@@ -84,7 +84,8 @@ int vec_grid(long n) {
 }
 
 int spmv_grid(const DevMatrix &K) {
-    long units = (K.fmt == DevFmt::CRS) ? K.nrb : (K.nrows + SPMV_THREADS - 1) / SPMV_THREADS;
+    long rows_per_unit = (K.fmt == DevFmt::CRS) ? (long)(SPMV_THREADS / K.group) * 4 : SPMV_THREADS;
+    long units = (K.nrows + rows_per_unit - 1) / rows_per_unit;
     long b = units;
     if (b > 4096) b = 4096;
     if (b < 1) b = 1;
@@ -114,12 +115,11 @@ static void finish_crs(DevMatrix *M, const std::vector<int> &rp0, const std::vec
     } else {
         M->val = dev_upload(rv.data(), (size_t)nnz);
     }
-    // rows per block: 256, fewer if the average row does not fit the LDS slab
+    // lanes per row: the smallest power of two >= the average row length
     double avg = M->nrows > 0 ? (double)nnz / M->nrows : 1.0;
-    int rpb = SPMV_THREADS;
-    while (rpb > 64 && avg * rpb * 1.15 > LDS_NNZ) rpb /= 2;
-    M->rpb = rpb;
-    M->nrb = (M->nrows + rpb - 1) / rpb;
+    int g = 2;
+    while (g < 64 && g < avg) g *= 2;
+    M->group = g;
     // diagonal (real_diagonal / complex_diagonal), zeros -> 1
     if (M->kind == ValKind::Complex) {
         std::vector<c128> d(M->nrows, c128(0.0));
@@ -313,6 +313,27 @@ __device__ inline void epi_accum(T (&e)[NV], const EpiArgs<T> &E, long q, T yv) 
     }
 }
 
+// Epilogue of a row whose value every lane of a G-lane group holds: for
+// P^T y lane g handles components g, g+G, ...; the other epilogues run on the
+// group's leader lane.  epi_store then sums over all lanes, which adds the
+// components of equal index.
+template <class T, int EPI, int NV, int G>
+__device__ inline void epi_accum_group(T (&e)[NV], const EpiArgs<T> &E, long q, T yv, int g) {
+    using TT = Tr<T>;
+    if constexpr (EPI == EPI_PDOT) {
+        const T *p = E.P + q * E.prow;
+#pragma unroll
+        for (int i = 0; i < NV; i++)
+            if (i % G == g && i < E.prow) {
+                T pv = p[i];
+                if (E.conjP) pv = TT::conj(pv);
+                e[i] = TT::fma(pv, yv, e[i]);
+            }
+    } else {
+        if (g == 0) epi_accum<T, EPI, NV>(e, E, q, yv);
+    }
+}
+
 template <class T, int EPI, int NV>
 __device__ inline void epi_store(T (&e)[NV], const EpiArgs<T> &E, T *lds) {
     if constexpr (EPI != EPI_NONE) {
@@ -328,7 +349,6 @@ struct SpmvArgs {
     int nrows, ncols;
     const int *rp, *col;
     const VT *val;
-    int rpb, nrb;
     int ndiag;
     const int *offsets;
     const T *x;
@@ -341,6 +361,15 @@ struct SpmvArgs {
     T *y;
 };
 
+// y(row) from the row sum, without storing it
+template <class VT, class T>
+__device__ inline T finish_value(const SpmvArgs<VT, T> &A, long row, T acc) {
+    if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
+    T yv = A.xs * acc;
+    if (A.z) yv = Tr<T>::fma(A.a, A.z[row], yv);
+    return yv;
+}
+
 template <class VT, class T>
 __device__ inline T finish_row(const SpmvArgs<VT, T> &A, long row, T acc) {
     if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
@@ -350,71 +379,130 @@ __device__ inline T finish_row(const SpmvArgs<VT, T> &A, long row, T acc) {
     return yv;
 }
 
-template <class VT, class T, int EPI, int NV>
+// Non-temporal loads for the matrix streams (read once per product): they
+// should not push the gathered x entries out of L2.
+template <class V>
+__device__ inline V ldnt(const V *p) { return __builtin_nontemporal_load(p); }
+template <class R>
+__device__ inline cplx<R> ldnt(const cplx<R> *p) {
+    const R *q = reinterpret_cast<const R *>(p);
+    return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+
+// CRS: a group of G lanes per row (G >= the typical row length), so a
+// wavefront covers 64/G consecutive rows whose entries are one contiguous,
+// coalesced stretch of values and column indices.  The block's row pointers
+// are staged in LDS once per chunk of rows, which leaves two dependent
+// memory latencies per row (column index, then x).  Each group keeps U rows
+// in flight: all their first-pass loads are issued before any is used; rows
+// longer than G finish in a strided remainder loop.  The group's partial
+// sums are combined by a butterfly inside the group, so every lane of the
+// group holds y(row) and the epilogue's loads are spread over the group.
+constexpr int CRS_CHUNK = 2048;   // rows whose row pointers sit in LDS at once
+
+template <class VT, class T, int EPI, int NV, int G>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, EpiArgs<T> E, const int *info) {
     if (info && *info >= 0) return;
-    __shared__ VT s_val[LDS_NNZ];
-    __shared__ int s_col[LDS_NNZ];
+    constexpr int GPB = SPMV_THREADS / G;
+    constexpr int U = 4;
+    __shared__ int s_rp[CRS_CHUNK + 1];
     __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
     T e[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) e[i] = T(0);
+    const int g = threadIdx.x % G, grp = threadIdx.x / G;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const long rb0 = (long)A.nrb * lb / gridDim.x, rb1 = (long)A.nrb * (lb + 1) / gridDim.x;
+    const long n = A.nrows;
+    const long r0 = n * lb / gridDim.x, r1 = n * (lb + 1) / gridDim.x;
     const T *__restrict__ x = A.x;
-    for (long rb = rb0; rb < rb1; rb++) {
-        const long r0 = rb * A.rpb;
-        const long r1 = min(r0 + (long)A.rpb, (long)A.nrows);
-        const int p0 = A.rp[r0], p1 = A.rp[r1];
-        const int nz = p1 - p0;
-        const long row = r0 + threadIdx.x;
-        const bool active = (threadIdx.x < A.rpb) && row < r1;
-        T acc = T(0);
-        if (nz <= LDS_NNZ) {
-            for (int q = threadIdx.x; q < nz; q += SPMV_THREADS) {
-                s_val[q] = A.val[p0 + q];
-                s_col[q] = A.col[p0 + q];
+    const VT *__restrict__ val = A.val;
+    const int *__restrict__ col = A.col;
+    for (long c0 = r0; c0 < r1; c0 += CRS_CHUNK) {
+        const long c1 = min(c0 + (long)CRS_CHUNK, r1);
+        __syncthreads();
+        for (long q = c0 + threadIdx.x; q <= c1; q += SPMV_THREADS) s_rp[q - c0] = A.rp[q];
+        __syncthreads();
+        for (long base = c0; base < c1; base += (long)GPB * U) {
+            long row[U];
+            int a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                row[u] = base + grp + (long)u * GPB;
+                const bool in = row[u] < c1;
+                a[u] = in ? s_rp[row[u] - c0] : 0;
+                b[u] = in ? s_rp[row[u] - c0 + 1] : 0;
             }
-            __syncthreads();
-            if (active) {
-                const int a = A.rp[row] - p0, b = A.rp[row + 1] - p0;
-                for (int j = a; j < b; j++) acc = fmav(s_val[j], x[s_col[j]], acc);
+            VT vv[U];
+            int cc[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int j = a[u] + g;
+                const bool ok = j < b[u];
+                vv[u] = ok ? ldnt(&val[j]) : VT(0);
+                cc[u] = ok ? ldnt(&col[j]) : -1;
             }
-            __syncthreads();
-        } else if (active) {
-            const int a = A.rp[row], b = A.rp[row + 1];
-            for (int j = a; j < b; j++) acc = fmav(A.val[j], x[A.col[j]], acc);
-        }
-        if (active) {
-            T yv = finish_row(A, row, acc);
-            epi_accum<T, EPI, NV>(e, E, row, yv);
+            T acc[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) acc[u] = cc[u] >= 0 ? mulv(vv[u], x[cc[u]]) : T(0);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                for (int j = a[u] + g + G; j < b[u]; j += G) acc[u] = fmav(ldnt(&val[j]), x[ldnt(&col[j])], acc[u]);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int m = G / 2; m >= 1; m >>= 1) acc[u] = acc[u] + shfl_xor(acc[u], m);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (row[u] < c1) {
+                    const T yv = finish_value(A, row[u], acc[u]);
+                    if (g == 0) A.y[row[u]] = yv;
+                    epi_accum_group<T, EPI, NV, G>(e, E, row[u], yv, g);
+                }
         }
     }
     epi_store<T, EPI, NV>(e, E, s_red);
 }
 
-template <class VT, class T, int EPI, int NV>
+// DIA: one thread per row over a contiguous row range; every diagonal is a
+// coalesced stream.  ND > 0 unrolls a known diagonal count.
+template <class VT, class T, int EPI, int NV, int ND>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, EpiArgs<T> E, const int *info) {
     if (info && *info >= 0) return;
     __shared__ int s_off[MAX_DIAG];
     __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
-    for (int d = threadIdx.x; d < A.ndiag; d += SPMV_THREADS) s_off[d] = A.offsets[d];
+    const int ndiag = ND > 0 ? ND : A.ndiag;
+    for (int d = threadIdx.x; d < ndiag; d += SPMV_THREADS) s_off[d] = A.offsets[d];
     __syncthreads();
     T e[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) e[i] = T(0);
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const long nchunk = (A.nrows + SPMV_THREADS - 1) / SPMV_THREADS;
+    const long n = A.nrows, ncols = A.ncols;
+    const long nchunk = (n + SPMV_THREADS - 1) / SPMV_THREADS;
     const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
     const T *__restrict__ x = A.x;
-    const long n = A.nrows;
+    const VT *__restrict__ val = A.val;
     for (long c = c0; c < c1; c++) {
         const long row = c * SPMV_THREADS + threadIdx.x;
         if (row >= n) continue;
         T acc = T(0);
-        for (int d = 0; d < A.ndiag; d++) {
-            const long cc = row + s_off[d];
-            if (cc >= 0 && cc < A.ncols) acc = fmav(A.val[(long)d * n + row], x[cc], acc);
+        if constexpr (ND > 0) {
+            VT vv[ND];
+            T xx[ND];
+#pragma unroll
+            for (int d = 0; d < ND; d++) {
+                const long cc = row + s_off[d];
+                const bool ok = cc >= 0 && cc < ncols;
+                vv[d] = ok ? ldnt(&val[(long)d * n + row]) : VT(0);
+                xx[d] = ok ? x[cc] : T(0);
+            }
+#pragma unroll
+            for (int d = 0; d < ND; d++) acc = fmav(vv[d], xx[d], acc);
+        } else {
+            for (int d = 0; d < ndiag; d++) {
+                const long cc = row + s_off[d];
+                if (cc >= 0 && cc < ncols) acc = fmav(ldnt(&val[(long)d * n + row]), x[cc], acc);
+            }
         }
         T yv = finish_row(A, row, acc);
         epi_accum<T, EPI, NV>(e, E, row, yv);
@@ -427,10 +515,24 @@ static void launch_spmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T>
                         hipStream_t st) {
     int grid = spmv_grid(K);
     E.nparts = grid;
-    if (K.fmt == DevFmt::CRS)
-        hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV>), dim3(grid), dim3(SPMV_THREADS), 0, st, A, E, info);
-    else
-        hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV>), dim3(grid), dim3(SPMV_THREADS), 0, st, A, E, info);
+    dim3 gd(grid), bd(SPMV_THREADS);
+    if (K.fmt == DevFmt::CRS) {
+        switch (K.group) {
+        case 2: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 2>), gd, bd, 0, st, A, E, info); break;
+        case 4: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 4>), gd, bd, 0, st, A, E, info); break;
+        case 8: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 8>), gd, bd, 0, st, A, E, info); break;
+        case 16: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 16>), gd, bd, 0, st, A, E, info); break;
+        case 32: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 32>), gd, bd, 0, st, A, E, info); break;
+        default: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 64>), gd, bd, 0, st, A, E, info); break;
+        }
+    } else {
+        switch (K.ndiag) {
+        case 3: hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV, 3>), gd, bd, 0, st, A, E, info); break;
+        case 5: hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV, 5>), gd, bd, 0, st, A, E, info); break;
+        case 7: hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV, 7>), gd, bd, 0, st, A, E, info); break;
+        default: hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV, 0>), gd, bd, 0, st, A, E, info); break;
+        }
+    }
     LAUNCH_CHECK();
 }
 
@@ -466,8 +568,6 @@ static SpmvArgs<VT, T> base_args(const DevMatrix &K, const VT *val) {
     A.rp = K.row_ptr;
     A.col = K.col;
     A.val = val;
-    A.rpb = K.rpb;
-    A.nrb = K.nrb;
     A.ndiag = K.ndiag;
     A.offsets = K.offsets;
     return A;

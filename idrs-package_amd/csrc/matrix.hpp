@@ -17,7 +17,6 @@ enum class ValKind { Pattern = 0, Real = 1, Complex = 2 };
 enum class DevFmt { CRS = 0, DIA = 1 };
 
 constexpr int SPMV_THREADS = 256;
-constexpr int LDS_NNZ = 2048;        // nonzeros staged in LDS per row block
 constexpr int MAX_DIAG = 1024;       // offsets held in LDS by the DIA kernel
 
 struct DevMatrix {
@@ -28,8 +27,7 @@ struct DevMatrix {
     long nnz = 0;            // stored entries (CRS) or ndiag*nrows (DIA)
     // CRS (0-based on device)
     int *row_ptr = nullptr, *col = nullptr;
-    int rpb = SPMV_THREADS;  // rows per row block
-    int nrb = 0;             // row blocks
+    int group = 8;           // lanes that share one row in the CRS kernel
     // DIA
     int ndiag = 0;
     int *offsets = nullptr;

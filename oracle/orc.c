@@ -32,6 +32,7 @@ double orc_uniform01(uint64_t seed, uint64_t index)
 #define ZERO 0.0
 #define ONE 1.0
 #include "orc_generic.inc"
+#include "orc_krylov.inc"
 #undef T
 #undef FN
 #undef CONJ
@@ -48,3 +49,4 @@ double orc_uniform01(uint64_t seed, uint64_t index)
 #define ZERO ((orc_cplx)0.0)
 #define ONE ((orc_cplx)1.0)
 #include "orc_generic.inc"
+#include "orc_krylov.inc"

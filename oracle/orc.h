@@ -77,6 +77,14 @@ typedef struct {
     /* Shadow space: raw random PT(s,n) before orthonormalisation.  NULL ->
        the oracle's default uniform [0,1) stream (orc_uniform01). */
     const void *PT_raw;
+    /* QMRIDR / MSQMRIDR inner solver parameters (present() flags) */
+    int has_inner_s; int inner_s;
+    int has_inner_tol; double inner_tol;
+    int has_inner_it; int inner_it;
+    /* multishift outputs: relres(n_sigma), colfac(n_sigma) (T), or NULL;
+       resvec is (resvec_len, n_sigma) column-major for the multishift solvers */
+    double *relres_vec;
+    void *colfac;
 } orc_idrs_opts;
 
 typedef struct {
@@ -121,5 +129,25 @@ int orc_r_idrs(const orc_matrix *A, int n, const double *b, int s,
                const orc_idrs_opts *o, double *x, orc_stats *st);
 int orc_c_idrs(const orc_matrix *A, int n, const orc_cplx *b, int s,
                const orc_idrs_opts *o, orc_cplx *x, orc_stats *st);
+
+/* ---- dense_la_module restatement ---- */
+void orc_r_solve(int n, const double *A, const double *b, double *x);
+void orc_c_solve(int n, const orc_cplx *A, const orc_cplx *b, orc_cplx *x);
+void orc_r_grot(double a, double b, double *c, double *s, double *r);
+void orc_c_grot(orc_cplx a, orc_cplx b, orc_cplx *c, orc_cplx *s, orc_cplx *r);
+
+/* ---- QMRIDR, MSIDRS, MSQMRIDR ---- */
+int orc_r_qmridr(const orc_matrix *A, int n, const double *b, int s, const orc_idrs_opts *o, double *x,
+                 orc_stats *st);
+int orc_c_qmridr(const orc_matrix *A, int n, const orc_cplx *b, int s, const orc_idrs_opts *o, orc_cplx *x,
+                 orc_stats *st);
+int orc_r_msidrs(const orc_matrix *A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                 const orc_idrs_opts *o, double *x, orc_stats *st);
+int orc_c_msidrs(const orc_matrix *A, int n, const orc_cplx *b, int n_sigma, const orc_cplx *sigma, int s,
+                 const orc_idrs_opts *o, orc_cplx *x, orc_stats *st);
+int orc_r_msqmridr(const orc_matrix *A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                   const orc_idrs_opts *o, double *x, orc_stats *st);
+int orc_c_msqmridr(const orc_matrix *A, int n, const orc_cplx *b, int n_sigma, const orc_cplx *sigma, int s,
+                   const orc_idrs_opts *o, orc_cplx *x, orc_stats *st);
 
 #endif

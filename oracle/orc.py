@@ -49,6 +49,11 @@ class Opts(ctypes.Structure):
         ("omega", c_void_p), ("n_omega", c_int),
         ("resvec", c_void_p), ("resvec_len", c_int),
         ("PT_raw", c_void_p),
+        ("has_inner_s", c_int), ("inner_s", c_int),
+        ("has_inner_tol", c_int), ("inner_tol", c_double),
+        ("has_inner_it", c_int), ("inner_it", c_int),
+        ("relres_vec", c_void_p),
+        ("colfac", c_void_p),
     ]
 
 
@@ -75,6 +80,10 @@ def load():
         _lib.orc_c_norm.restype = c_double
         for p in ("r", "c"):
             getattr(_lib, f"orc_{p}_idrs").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, POINTER(Opts), c_void_p, POINTER(Stats)]
+            getattr(_lib, f"orc_{p}_qmridr").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, POINTER(Opts), c_void_p, POINTER(Stats)]
+            for name in ("msidrs", "msqmridr"):
+                getattr(_lib, f"orc_{p}_{name}").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, c_void_p, c_int,
+                                                             POINTER(Opts), c_void_p, POINTER(Stats)]
             getattr(_lib, f"orc_{p}_compute_alpha").restype = c_int
     return _lib
 
@@ -261,6 +270,133 @@ def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None
     if rv is not None:
         rv = rv[: st.iterations + 1]
     return x, st.flag, st.relres, st.iterations, rv
+
+
+def _opts(A, n, dt, keep, M1=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, omega=None,
+          PT_raw=None, inner_s=None, inner_tolerance=None, inner_iterations=None):
+    o = Opts()
+
+    def hold(a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=dt))
+        keep.append(a)
+        return a.ctypes.data
+
+    if M1 is not None:
+        o.has_M1, o.M1 = 1, ctypes.addressof(M1.m)
+    if tolerance is not None:
+        o.has_tol, o.tol = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maxit, o.maxit = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    if x0 is not None:
+        o.x0 = hold(x0)
+    if omega is not None:
+        o.omega, o.n_omega = hold(np.atleast_1d(omega)), len(np.atleast_1d(omega))
+    if PT_raw is not None:
+        o.PT_raw = hold(PT_raw)
+    if inner_s is not None:
+        o.has_inner_s, o.inner_s = 1, int(inner_s)
+    if inner_tolerance is not None:
+        o.has_inner_tol, o.inner_tol = 1, float(inner_tolerance)
+    if inner_iterations is not None:
+        o.has_inner_it, o.inner_it = 1, int(inner_iterations)
+    return o
+
+
+def qmridr(A, b, s, resvec=False, **kw):
+    """Oracle QMRIDR (qmridr.inc); returns (x, flag, relres, iterations, resvec)."""
+    lib = load()
+    dt = np.complex128 if A.complex else np.float64
+    n = len(b)
+    b = np.ascontiguousarray(b, dtype=dt)
+    x = np.empty_like(b)
+    keep = []
+    o = _opts(A, n, dt, keep, **kw)
+    maxit = kw.get("maximum_iterations") or min(2 * n, 1000)
+    rv = None
+    if resvec:
+        rv = np.zeros(maxit + 1)
+        o.resvec, o.resvec_len = rv.ctypes.data, len(rv)
+    st = Stats()
+    f = lib.orc_c_qmridr if A.complex else lib.orc_r_qmridr
+    if f(ctypes.byref(A.m), n, c_void_p(b.ctypes.data), s, ctypes.byref(o), c_void_p(x.ctypes.data), ctypes.byref(st)):
+        raise ValueError("oracle qmridr rejected its arguments")
+    return x, st.flag, st.relres, st.iterations, (rv[: st.iterations + 1] if rv is not None else None)
+
+
+def _ms(name, A, b, sigma, s, resvec=False, colfac=False, **kw):
+    lib = load()
+    dt = np.complex128 if A.complex else np.float64
+    n = len(b)
+    b = np.ascontiguousarray(b, dtype=dt)
+    sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=dt)
+    ns = len(sig)
+    x = np.zeros((n, ns), dtype=dt, order="F")
+    keep = []
+    o = _opts(A, n, dt, keep, **kw)
+    maxit = kw.get("maximum_iterations") or min(2 * n, 1000)
+    rv = None
+    if resvec:
+        rv = np.zeros((maxit + 1, ns), order="F")
+        o.resvec, o.resvec_len = rv.ctypes.data, maxit + 1
+    rr = np.zeros(ns)
+    o.relres_vec = rr.ctypes.data
+    cf = None
+    if colfac:
+        cf = np.zeros(ns, dtype=dt)
+        o.colfac = cf.ctypes.data
+    st = Stats()
+    f = getattr(lib, f"orc_{'c' if A.complex else 'r'}_{name}")
+    if f(ctypes.byref(A.m), n, c_void_p(b.ctypes.data), ns, c_void_p(sig.ctypes.data), s, ctypes.byref(o),
+         c_void_p(x.ctypes.data), ctypes.byref(st)):
+        raise ValueError(f"oracle {name} rejected its arguments")
+    out = dict(x=x, flag=st.flag, relres=rr, iterations=st.iterations)
+    if rv is not None:
+        out["resvec"] = rv[: st.iterations + 1]
+    if cf is not None:
+        out["colfac"] = cf
+    return out
+
+
+def msidrs(A, b, sigma, s, **kw):
+    """Oracle MSIDRS (msidrs.inc); returns dict(x (n,n_sigma), flag, relres (n_sigma), iterations[, resvec, colfac])."""
+    return _ms("msidrs", A, b, sigma, s, **kw)
+
+
+def msqmridr(A, b, sigma, s, **kw):
+    """Oracle MSQMRIDR (msqmridr.inc)."""
+    return _ms("msqmridr", A, b, sigma, s, **kw)
+
+
+def solve(Am, bv):
+    lib = load()
+    cplx = np.iscomplexobj(Am) or np.iscomplexobj(bv)
+    dt = np.complex128 if cplx else np.float64
+    Am = np.asfortranarray(Am, dtype=dt)
+    bv = np.ascontiguousarray(bv, dtype=dt)
+    x = np.empty_like(bv)
+    f = lib.orc_c_solve if cplx else lib.orc_r_solve
+    f(c_int(len(bv)), c_void_p(Am.ctypes.data), c_void_p(bv.ctypes.data), c_void_p(x.ctypes.data))
+    return x
+
+
+def grot(a, b):
+    lib = load()
+    if np.iscomplexobj(a) or np.iscomplexobj(b):
+        c, s_, r = (np.zeros(1, np.complex128) for _ in range(3))
+        class Z(ctypes.Structure):  # double _Complex by value (two SSE words)
+            _fields_ = [("re", c_double), ("im", c_double)]
+
+        lib.orc_c_grot.argtypes = [Z, Z, c_void_p, c_void_p, c_void_p]
+        a, b = complex(a), complex(b)
+        lib.orc_c_grot(Z(a.real, a.imag), Z(b.real, b.imag),
+                       c_void_p(c.ctypes.data), c_void_p(s_.ctypes.data), c_void_p(r.ctypes.data))
+    else:
+        c, s_, r = (np.zeros(1) for _ in range(3))
+        lib.orc_r_grot.argtypes = [c_double, c_double, c_void_p, c_void_p, c_void_p]
+        lib.orc_r_grot(float(a), float(b), c_void_p(c.ctypes.data), c_void_p(s_.ctypes.data), c_void_p(r.ctypes.data))
+    return c[0], s_[0], r[0]
 
 
 def uniform01(seed, index):
