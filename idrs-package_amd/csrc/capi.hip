@@ -2,6 +2,7 @@
 #include "idrs_capi.h"
 
 #include "idrs.hpp"
+#include "krylov.hpp"
 
 #include <cstdarg>
 #include <cstring>
@@ -134,6 +135,82 @@ static int idrs_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs
             stats->solve_ms = ss.solve_ms;
             stats->spmv_ms = ss.spmv_ms;
             stats->spmv_launches = ss.spmv_launches;
+            stats->reserved = 0;
+        }
+    });
+}
+
+enum class Kry { QMRIDR, MSIDRS, MSQMRIDR };
+
+template <class T>
+static int krylov_entry(Kry kind, idrs_matrix_t A, int n, const double *b, int ns, const double *sigma, int s,
+                        const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats) {
+    return guarded([&] {
+        need(A && b && X, "null argument");
+        need(kind == Kry::QMRIDR || (sigma && ns >= 1), "sigma must hold at least one shift");
+        hipStream_t st = lib_stream();
+        idrs_krylov_opts_t o{};
+        if (opts) o = *opts;
+        KrylovOpts<T> ko;
+        ko.has_m1 = o.has_M1 != 0;
+        ko.M1 = (o.has_M1 && o.M1) ? &o.M1->op : nullptr;
+        if (o.has_tolerance) ko.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            ko.has_maxit = true;
+            ko.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) ko.variant = o.variant;
+        ko.has_inner_s = o.has_inner_s != 0;
+        ko.inner_s = o.inner_s;
+        ko.has_inner_tol = o.has_inner_tolerance != 0;
+        ko.inner_tol = o.inner_tolerance;
+        ko.has_inner_it = o.has_inner_iterations != 0;
+        ko.inner_it = o.inner_iterations;
+        ko.omega = (const T *)o.omega;
+        ko.n_omega = o.omega ? o.n_omega : 0;
+        ko.resvec = o.resvec;
+        ko.resvec_len = o.resvec_len;
+        ko.p_seed = o.p_seed ? o.p_seed : 1;
+        ko.colfac = (T *)o.colfac;
+        ko.want_colfac = kind == Kry::MSIDRS && o.colfac != nullptr;
+        const int nsig = kind == Kry::QMRIDR ? 1 : ns;
+        const bool pt_rows_s = !(kind == Kry::MSIDRS && ko.variant == 2);
+        KrylovStats ks;
+        auto run = [&](const T *db, T *dX) {
+            switch (kind) {
+            case Kry::QMRIDR: qmridr_solve<T>(A->op, n, s, db, dX, ko, ks, st); break;
+            case Kry::MSIDRS: msidrs_solve<T>(A->op, n, ns, (const T *)sigma, s, db, dX, ko, ks, st); break;
+            case Kry::MSQMRIDR: msqmridr_solve<T>(A->op, n, ns, (const T *)sigma, s, db, dX, ko, ks, st); break;
+            }
+        };
+        if (o.device_pointers) {
+            ko.x0 = (const T *)o.x0;
+            ko.PT_raw = (const T *)o.PT_raw;
+            run((const T *)b, (T *)X);
+        } else {
+            need(n >= 1, "empty system");
+            DVec<T> db((const T *)b, n, st), dX((size_t)n * nsig), dx0, dP;
+            if (o.x0 && kind == Kry::QMRIDR) {
+                dx0 = DVec<T>((const T *)o.x0, n, st);
+                ko.x0 = dx0.p;
+            }
+            if (o.PT_raw && pt_rows_s) {
+                need(s >= 1 && s <= SMAX, "s must be in 1..16");
+                dP = DVec<T>((const T *)o.PT_raw, (size_t)n * s, st);
+                ko.PT_raw = dP.p;
+            }
+            run(db.p, dX.p);
+            dX.download((T *)X, st);
+        }
+        if (relres && kind != Kry::QMRIDR)
+            for (int k = 0; k < ns; k++) relres[k] = ks.relres_vec[k];
+        if (stats) {
+            stats->flag = ks.flag;
+            stats->iterations = ks.iterations;
+            stats->relres = ks.relres;
+            stats->solve_ms = ks.solve_ms;
+            stats->spmv_ms = 0.0;
+            stats->spmv_launches = 0;
             stats->reserved = 0;
         }
     });
@@ -444,6 +521,31 @@ int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opt
 int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                    idrs_stats_t *stats) {
     return idrs_entry<c128>(A, n, b, s, opts, x, stats);
+}
+
+int idrs_qmridr_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_krylov_opts_t *opts, double *x,
+                    idrs_stats_t *stats) {
+    return krylov_entry<double>(Kry::QMRIDR, A, n, b, 1, nullptr, s, opts, x, nullptr, stats);
+}
+int idrs_qmridr_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_krylov_opts_t *opts, double *x,
+                     idrs_stats_t *stats) {
+    return krylov_entry<c128>(Kry::QMRIDR, A, n, b, 1, nullptr, s, opts, x, nullptr, stats);
+}
+int idrs_msidrs_f64(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                    const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats) {
+    return krylov_entry<double>(Kry::MSIDRS, A, n, b, n_sigma, sigma, s, opts, X, relres, stats);
+}
+int idrs_msidrs_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                     const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats) {
+    return krylov_entry<c128>(Kry::MSIDRS, A, n, b, n_sigma, sigma, s, opts, X, relres, stats);
+}
+int idrs_msqmridr_f64(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                      const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats) {
+    return krylov_entry<double>(Kry::MSQMRIDR, A, n, b, n_sigma, sigma, s, opts, X, relres, stats);
+}
+int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                       const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats) {
+    return krylov_entry<c128>(Kry::MSQMRIDR, A, n, b, n_sigma, sigma, s, opts, X, relres, stats);
 }
 
 int idrs_make_p_f64(int n, int s, int method, const double *PT_raw, const double *r, uint64_t seed, double *PT) {

@@ -446,17 +446,18 @@ template <class T>
 T *m1_diag(const KrylovOpts<T> &o, long n, Work<T> &w) {
     if (!o.has_m1 || !o.M1 || o.M1->which_precon != 1 || !o.M1->pdiag) return nullptr;
     if (o.M1->K->nrows != n) throw std::invalid_argument("M1 size does not match the system");
-    std::vector<typename HostT<T>::type> h(n);
     T *d = w.alloc(n);
     if (o.M1->pdiag_kind == ValKind::Complex) {
         if constexpr (!Tr<T>::is_complex) throw std::invalid_argument("complex preconditioner for a real solve");
         else HIP_TRY(hipMemcpyAsync(d, o.M1->pdiag, sizeof(T) * n, hipMemcpyDeviceToDevice, w.st));
     } else {
         std::vector<double> r(n);
-        HIP_TRY(hipMemcpy(r.data(), o.M1->pdiag, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpyAsync(r.data(), o.M1->pdiag, sizeof(double) * n, hipMemcpyDeviceToHost, w.st));
+        HIP_TRY(hipStreamSynchronize(w.st));
         std::vector<T> t(n);
         for (long i = 0; i < n; i++) t[i] = T(r[i]);
-        HIP_TRY(hipMemcpy(d, t.data(), sizeof(T) * n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(d, t.data(), sizeof(T) * n, hipMemcpyHostToDevice, w.st));
+        HIP_TRY(hipStreamSynchronize(w.st));
     }
     return d;
 }

@@ -9,13 +9,14 @@
 //   polpre        matrix_module.f90:266-304;  operator(/) :306-400
 //   chebyshev / neumann / compute_alpha  matrix_module.f90:851-1057
 //
-// Design: one thread per row; each 256-row block stages its contiguous
-// slice of values and column indices in LDS with coalesced loads, then each
-// thread runs the row's fused multiply-add chain in stored order (the
-// reference's summation order).  Consecutive rows make the per-position x
-// gathers of banded/stencil matrices contiguous across the wavefront.
-// Workgroups take contiguous chunks of row blocks and are remapped so that
-// each XCD streams one contiguous slab of rows (L2 reuse of x).
+// Design: CRS rows are handled by groups of G lanes (G a power of two near
+// the average row length), so a wavefront streams the values and column
+// indices of 64/G consecutive rows as one coalesced stretch; DIA/CDS rows
+// are one thread each, every diagonal a coalesced stream.  Workgroups take
+// contiguous slabs of rows and are remapped so that each XCD streams one
+// contiguous part of the matrix (L2 reuse of the x entries neighbouring rows
+// share).  The operator's follow-up work (Horner term, shadow-space dots,
+// omega dots, norms) is fused into the product as an epilogue.
 #include "matrix.hpp"
 
 #include <algorithm>
@@ -290,60 +291,14 @@ __device__ inline cplx<R> divv(cplx<R> x, cplx<R> d) { return x / d; }
 
 // ---------------------------------------------------------------------------
 // SpMV epilogues
+//
+// The operands of a row's finish and epilogue (z, the P row, r) do not depend
+// on the row sum, so the kernels load them next to the matrix entries and
+// without a branch around the load: an absent z is replaced by a valid
+// address whose value is ignored, and P components beyond prow re-read the
+// last one and are multiplied by zero.  One memory latency then covers all
+// loads of a row instead of one latency per operand.
 // ---------------------------------------------------------------------------
-template <class T, int EPI, int NV>
-__device__ inline void epi_accum(T (&e)[NV], const EpiArgs<T> &E, long q, T yv) {
-    using TT = Tr<T>;
-    if constexpr (EPI == EPI_PDOT) {
-        const T *p = E.P + q * E.prow;
-#pragma unroll
-        for (int i = 0; i < NV; i++)
-            if (i < E.prow) {
-                T pv = p[i];
-                if (E.conjP) pv = TT::conj(pv);
-                e[i] = TT::fma(pv, yv, e[i]);
-            }
-    } else if constexpr (EPI == EPI_OMEGA) {
-        T rr = E.r[q];
-        e[0] = TT::fma(TT::conj(rr), rr, e[0]);
-        e[1] = TT::fma(TT::conj(yv), yv, e[1]);
-        e[2] = TT::fma(TT::conj(yv), rr, e[2]);
-    } else if constexpr (EPI == EPI_NORM) {
-        e[0] = e[0] + T(TT::abs2(yv));
-    }
-}
-
-// Epilogue of a row whose value every lane of a G-lane group holds: for
-// P^T y lane g handles components g, g+G, ...; the other epilogues run on the
-// group's leader lane.  epi_store then sums over all lanes, which adds the
-// components of equal index.
-template <class T, int EPI, int NV, int G>
-__device__ inline void epi_accum_group(T (&e)[NV], const EpiArgs<T> &E, long q, T yv, int g) {
-    using TT = Tr<T>;
-    if constexpr (EPI == EPI_PDOT) {
-        const T *p = E.P + q * E.prow;
-#pragma unroll
-        for (int i = 0; i < NV; i++)
-            if (i % G == g && i < E.prow) {
-                T pv = p[i];
-                if (E.conjP) pv = TT::conj(pv);
-                e[i] = TT::fma(pv, yv, e[i]);
-            }
-    } else {
-        if (g == 0) epi_accum<T, EPI, NV>(e, E, q, yv);
-    }
-}
-
-template <class T, int EPI, int NV>
-__device__ inline void epi_store(T (&e)[NV], const EpiArgs<T> &E, T *lds) {
-    if constexpr (EPI != EPI_NONE) {
-        block_sum<T, NV>(e, lds);
-        if (threadIdx.x == 0)
-#pragma unroll
-            for (int i = 0; i < NV; i++) E.partials[(long)i * E.nparts + blockIdx.x] = e[i];
-    }
-}
-
 template <class VT, class T>
 struct SpmvArgs {
     int nrows, ncols;
@@ -361,22 +316,87 @@ struct SpmvArgs {
     T *y;
 };
 
-// y(row) from the row sum, without storing it
+// y(row) = xs*(acc - seed*x(row)/sdiag(row)) + a*z(row); zq holds z(row)
+// (ignored without z)
 template <class VT, class T>
-__device__ inline T finish_value(const SpmvArgs<VT, T> &A, long row, T acc) {
+__device__ inline T finish_value(const SpmvArgs<VT, T> &A, long row, T acc, T zq) {
     if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
     T yv = A.xs * acc;
-    if (A.z) yv = Tr<T>::fma(A.a, A.z[row], yv);
+    if (A.z) yv = Tr<T>::fma(A.a, zq, yv);
     return yv;
 }
 
-template <class VT, class T>
-__device__ inline T finish_row(const SpmvArgs<VT, T> &A, long row, T acc) {
-    if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
-    T yv = A.xs * acc;
-    if (A.z) yv = Tr<T>::fma(A.a, A.z[row], yv);
-    A.y[row] = yv;
-    return yv;
+// Operands of one row's epilogue: PC components of its P row (PDOT: this
+// lane's components c, c+step, ...), or r(row) (OMEGA).
+template <class T, int EPI, int PC>
+__device__ inline void epi_load(T (&pe)[PC], const EpiArgs<T> &E, long row, int c, int step) {
+    if constexpr (EPI == EPI_PDOT) {
+        const T *p = E.P + row * E.prow;
+        const int last = E.prow - 1;
+#pragma unroll
+        for (int k = 0; k < PC; k++) pe[k] = p[min(c + k * step, last)];
+    } else if constexpr (EPI == EPI_OMEGA) {
+        pe[0] = E.r[row];
+    }
+}
+
+// Accumulates a row into the epilogue sums: PDOT adds P components
+// c, c+step, ... (zero beyond prow) times y into e[0..PC); OMEGA adds
+// (r,r), (y,y), (y,r); NORM adds |y|^2.
+template <class T, int EPI, int NV, int PC>
+__device__ inline void epi_accum(T (&e)[NV], const T (&pe)[PC], const EpiArgs<T> &E, T yv, int c, int step) {
+    using TT = Tr<T>;
+    if constexpr (EPI == EPI_PDOT) {
+#pragma unroll
+        for (int k = 0; k < PC; k++) {
+            T pv = E.conjP ? TT::conj(pe[k]) : pe[k];
+            pv = (c + k * step < E.prow) ? pv : T(0);
+            e[k] = TT::fma(pv, yv, e[k]);
+        }
+    } else if constexpr (EPI == EPI_OMEGA) {
+        const T rr = pe[0];
+        e[0] = TT::fma(TT::conj(rr), rr, e[0]);
+        e[1] = TT::fma(TT::conj(yv), yv, e[1]);
+        e[2] = TT::fma(TT::conj(yv), rr, e[2]);
+    } else if constexpr (EPI == EPI_NORM) {
+        e[0] = e[0] + T(TT::abs2(yv));
+    }
+}
+
+// Block partials of the epilogue sums where every thread holds all NV sums.
+template <class T, int EPI, int NV>
+__device__ inline void epi_store(T (&e)[NV], const EpiArgs<T> &E, T *lds) {
+    if constexpr (EPI != EPI_NONE) {
+        block_sum<T, NV>(e, lds);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int i = 0; i < NV; i++) E.partials[(long)i * E.nparts + blockIdx.x] = e[i];
+    }
+}
+
+// Block partials of P^T y when lane g of every G-lane group holds the
+// components g, g+G, ... in e[0..PC): a butterfly over the lanes of equal g
+// inside the wavefront, then one LDS slot per (component, wavefront).
+template <class T, int NV, int G, int PC>
+__device__ inline void epi_store_group(T (&e)[PC], const EpiArgs<T> &E, T *lds) {
+    constexpr int NW = SPMV_THREADS / WAVE;
+#pragma unroll
+    for (int k = 0; k < PC; k++)
+#pragma unroll
+        for (int m = G; m < WAVE; m <<= 1) e[k] = e[k] + shfl_xor(e[k], m);
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    if (lane < G)
+#pragma unroll
+        for (int k = 0; k < PC; k++) {
+            const int i = lane + k * G;
+            if (i < NV) lds[i * NW + w] = e[k];
+        }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        T sum = lds[threadIdx.x * NW];
+        for (int v = 1; v < NW; v++) sum = sum + lds[threadIdx.x * NW + v];
+        E.partials[(long)threadIdx.x * E.nparts + blockIdx.x] = sum;
+    }
 }
 
 // Non-temporal loads for the matrix streams (read once per product): they
@@ -394,10 +414,11 @@ __device__ inline cplx<R> ldnt(const cplx<R> *p) {
 // coalesced stretch of values and column indices.  The block's row pointers
 // are staged in LDS once per chunk of rows, which leaves two dependent
 // memory latencies per row (column index, then x).  Each group keeps U rows
-// in flight: all their first-pass loads are issued before any is used; rows
-// longer than G finish in a strided remainder loop.  The group's partial
-// sums are combined by a butterfly inside the group, so every lane of the
-// group holds y(row) and the epilogue's loads are spread over the group.
+// in flight: all their first-pass loads, and the z / P / r operands of the
+// finish and epilogue, are issued before any is used; rows longer than G
+// finish in a strided remainder loop.  The group's partial sums are combined
+// by a butterfly inside the group, so every lane of the group holds y(row);
+// for P^T y lane g takes the P components g, g+G, ... of its rows.
 constexpr int CRS_CHUNK = 2048;   // rows whose row pointers sit in LDS at once
 
 template <class VT, class T, int EPI, int NV, int G>
@@ -405,11 +426,13 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
     if (info && *info >= 0) return;
     constexpr int GPB = SPMV_THREADS / G;
     constexpr int U = 4;
+    // epilogue operands per row and lane: P components (PDOT), r (OMEGA)
+    constexpr int PC = EPI == EPI_PDOT ? (NV + G - 1) / G : 1;
     __shared__ int s_rp[CRS_CHUNK + 1];
     __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
-    T e[NV];
+    T e[EPI == EPI_PDOT ? PC : NV];
 #pragma unroll
-    for (int i = 0; i < NV; i++) e[i] = T(0);
+    for (int i = 0; i < (EPI == EPI_PDOT ? PC : NV); i++) e[i] = T(0);
     const int g = threadIdx.x % G, grp = threadIdx.x / G;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const long n = A.nrows;
@@ -417,18 +440,20 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
     const T *__restrict__ x = A.x;
     const VT *__restrict__ val = A.val;
     const int *__restrict__ col = A.col;
+    const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
     for (long c0 = r0; c0 < r1; c0 += CRS_CHUNK) {
         const long c1 = min(c0 + (long)CRS_CHUNK, r1);
         __syncthreads();
         for (long q = c0 + threadIdx.x; q <= c1; q += SPMV_THREADS) s_rp[q - c0] = A.rp[q];
         __syncthreads();
         for (long base = c0; base < c1; base += (long)GPB * U) {
-            long row[U];
+            long row[U], rq[U];
             int a[U], b[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 row[u] = base + grp + (long)u * GPB;
                 const bool in = row[u] < c1;
+                rq[u] = in ? row[u] : c0;
                 a[u] = in ? s_rp[row[u] - c0] : 0;
                 b[u] = in ? s_rp[row[u] - c0 + 1] : 0;
             }
@@ -440,6 +465,12 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
                 const bool ok = j < b[u];
                 vv[u] = ok ? ldnt(&val[j]) : VT(0);
                 cc[u] = ok ? ldnt(&col[j]) : -1;
+            }
+            T zq[U], pe[U][PC];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                zq[u] = zp[rq[u]];
+                epi_load<T, EPI, PC>(pe[u], E, rq[u], g, G);
             }
             T acc[U];
 #pragma unroll
@@ -454,13 +485,18 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
 #pragma unroll
             for (int u = 0; u < U; u++)
                 if (row[u] < c1) {
-                    const T yv = finish_value(A, row[u], acc[u]);
+                    const T yv = finish_value(A, row[u], acc[u], zq[u]);
                     if (g == 0) A.y[row[u]] = yv;
-                    epi_accum_group<T, EPI, NV, G>(e, E, row[u], yv, g);
+                    if constexpr (EPI == EPI_PDOT) {
+                        epi_accum<T, EPI, PC, PC>(e, pe[u], E, yv, g, G);
+                    } else if (g == 0) {
+                        epi_accum<T, EPI, NV, PC>(e, pe[u], E, yv, 0, 1);
+                    }
                 }
         }
     }
-    epi_store<T, EPI, NV>(e, E, s_red);
+    if constexpr (EPI == EPI_PDOT) epi_store_group<T, NV, G, PC>(e, E, s_red);
+    else epi_store<T, EPI, NV>(e, E, s_red);
 }
 
 // DIA: one thread per row over a contiguous row range; every diagonal is a
@@ -468,6 +504,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
 template <class VT, class T, int EPI, int NV, int ND>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, EpiArgs<T> E, const int *info) {
     if (info && *info >= 0) return;
+    constexpr int PC = EPI == EPI_PDOT ? NV : 1;
     __shared__ int s_off[MAX_DIAG];
     __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
     const int ndiag = ND > 0 ? ND : A.ndiag;
@@ -482,10 +519,14 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
     const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
     const T *__restrict__ x = A.x;
     const VT *__restrict__ val = A.val;
+    const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
     for (long c = c0; c < c1; c++) {
         const long row = c * SPMV_THREADS + threadIdx.x;
         if (row >= n) continue;
         T acc = T(0);
+        const T zq = zp[row];
+        T pe[PC];
+        epi_load<T, EPI, PC>(pe, E, row, 0, 1);
         if constexpr (ND > 0) {
             VT vv[ND];
             T xx[ND];
@@ -504,8 +545,9 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
                 if (cc >= 0 && cc < ncols) acc = fmav(ldnt(&val[(long)d * n + row]), x[cc], acc);
             }
         }
-        T yv = finish_row(A, row, acc);
-        epi_accum<T, EPI, NV>(e, E, row, yv);
+        const T yv = finish_value(A, row, acc, zq);
+        A.y[row] = yv;
+        epi_accum<T, EPI, NV, PC>(e, pe, E, yv, 0, 1);
     }
     epi_store<T, EPI, NV>(e, E, s_red);
 }
@@ -544,6 +586,7 @@ static void dispatch_epi(const DevMatrix &K, const SpmvArgs<VT, T> &A, const Epi
     case EPI_NORM: launch_spmv<VT, T, EPI_NORM, 1>(K, A, E, info, st); break;
     case EPI_OMEGA: launch_spmv<VT, T, EPI_OMEGA, 3>(K, A, E, info, st); break;
     case EPI_PDOT:
+        if (E.prow < 1 || E.prow > 16 || !E.P) throw std::invalid_argument("P^T y epilogue needs 1..16 rows of P");
         if (E.prow <= 1) launch_spmv<VT, T, EPI_PDOT, 1>(K, A, E, info, st);
         else if (E.prow <= 2) launch_spmv<VT, T, EPI_PDOT, 2>(K, A, E, info, st);
         else if (E.prow <= 4) launch_spmv<VT, T, EPI_PDOT, 4>(K, A, E, info, st);

@@ -33,6 +33,33 @@ module idrs_capi
       integer(c_int32_t) :: time_spmv = 0
    end type idrs_opts_t
 
+! Optional arguments of QMRIDR, MSIDRS and MSQMRIDR (idrs_krylov_opts_t)
+   type, bind(C) :: idrs_krylov_opts_t
+      integer(c_int32_t) :: has_M1 = 0
+      integer(c_int32_t) :: has_tolerance = 0
+      integer(c_int32_t) :: has_maximum_iterations = 0
+      integer(c_int32_t) :: has_variant = 0
+      type(c_ptr)        :: M1 = c_null_ptr
+      real(c_double)     :: tolerance = 0._c_double
+      integer(c_int32_t) :: maximum_iterations = 0
+      integer(c_int32_t) :: variant = 0
+      integer(c_int32_t) :: has_inner_s = 0
+      integer(c_int32_t) :: inner_s = 0
+      integer(c_int32_t) :: has_inner_tolerance = 0
+      integer(c_int32_t) :: has_inner_iterations = 0
+      real(c_double)     :: inner_tolerance = 0._c_double
+      integer(c_int32_t) :: inner_iterations = 0
+      integer(c_int32_t) :: n_omega = 0
+      type(c_ptr)        :: x0 = c_null_ptr
+      type(c_ptr)        :: omega = c_null_ptr
+      type(c_ptr)        :: resvec = c_null_ptr
+      integer(c_int32_t) :: resvec_len = 0
+      integer(c_int32_t) :: device_pointers = 0
+      type(c_ptr)        :: PT_raw = c_null_ptr
+      integer(c_int64_t) :: p_seed = 0
+      type(c_ptr)        :: colfac = c_null_ptr
+   end type idrs_krylov_opts_t
+
    type, bind(C) :: idrs_stats_t
       integer(c_int32_t) :: flag = -1
       integer(c_int32_t) :: iterations = 0
@@ -278,6 +305,74 @@ module idrs_capi
          complex(c_double_complex), intent(in) :: b(*)
          type(idrs_opts_t), intent(in) :: opts
          complex(c_double_complex), intent(out) :: x(*)
+         type(idrs_stats_t), intent(out) :: stats
+      end function
+
+      integer(c_int) function idrs_qmridr_f64(A, n, b, s, opts, x, stats) bind(C, name="idrs_qmridr_f64")
+         import :: c_int, c_double, c_ptr, idrs_krylov_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, s
+         real(c_double), intent(in) :: b(*)
+         type(idrs_krylov_opts_t), intent(in) :: opts
+         real(c_double), intent(out) :: x(*)
+         type(idrs_stats_t), intent(out) :: stats
+      end function
+
+      integer(c_int) function idrs_qmridr_c128(A, n, b, s, opts, x, stats) bind(C, name="idrs_qmridr_c128")
+         import :: c_int, c_double_complex, c_ptr, idrs_krylov_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, s
+         complex(c_double_complex), intent(in) :: b(*)
+         type(idrs_krylov_opts_t), intent(in) :: opts
+         complex(c_double_complex), intent(out) :: x(*)
+         type(idrs_stats_t), intent(out) :: stats
+      end function
+
+      integer(c_int) function idrs_msidrs_f64(A, n, b, n_sigma, sigma, s, opts, X, relres, stats) &
+            bind(C, name="idrs_msidrs_f64")
+         import :: c_int, c_double, c_ptr, idrs_krylov_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, n_sigma, s
+         real(c_double), intent(in) :: b(*), sigma(*)
+         type(idrs_krylov_opts_t), intent(in) :: opts
+         real(c_double), intent(out) :: X(*)
+         real(c_double), intent(out) :: relres(*)
+         type(idrs_stats_t), intent(out) :: stats
+      end function
+
+      integer(c_int) function idrs_msidrs_c128(A, n, b, n_sigma, sigma, s, opts, X, relres, stats) &
+            bind(C, name="idrs_msidrs_c128")
+         import :: c_int, c_double_complex, c_double, c_ptr, idrs_krylov_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, n_sigma, s
+         complex(c_double_complex), intent(in) :: b(*), sigma(*)
+         type(idrs_krylov_opts_t), intent(in) :: opts
+         complex(c_double_complex), intent(out) :: X(*)
+         real(c_double), intent(out) :: relres(*)
+         type(idrs_stats_t), intent(out) :: stats
+      end function
+
+      integer(c_int) function idrs_msqmridr_f64(A, n, b, n_sigma, sigma, s, opts, X, relres, stats) &
+            bind(C, name="idrs_msqmridr_f64")
+         import :: c_int, c_double, c_ptr, idrs_krylov_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, n_sigma, s
+         real(c_double), intent(in) :: b(*), sigma(*)
+         type(idrs_krylov_opts_t), intent(in) :: opts
+         real(c_double), intent(out) :: X(*)
+         real(c_double), intent(out) :: relres(*)
+         type(idrs_stats_t), intent(out) :: stats
+      end function
+
+      integer(c_int) function idrs_msqmridr_c128(A, n, b, n_sigma, sigma, s, opts, X, relres, stats) &
+            bind(C, name="idrs_msqmridr_c128")
+         import :: c_int, c_double_complex, c_double, c_ptr, idrs_krylov_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, n_sigma, s
+         complex(c_double_complex), intent(in) :: b(*), sigma(*)
+         type(idrs_krylov_opts_t), intent(in) :: opts
+         complex(c_double_complex), intent(out) :: X(*)
+         real(c_double), intent(out) :: relres(*)
          type(idrs_stats_t), intent(out) :: stats
       end function
 

@@ -7,6 +7,7 @@ the IDRS solver); see api.py.
 from ._lib import IdrsError, load as load_library  # noqa: F401
 from .api import (  # noqa: F401
     IdrsResult,
+    KrylovResult,
     Matrix,
     UserMatrix,
     cds_format,
@@ -17,6 +18,9 @@ from .api import (  # noqa: F401
     dia_format,
     idrs,
     make_p,
+    msidrs,
+    msqmridr,
+    qmridr,
     real_diagonal,
     real_preconditioner,
     scaleback,

@@ -47,6 +47,34 @@ class idrs_opts_t(ctypes.Structure):
     ]
 
 
+class idrs_krylov_opts_t(ctypes.Structure):
+    _fields_ = [
+        ("has_M1", c_int32),
+        ("has_tolerance", c_int32),
+        ("has_maximum_iterations", c_int32),
+        ("has_variant", c_int32),
+        ("M1", c_void_p),
+        ("tolerance", c_double),
+        ("maximum_iterations", c_int32),
+        ("variant", c_int32),
+        ("has_inner_s", c_int32),
+        ("inner_s", c_int32),
+        ("has_inner_tolerance", c_int32),
+        ("has_inner_iterations", c_int32),
+        ("inner_tolerance", c_double),
+        ("inner_iterations", c_int32),
+        ("n_omega", c_int32),
+        ("x0", c_void_p),
+        ("omega", c_void_p),
+        ("resvec", c_void_p),
+        ("resvec_len", c_int32),
+        ("device_pointers", c_int32),
+        ("PT_raw", c_void_p),
+        ("p_seed", c_uint64),
+        ("colfac", c_void_p),
+    ]
+
+
 class idrs_stats_t(ctypes.Structure):
     _fields_ = [
         ("flag", c_int32),
@@ -99,6 +127,18 @@ _PROTOS = {
     "idrs_scaleback_c128": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_idrs_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, POINTER(idrs_stats_t)]),
     "idrs_idrs_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, POINTER(idrs_stats_t)]),
+    "idrs_qmridr_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t), c_void_p,
+                                POINTER(idrs_stats_t)]),
+    "idrs_qmridr_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t), c_void_p,
+                                 POINTER(idrs_stats_t)]),
+    "idrs_msidrs_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
+                                c_void_p, c_void_p, POINTER(idrs_stats_t)]),
+    "idrs_msidrs_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
+                                 c_void_p, c_void_p, POINTER(idrs_stats_t)]),
+    "idrs_msqmridr_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
+                                  c_void_p, c_void_p, POINTER(idrs_stats_t)]),
+    "idrs_msqmridr_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
+                                   c_void_p, c_void_p, POINTER(idrs_stats_t)]),
     "idrs_make_p_f64": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_uint64, c_void_p]),
     "idrs_make_p_c128": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_uint64, c_void_p]),
 }
@@ -106,11 +146,29 @@ _PROTOS = {
 _lib = None
 
 
+def _init_torch_runtime_first():
+    """torch ships a HIP runtime of its own beside the system one that
+    libidrs_hip.so links.  With both in one process, torch finds no device
+    if the system runtime initialised the GPU first, so torch's runtime is
+    brought up before the library is loaded (device tensors then pass
+    between the two as plain device addresses)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    try:
+        if torch.cuda.device_count() > 0:
+            torch.cuda.init()
+    except RuntimeError:
+        pass
+
+
 def load():
     """Load libidrs_hip.so (built in-tree by __graft_entry__.build())."""
     global _lib
     if _lib is not None:
         return _lib
+    _init_torch_runtime_first()
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')"

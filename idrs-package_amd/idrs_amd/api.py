@@ -19,9 +19,12 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import check, idrs_opts_t, idrs_stats_t
+from ._lib import check, idrs_krylov_opts_t, idrs_opts_t, idrs_stats_t
 
 IdrsResult = namedtuple("IdrsResult", "x flag relres iterations resvec solve_ms spmv_ms spmv_launches")
+# relres: per-shift relative residuals (array) for the multishift solvers;
+# colfac: MSIDRS collinearity factors when asked for, else None
+KrylovResult = namedtuple("KrylovResult", "x flag relres iterations resvec solve_ms colfac")
 
 
 def _ptr(a):
@@ -351,3 +354,137 @@ def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None
     if rv is not None:
         rv = rv[: st.iterations + 1]
     return IdrsResult(x, st.flag, st.relres, st.iterations, rv, st.solve_ms, st.spmv_ms, st.spmv_launches)
+
+
+def _krylov(kind, A, b, s, sigma=None, M1=None, tolerance=None, maximum_iterations=None, variant=None,
+            inner_s=None, inner_tolerance=None, inner_iterations=None, x0=None, omega=None, resvec=False,
+            colfac=False, PT_raw=None, p_seed=None, out=None):
+    lib = _lib.load()
+    device = _is_torch(b)
+    n = int(b.shape[0])
+    o = idrs_krylov_opts_t()
+    keep = []
+    if M1 is not None:
+        o.has_M1 = 1
+        o.M1 = M1.handle if isinstance(M1, Matrix) else None
+    if tolerance is not None:
+        if tolerance < 0:
+            raise ValueError("Illegal value for parameter tolerance")
+        o.has_tolerance, o.tolerance = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    if inner_s is not None:
+        o.has_inner_s, o.inner_s = 1, int(inner_s)
+    if inner_tolerance is not None:
+        o.has_inner_tolerance, o.inner_tolerance = 1, float(inner_tolerance)
+    if inner_iterations is not None:
+        o.has_inner_iterations, o.inner_iterations = 1, int(inner_iterations)
+    maxit = int(maximum_iterations) if maximum_iterations is not None else min(2 * n, 1000)
+    ms = kind != "qmridr"
+    if device:
+        import torch
+
+        cplx = b.is_complex() or bool(sigma is not None and np.iscomplexobj(sigma)) or A.is_complex
+        dt = torch.complex128 if cplx else torch.float64
+        if b.dtype != dt or not b.is_contiguous():
+            raise TypeError(f"device b must be a contiguous {dt} CUDA tensor")
+        npdt = np.complex128 if cplx else np.float64
+    else:
+        cplx = A.is_complex or np.iscomplexobj(b) or bool(sigma is not None and np.iscomplexobj(sigma))
+        npdt = np.complex128 if cplx else np.float64
+        b = np.ascontiguousarray(b, dtype=npdt)
+    ns = 1
+    sp = None
+    if ms:
+        sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=npdt)
+        keep.append(sig)
+        ns, sp = len(sig), sig.ctypes.data
+    if device:
+        X = out if out is not None else torch.empty((ns, n) if ms else (n,), dtype=dt, device=b.device)
+        o.device_pointers = 1
+        for name, arr in (("x0", x0), ("PT_raw", PT_raw)):
+            if arr is not None:
+                if not (_is_torch(arr) and arr.is_cuda and arr.dtype == dt and arr.is_contiguous()):
+                    raise TypeError(f"{name} must be a contiguous CUDA tensor of dtype {dt}")
+                setattr(o, name, arr.data_ptr())
+        bp, xp = ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(X.data_ptr())
+        torch.cuda.current_stream().synchronize()
+    else:
+        X = np.empty((ns, n) if ms else n, dtype=npdt)  # row k = solution of shift k (column-major X(n,ns))
+        if x0 is not None:
+            a = np.ascontiguousarray(x0, dtype=npdt)
+            keep.append(a)
+            o.x0 = a.ctypes.data
+        if PT_raw is not None:
+            a = np.ascontiguousarray(PT_raw, dtype=npdt)
+            keep.append(a)
+            o.PT_raw = a.ctypes.data
+        bp, xp = _ptr(b), _ptr(X)
+    if omega is not None:
+        om = np.ascontiguousarray(np.atleast_1d(omega), dtype=npdt)
+        keep.append(om)
+        o.omega, o.n_omega = om.ctypes.data, len(om)
+    rv = None
+    if resvec:
+        rv = np.zeros((ns, maxit + 1))  # resvec(maxit+1, ns) column-major
+        o.resvec, o.resvec_len = rv.ctypes.data, maxit + 1
+    cf = None
+    if colfac:
+        cf = np.zeros(ns, dtype=npdt)
+        o.colfac = cf.ctypes.data
+    o.p_seed = int(p_seed) if p_seed is not None else 0
+    st = idrs_stats_t()
+    sfx = "c128" if cplx else "f64"
+    if ms:
+        rr = np.zeros(ns)
+        f = getattr(lib, f"idrs_{kind}_{sfx}")
+        check(f(A.handle, n, bp, ns, sp, int(s), ctypes.byref(o), xp, rr.ctypes.data, ctypes.byref(st)))
+        relres = rr
+    else:
+        f = getattr(lib, f"idrs_qmridr_{sfx}")
+        check(f(A.handle, n, bp, int(s), ctypes.byref(o), xp, ctypes.byref(st)))
+        relres = st.relres
+    if rv is not None:
+        rv = rv[:, : st.iterations + 1]
+        if not ms:
+            rv = rv[0]
+    if ms and not device:
+        X = X.T  # (n, ns) view, like the reference's x(n, n_sigma)
+    elif ms:
+        X = X.t()
+    return KrylovResult(X, st.flag, relres, st.iterations, rv, st.solve_ms, cf)
+
+
+def qmridr(A, b, s, M1=None, tolerance=None, maximum_iterations=None, inner_s=None, inner_tolerance=None,
+           inner_iterations=None, x0=None, omega=None, resvec=False, PT_raw=None, p_seed=None, out=None):
+    """x = QMRIDR(A, b, s, M1, tolerance, maximum_iterations, flag, relres,
+    iterations, inner_s, inner_tolerance, inner_iterations, x0, omega,
+    resvec)  (idrs_module.f90:253-408).  Returns KrylovResult."""
+    return _krylov("qmridr", A, b, s, M1=M1, tolerance=tolerance, maximum_iterations=maximum_iterations,
+                   inner_s=inner_s, inner_tolerance=inner_tolerance, inner_iterations=inner_iterations, x0=x0,
+                   omega=omega, resvec=resvec, PT_raw=PT_raw, p_seed=p_seed, out=out)
+
+
+def msidrs(A, b, sigma, s, M1=None, tolerance=None, maximum_iterations=None, variant=None, omega=None,
+           resvec=False, colfac=False, PT_raw=None, p_seed=None, out=None):
+    """X = MSIDRS(A, b, sigma, s, M1, tolerance, maximum_iterations,
+    variant, flag, relres, iterations, omega, resvec, colfac)
+    (idrs_module.f90:572-713).  X has shape (n, n_sigma); relres and colfac
+    hold one entry per shift; resvec has shape (n_sigma, iterations+1)."""
+    return _krylov("msidrs", A, b, s, sigma=sigma, M1=M1, tolerance=tolerance,
+                   maximum_iterations=maximum_iterations, variant=variant, omega=omega, resvec=resvec,
+                   colfac=colfac, PT_raw=PT_raw, p_seed=p_seed, out=out)
+
+
+def msqmridr(A, b, sigma, s, M1=None, tolerance=None, maximum_iterations=None, inner_s=None,
+             inner_tolerance=None, inner_iterations=None, omega=None, resvec=False, PT_raw=None, p_seed=None,
+             out=None):
+    """X = MSQMRIDR(A, b, sigma, s, M1, tolerance, maximum_iterations, flag,
+    relres, iterations, inner_s, inner_tolerance, inner_iterations, omega,
+    resvec)  (idrs_module.f90:410-570)."""
+    return _krylov("msqmridr", A, b, s, sigma=sigma, M1=M1, tolerance=tolerance,
+                   maximum_iterations=maximum_iterations, inner_s=inner_s, inner_tolerance=inner_tolerance,
+                   inner_iterations=inner_iterations, omega=omega, resvec=resvec, PT_raw=PT_raw, p_seed=p_seed,
+                   out=out)

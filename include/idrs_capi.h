@@ -191,6 +191,62 @@ int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opt
 int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                    idrs_stats_t *stats);
 
+/* Optional arguments of QMRIDR, MSIDRS and MSQMRIDR
+   (MODULES/IDRS/idrs_module.f90:253-713).  has_* flags are present(). */
+typedef struct {
+    int32_t has_M1;
+    int32_t has_tolerance;
+    int32_t has_maximum_iterations;
+    int32_t has_variant;        /* MSIDRS: 1 msidrs, 2 msbicgstab         */
+    idrs_matrix_t M1;           /* QMRIDR: preconditioner; MS*: mass matrix */
+    double tolerance;
+    int32_t maximum_iterations;
+    int32_t variant;
+    int32_t has_inner_s;        /* QMRIDR, MSQMRIDR                       */
+    int32_t inner_s;
+    int32_t has_inner_tolerance;
+    int32_t has_inner_iterations;
+    double inner_tolerance;
+    int32_t inner_iterations;
+    int32_t n_omega;
+    const void *x0;             /* QMRIDR: x0(n)                          */
+    const void *omega;          /* omega(n_omega), host                   */
+    double *resvec;             /* resvec(resvec_len, n_sigma), host, output */
+    int32_t resvec_len;         /* leading dimension of resvec, >= maxit+1 */
+    int32_t device_pointers;    /* b, x, x0, PT_raw are device pointers   */
+    const void *PT_raw;         /* raw shadow space PT(s,n); NULL: uniform stream */
+    uint64_t p_seed;
+    void *colfac;               /* MSIDRS: colfac(n_sigma), host, output  */
+} idrs_krylov_opts_t;
+
+/* x = QMRIDR(A, b, s, [M1, tolerance, maximum_iterations, flag, relres,
+   iterations, inner_s, inner_tolerance, inner_iterations, x0, omega,
+   resvec]): RQMRIDR, idrs_module.f90:331-408 with qmridr.inc:1-263. */
+int idrs_qmridr_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_krylov_opts_t *opts, double *x,
+                    idrs_stats_t *stats);
+/* CQMRIDR, idrs_module.f90:253-329 */
+int idrs_qmridr_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_krylov_opts_t *opts, double *x,
+                     idrs_stats_t *stats);
+/* X(n,n_sigma) = MSIDRS(A, b, sigma, s, [M1, tolerance,
+   maximum_iterations, variant, flag, relres(n_sigma), iterations, omega,
+   resvec, colfac]): RMSIDRS, idrs_module.f90:572-641 with msidrs.inc:1-334.
+   sigma is a host array (complex: interleaved pairs); relres may be NULL;
+   stats->relres is the largest of relres. */
+int idrs_msidrs_f64(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                    const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats);
+/* CMSIDRS, idrs_module.f90:643-713 */
+int idrs_msidrs_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                     const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats);
+/* X(n,n_sigma) = MSQMRIDR(A, b, sigma, s, [M1, tolerance,
+   maximum_iterations, flag, relres(n_sigma), iterations, inner_s,
+   inner_tolerance, inner_iterations, omega, resvec]): RMSQMRIDR,
+   idrs_module.f90:491-570 with msqmridr.inc:1-293. */
+int idrs_msqmridr_f64(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                      const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats);
+/* CMSQMRIDR, idrs_module.f90:410-489 */
+int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
+                       const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats);
+
 /* PT = MAKE_P(...) for inspection: the orthonormalised shadow space PT(s,n)
    (RMAKE_P / CMAKE_P, idrs_module.f90:839-937).  method 1 Gram-Schmidt of
    PT_raw (or the library stream when NULL); method 2 PT = conj(r). Host arrays. */

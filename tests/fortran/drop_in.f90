@@ -15,6 +15,7 @@ program drop_in
    call toeplitz_real()
    call cdr_preconditioned()
    call toeplitz_complex()
+   call krylov_family()
 
 contains
 
@@ -139,5 +140,52 @@ contains
       write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'toeplitz_complex_s4', ' iter', iter, ' flag', flag, &
          ' relres', relres, ' err', norm( b - A%K*x )/norm( b )
    end subroutine toeplitz_complex
+
+! QMRIDR, MSIDRS and MSQMRIDR on the real Toeplitz system of toeplitz_real
+! and its shifted versions (K - sigma I) x = b; err is the true relative
+! residual of each shifted system.
+   subroutine krylov_family()
+      integer, parameter            :: N = 60, maxit = 4000, ns = 3
+      real(kind=rp), parameter      :: Pe = 0.5
+      type(matrix)                  :: A, M
+      real(kind=rp)                 :: x(N), b(N), X2(N,ns), sig(ns), rr(ns), tol, relres
+      integer                       :: row_ptr(N+1), col_ind(3*N), flag, iter, i, k
+      real(kind=rp)                 :: val(3*N)
+      character(len=24)             :: name
+
+      k = 0
+      row_ptr(1) = 1
+      do i = 1, N
+         if ( i > 1 ) then
+            k = k + 1; col_ind(k) = i-1; val(k) = -1._rp-Pe
+         end if
+         k = k + 1; col_ind(k) = i; val(k) = 2._rp
+         if ( i < N ) then
+            k = k + 1; col_ind(k) = i+1; val(k) = -1._rp+Pe
+         end if
+         row_ptr(i+1) = k + 1
+      end do
+      A%K = crs_format( N, N, row_ptr, col_ind(1:k), val(1:k) )
+      b = 0.
+      b(1) = 1._rp+Pe
+      b(N) = 1._rp-Pe
+      tol = 1.e-8_rp
+      x = qmridr( A, b, 4, M, tol, maxit, flag, relres, iter )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'qmridr_s4', ' iter', iter, ' flag', flag, &
+         ' relres', relres, ' err', maxval(abs(x-1._rp))
+      sig = [0._rp, -0.5_rp, -2._rp]
+      X2 = msidrs( A, b, sig, 4, tolerance=tol, maximum_iterations=maxit, flag=flag, relres=rr, iterations=iter )
+      do i = 1, ns
+         write(name,'(a,i0)') 'msidrs_shift', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', rr(i), ' err', norm( b - (A%K*X2(:,i) - sig(i)*X2(:,i)) )/norm( b )
+      end do
+      X2 = msqmridr( A, b, sig, 4, tolerance=tol, maximum_iterations=maxit, flag=flag, relres=rr, iterations=iter )
+      do i = 1, ns
+         write(name,'(a,i0)') 'msqmridr_shift', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', rr(i), ' err', norm( b - (A%K*X2(:,i) - sig(i)*X2(:,i)) )/norm( b )
+      end do
+   end subroutine krylov_family
 
 end program drop_in

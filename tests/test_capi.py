@@ -4,8 +4,10 @@ the library reports errors instead of crashing when no device is present."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
+import pytest
 
 import idrs_amd
 from idrs_amd import _lib
@@ -32,11 +34,30 @@ def test_ctypes_binding_covers_header():
     assert set(declared_functions()) == set(_lib.exported_symbols())
 
 
-def test_opts_struct_layout():
-    # the Fortran bind(C) type and the ctypes structure mirror the C layout
-    assert ctypes.sizeof(_lib.idrs_opts_t) == 104
-    assert _lib.idrs_opts_t.p_seed.offset == 88
-    assert ctypes.sizeof(_lib.idrs_stats_t) == 40
+def _c_layout(struct, fields, tmp_path):
+    """sizeof and offsetof of every field as the C compiler lays out the
+    header's struct."""
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "idrs_capi.h"', "int main(void){",
+             f'printf("%zu\\n", sizeof({struct}));']
+    lines += [f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields]
+    lines.append("return 0;}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    return int(out[0]), [int(v) for v in out[1:]]
+
+
+@pytest.mark.parametrize("struct", ["idrs_opts_t", "idrs_krylov_opts_t", "idrs_stats_t"])
+def test_struct_layout_matches_header(struct, tmp_path):
+    # the ctypes structures (and the Fortran bind(C) types, which list the
+    # same fields in the same order) mirror the C layout field by field
+    cls = getattr(_lib, struct)
+    names = [f[0] for f in cls._fields_]
+    size, offs = _c_layout(struct, names, tmp_path)
+    assert ctypes.sizeof(cls) == size
+    assert [getattr(cls, f).offset for f in names] == offs
 
 
 def test_no_device_reports_error():

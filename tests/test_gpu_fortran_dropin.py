@@ -62,3 +62,23 @@ def test_cdr_pp_idrs_sequence(cases):
 def test_complex_toeplitz(cases):
     c = cases["toeplitz_complex_s4"]
     assert c["flag"] == 0 and c["err"] <= 1e-12
+
+
+def test_qmridr(cases):
+    c = cases["qmridr_s4"]
+    crs, _, b = P.toeplitz_1d()
+    Ao = orc.OracleMatrix(orc.CRS, 60, crs=crs)
+    _, fo, _, io, _ = orc.qmridr(Ao, b, 4, M1=Ao, tolerance=1e-8, maximum_iterations=4000)
+    assert c["flag"] == fo == 0 and abs(c["iter"] - io) <= 2 and c["err"] < 1e-6
+
+
+@pytest.mark.parametrize("solver", ["msidrs", "msqmridr"])
+def test_multishift(cases, solver):
+    crs, _, b = P.toeplitz_1d()
+    Ao = orc.OracleMatrix(orc.CRS, 60, crs=crs)
+    sig = np.array([0.0, -0.5, -2.0])
+    ref = getattr(orc, solver)(Ao, b, sig, 4, tolerance=1e-8, maximum_iterations=4000)
+    for k in range(3):
+        c = cases[f"{solver}_shift{k + 1}"]
+        assert c["flag"] == ref["flag"] == 0 and abs(c["iter"] - ref["iterations"]) <= 2
+        assert c["err"] < 1e-7 and c["relres"] < 1e-7
