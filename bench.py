@@ -112,6 +112,110 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8, "dispatches": 5}))
 
 
+def batched_systems(nsys, seed=0):
+    """The batched workload: nsys 2-D 5-point convection-diffusion systems,
+    side floor(32 * 16^u) with u ~ U(0,1), Pe ~ U[1,100], convection angle
+    ~ U[0, 2 pi), b ~ U[-1,1] (all seeded)."""
+    import numpy as np
+    from idrs_amd import problems as P
+
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0.0, 1.0, nsys)
+    pe = rng.uniform(1.0, 100.0, nsys)
+    ang = rng.uniform(0.0, 2 * np.pi, nsys)
+    out = []
+    for k in range(nsys):
+        m = int(np.floor(32 * 16 ** u[k]))
+        crs, _ = P.convection_diffusion_2d(m, pe[k], angle=ang[k])
+        out.append((crs, rng.uniform(-1.0, 1.0, crs.n)))
+    return out
+
+
+def bench_batched(args):
+    """One batched IDR(2) solve of all systems vs a loop of single-system
+    solves on the same device-resident data."""
+    import numpy as np
+    import torch
+
+    import idrs_amd
+
+    t0 = time.time()
+    systems = batched_systems(args.nsys)
+    ntot = sum(c.n for c, _ in systems)
+    nnz = sum(c.nnz for c, _ in systems)
+    print(f"# {args.nsys} systems built: rows={ntot} nnz={nnz} ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    Ks = [idrs_amd.crs_format(c.n, c.n, c.row_ptr, c.col_ind, rval=c.val) for c, _ in systems]
+    KB = idrs_amd.batch_format(Ks)
+    AB = idrs_amd.Matrix(KB)
+    b_all = torch.from_numpy(np.concatenate([bb for _, bb in systems])).to(dev)
+    x_all = torch.empty_like(b_all)
+    s, tol, maxit = 2, 1e-8, args.batch_maxit
+
+    def batch_step():
+        return idrs_amd.idrs_batched(AB, b_all, s, tolerance=tol, maximum_iterations=maxit, out=x_all)
+
+    for _ in range(args.warmup):
+        batch_step()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        rb = batch_step()
+    torch.cuda.synchronize()
+    batch_ms = (time.perf_counter() - t_start) * 1e3 / args.steps
+    print(f"# batched: {batch_ms:.1f} ms, flags {np.bincount(rb.flag)}, iterations min/median/max "
+          f"{rb.iterations.min()}/{int(np.median(rb.iterations))}/{rb.iterations.max()}", file=sys.stderr, flush=True)
+    # per-system loop over the same device data
+    As = [idrs_amd.Matrix(K) for K in Ks]
+    offs = np.cumsum([0] + [c.n for c, _ in systems])
+    xs = torch.empty_like(b_all)
+    bs = [b_all[offs[k]:offs[k + 1]] for k in range(args.nsys)]
+    outs = [xs[offs[k]:offs[k + 1]] for k in range(args.nsys)]
+    loop_its = np.zeros(args.nsys, dtype=int)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.nsys):
+        r = idrs_amd.idrs(As[k], bs[k], s, tolerance=tol, maximum_iterations=maxit, out=outs[k])
+        loop_its[k] = r.iterations
+    torch.cuda.synchronize()
+    loop_ms = (time.perf_counter() - t_start) * 1e3
+    diff = float((x_all - xs).norm() / xs.norm())
+    print(f"# per-system loop: {loop_ms:.1f} ms; max |it batch - it loop| = "
+          f"{int(np.abs(rb.iterations - loop_its).max())}; ||x_batch - x_loop|| / ||x_loop|| = {diff:.2e}",
+          file=sys.stderr, flush=True)
+    # algorithmic bytes of the work done (each system's own iterations)
+    it_bytes = 0.0
+    for (c, _), it in zip(systems, rb.iterations):
+        b_spmv = 12 * c.nnz + 4 * (c.n + 1) + 16 * c.n
+        it_bytes += it * (b_spmv + 8 * c.n * (3 * s * s + 11 * s + 3) / (s + 1))
+    gbs = it_bytes / (batch_ms * 1e-3) / 1e9
+    out = {
+        "metric": "batched IDR(2) time-to-solution (ms) for 512 2-D convection-diffusion systems, vs a per-system loop",
+        "value": round(batch_ms, 2),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_ms, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.nsys} 2-D 5-point convection-diffusion systems, side floor(32*16^u), "
+                               f"Pe U[1,100], IDR(2), tol 1e-8, FP64 CRS", "rows": int(ntot), "nnz": int(nnz),
+                   "per_system_loop_ms": round(loop_ms, 2), "speedup_vs_loop": round(loop_ms / batch_ms, 2),
+                   "iterations_median": int(np.median(rb.iterations)), "iterations_max": int(rb.iterations.max()),
+                   "flags": {str(k): int(v) for k, v in enumerate(np.bincount(rb.flag))},
+                   "x_rel_diff_vs_loop": diff},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole batched solve (algorithmic bytes of each system's iterations)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +226,10 @@ def main():
     ap.add_argument("--degree", type=int, default=8)
     ap.add_argument("--format", choices=["both", "crs", "dia"], default="both",
                     help="storage: both (headline DIA, then CRS), or one of them")
+    ap.add_argument("--form", choices=["solve", "batched"], default="solve",
+                    help="solve: the 200^3 IDR(4)+Chebyshev solve (default); batched: many small systems")
+    ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
+    ap.add_argument("--batch-maxit", type=int, default=4000)
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -134,6 +242,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and not measured.", file=sys.stderr)
         sys.exit(2)
+    if args.form == "batched":
+        bench_batched(args)
+        return
 
     import numpy as np
     import torch

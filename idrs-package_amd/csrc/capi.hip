@@ -3,6 +3,7 @@
 
 #include "idrs.hpp"
 #include "krylov.hpp"
+#include "batch.hpp"
 
 #include <cstdarg>
 #include <cstring>
@@ -212,6 +213,62 @@ static int krylov_entry(Kry kind, idrs_matrix_t A, int n, const double *b, int n
             stats->spmv_ms = 0.0;
             stats->spmv_launches = 0;
             stats->reserved = 0;
+        }
+    });
+}
+
+template <class T>
+static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
+                         idrs_stats_t *stats) {
+    return guarded([&] {
+        need(A && b && x && stats, "null argument");
+        const Partition *part = A->op.K->part.get();
+        need(part != nullptr, "not a batch matrix (idrs_batch_format)");
+        hipStream_t st = lib_stream();
+        idrs_opts_t o{};
+        if (opts) o = *opts;
+        need(!o.has_M1 && !o.U0 && !o.resvec, "M1, U0 and resvec are not offered for batches");
+        SolveOpts<T> so;
+        if (o.has_tolerance) so.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            so.has_maxit = true;
+            so.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) so.variant = o.variant;
+        so.omega = (const T *)o.omega;
+        so.n_omega = o.omega ? o.n_omega : 0;
+        so.p_seed = o.p_seed ? o.p_seed : 1;
+        const int nsys = part->nsys;
+        const long ntot = part->h_row0[nsys];
+        const int prow = (so.variant == 1) ? s : 1;
+        std::vector<SolveStats> ss;
+        double ms = 0.0;
+        if (o.device_pointers) {
+            so.x0 = (const T *)o.x0;
+            so.PT_raw = (const T *)o.PT_raw;
+            batch_idrs_solve<T>(A->op, s, (const T *)b, (T *)x, so, ss, ms, st);
+        } else {
+            DVec<T> db((const T *)b, ntot, st), dx(ntot), dx0, dP;
+            if (o.x0) {
+                dx0 = DVec<T>((const T *)o.x0, ntot, st);
+                so.x0 = dx0.p;
+            }
+            if (o.PT_raw) {
+                need(s >= 1 && s <= SMAX, "s must be in 1..16");
+                dP = DVec<T>((const T *)o.PT_raw, (size_t)ntot * prow, st);
+                so.PT_raw = dP.p;
+            }
+            batch_idrs_solve<T>(A->op, s, db.p, dx.p, so, ss, ms, st);
+            dx.download((T *)x, st);
+        }
+        for (int k = 0; k < nsys; k++) {
+            stats[k].flag = ss[k].flag;
+            stats[k].iterations = ss[k].iterations;
+            stats[k].relres = ss[k].relres;
+            stats[k].solve_ms = ms;
+            stats[k].spmv_ms = 0.0;
+            stats[k].spmv_launches = 0;
+            stats[k].reserved = 0;
         }
     });
 }
@@ -546,6 +603,36 @@ int idrs_msqmridr_f64(idrs_matrix_t A, int n, const double *b, int n_sigma, cons
 int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
                        const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats) {
     return krylov_entry<c128>(Kry::MSQMRIDR, A, n, b, n_sigma, sigma, s, opts, X, relres, stats);
+}
+
+int idrs_batch_format(int nsys, const idrs_user_matrix_t *K, idrs_user_matrix_t *A) {
+    return guarded([&] {
+        need(K && A && nsys >= 1, "bad arguments");
+        std::vector<const DevMatrix *> ms(nsys);
+        for (int k = 0; k < nsys; k++) {
+            need(K[k] != nullptr, "null system matrix");
+            ms[k] = K[k]->M;
+        }
+        std::unique_ptr<DevMatrix> M(make_blockdiag(nsys, ms.data()));
+        *A = new idrs_user_matrix_s{M.release()};
+    });
+}
+int idrs_batch_info(idrs_user_matrix_t A, int *nsys, int64_t *row0) {
+    return guarded([&] {
+        need(A && nsys, "null argument");
+        const Partition *p = A->M->part.get();
+        *nsys = p ? p->nsys : 0;
+        if (p && row0)
+            for (int k = 0; k <= p->nsys; k++) row0[k] = p->h_row0[k];
+    });
+}
+int idrs_idrs_batched_f64(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
+                          idrs_stats_t *stats) {
+    return batched_entry<double>(A, b, s, opts, x, stats);
+}
+int idrs_idrs_batched_c128(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
+                           idrs_stats_t *stats) {
+    return batched_entry<c128>(A, b, s, opts, x, stats);
 }
 
 int idrs_make_p_f64(int n, int s, int method, const double *PT_raw, const double *r, uint64_t seed, double *PT) {

@@ -22,6 +22,7 @@
 // and the omega step is: SpMV chain of A*r with the (r,r),(t,t),(t,r)
 // epilogue, fin_om, omega_update (r, x, ||r||^2 and the next P^T r), fin_cycle.
 #include "idrs.hpp"
+#include "idrs_state.hpp"
 
 #include <cfloat>
 #include <map>
@@ -79,30 +80,7 @@ struct Buf {
 };
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// device state
-// ---------------------------------------------------------------------------
-template <class T>
-struct State {
-    T M[SMAX * SMAX];          // M(i,j) -> M[i + SMAX*j]
-    T f[SMAX], mu[SMAX], alpha[SMAX], beta[SMAX], gamma[SMAX], gs[SMAX];
-    T om;
-    double normb, tolb, normr, kappa, rho, tol;
-    int info, iter, jj, maxit, s, method, prow, conjP, user_omega, n_omega, inispace;
-    int rv_cap;                // resvec entries available on the device
-};
-
 #define GRID_LOOP(q, n) for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < (n); q += (long)gridDim.x * blockDim.x)
-
-__device__ inline double sdiv(double a, double b) { return a / b; }
-__device__ inline c128 sdiv(c128 a, c128 b) { return a / b; }
-__device__ inline double sabs(double a) { return fabs(a); }
-__device__ inline double sabs(c128 a) { return hypot(a.re, a.im); }
-__device__ inline double sre(double a) { return a; }
-__device__ inline double sre(c128 a) { return a.re; }
-
-template <class T>
-__device__ inline T fmaT(T a, T b, T c) { return Tr<T>::fma(a, b, c); }
 
 // ---------------------------------------------------------------------------
 // setup kernels
@@ -110,24 +88,8 @@ __device__ inline T fmaT(T a, T b, T c) { return Tr<T>::fma(a, b, c); }
 template <class T>
 __global__ void k_init(State<T> *st, int s, int method, int prow, int conjP, int maxit, double kappa, double tol,
                        int user_omega, int n_omega, int inispace, int rv_cap) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        st->rv_cap = rv_cap;
-        st->s = s;
-        st->method = method;
-        st->prow = prow;
-        st->conjP = conjP;
-        st->maxit = maxit;
-        st->kappa = kappa;
-        st->tol = tol;
-        st->user_omega = user_omega;
-        st->n_omega = n_omega;
-        st->inispace = inispace;
-        st->om = T(1.0);
-        st->info = -1;
-        st->iter = 0;
-        st->jj = 0;
-        for (int i = 0; i < SMAX * SMAX; i++) st->M[i] = T(0);
-    }
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        scalar_init(st, s, method, prow, conjP, maxit, kappa, tol, user_omega, n_omega, inispace, rv_cap);
 }
 
 // partials[block] = sum |v|^2 over the block's rows
@@ -306,49 +268,8 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_f(State<T> *st, const T *pa
 }
 
 // ---------------------------------------------------------------------------
-// s x s scalar logic (thread 0 of a one-block kernel); contraction off so
-// each operation rounds like the reference's statements.
+// one-block reduction + scalar logic kernels
 // ---------------------------------------------------------------------------
-#define MS(i, j) st->M[(i) + SMAX * (j)]
-
-// gamma(i), i = k..s-1: forward substitution on the lower-triangular M
-// (idrs.inc:159-165)
-template <class T>
-__device__ void scalar_gamma(State<T> *st, int k) {
-#pragma clang fp contract(off)
-    const int s = st->s;
-    for (int i = k; i < s; i++) {
-        T g = st->f[i];
-        for (int j = k; j < i; j++) g = g - MS(i, j) * st->gamma[j];
-        st->gamma[i] = sdiv(g, MS(i, i));
-    }
-}
-
-// after G(:,k) = A*U(:,k) and mu = P^T G(:,k)  (idrs.inc:198-229)
-template <class T>
-__device__ void scalar_mu(State<T> *st, int k, const T *red) {
-#pragma clang fp contract(off)
-    const int s = st->s, prow = st->prow;
-    T mu[SMAX];
-    for (int i = 0; i < s; i++) mu[i] = i < prow ? red[i] : T(0);
-    for (int i = 0; i < k; i++) {
-        T a = mu[i];
-        for (int j = 0; j < i; j++) a = a - MS(i, j) * st->alpha[j];
-        a = sdiv(a, MS(i, i));
-        st->alpha[i] = a;
-        for (int q = k; q < s; q++) mu[q] = mu[q] - MS(q, i) * a;
-    }
-    for (int q = k; q < s; q++) MS(q, k) = mu[q];
-    if (sabs(MS(k, k)) <= DBL_MIN) {                                       // idrs.inc:216
-        st->info = 3;
-        return;
-    }
-    T beta = sdiv(st->f[k], MS(k, k));                                     // idrs.inc:222
-    st->beta[k] = beta;
-    for (int q = k + 1; q < s; q++) st->f[q] = st->f[q] - beta * MS(q, k);  // idrs.inc:228
-    if (k + 1 < s && st->jj > 0) scalar_gamma(st, k + 1);
-}
-
 template <class T>
 __global__ __launch_bounds__(FIN_THREADS) void k_fin_mu(int k, State<T> *st, const T *partials, int nparts) {
     if (st->info >= 0) return;
@@ -357,49 +278,12 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_mu(int k, State<T> *st, con
     if (threadIdx.x == 0) scalar_mu(st, k, red);
 }
 
-// normr, iteration count, convergence (idrs.inc:240-249)
 template <class T>
 __global__ __launch_bounds__(FIN_THREADS) void k_fin_norm(State<T> *st, const T *partials, int nparts, double *resvec) {
     if (st->info >= 0) return;
     __shared__ T red[1];
     reduce_partials(partials, nparts, 1, red);
-    if (threadIdx.x != 0) return;
-    double nr = sqrt(sre(red[0]));
-    st->normr = nr;
-    int it = st->iter + 1;
-    st->iter = it;
-    if (resvec && it < st->rv_cap) resvec[it] = nr;
-    if (nr < st->tolb) st->info = 0;
-    else if (it == st->maxit) st->info = 1;
-}
-
-// omega (idrs.inc:260-284; COMP_OM idrs_module.f90:719-785)
-template <class T>
-__device__ void scalar_om(State<T> *st, const T *red, const T *omega) {
-#pragma clang fp contract(off)
-    const int jj = st->jj + 1;
-    st->jj = jj;
-    T om;
-    if (st->user_omega) {
-        int i = jj % st->n_omega;
-        if (i == 0) i = st->n_omega;
-        om = omega[i - 1];
-    } else {
-        const T rr = red[0], tt = red[1], tr = red[2];
-        const double kappa = st->kappa;
-        if (kappa == 0.0) {
-            om = sdiv(tr, tt);
-        } else {
-            double rho = sabs(tr) / sqrt(sabs(rr * tt));
-            om = sdiv(tr, tt);
-            if (rho < kappa) {
-                om = om * T(kappa);
-                om = sdiv(om, T(rho));
-            }
-        }
-    }
-    st->om = om;
-    if (sabs(om) <= DBL_EPSILON) st->info = 3;                             // idrs.inc:281
+    if (threadIdx.x == 0) scalar_norm(st, sqrt(sre(red[0])), resvec);
 }
 
 template <class T>
@@ -410,31 +294,13 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_om(State<T> *st, const T *p
     if (threadIdx.x == 0) scalar_om(st, red, omega);
 }
 
-// end of the omega step (idrs.inc:292-299) and the next cycle's f = P^T r
-// and gamma (idrs.inc:147, 159-165)
 template <class T>
 __global__ __launch_bounds__(FIN_THREADS) void k_fin_cycle(State<T> *st, const T *partials, int nparts, double *resvec) {
     if (st->info >= 0) return;
     __shared__ T red[SMAX + 1];
-    const int prow = st->prow;
-    reduce_partials(partials, nparts, 1 + prow, red);
-    if (threadIdx.x != 0) return;
-    double nr = sqrt(sre(red[0]));
-    st->normr = nr;
-    int it = st->iter + 1;
-    st->iter = it;
-    if (resvec && it < st->rv_cap) resvec[it] = nr;
-    if (nr < st->tolb) {
-        st->info = 0;
-        return;
-    } else if (it == st->maxit) {
-        st->info = 1;
-        return;
-    }
-    for (int i = 0; i < st->s; i++) st->f[i] = i < prow ? red[1 + i] : T(0);
-    scalar_gamma(st, 0);
+    reduce_partials(partials, nparts, 1 + st->prow, red);
+    if (threadIdx.x == 0) scalar_cycle(st, red, resvec);
 }
-#undef MS
 
 // ---------------------------------------------------------------------------
 // streaming kernels of the iteration

@@ -18,6 +18,7 @@
 // share).  The operator's follow-up work (Horner term, shadow-space dots,
 // omega dots, norms) is fused into the product as an epilogue.
 #include "matrix.hpp"
+#include "spmv_epi.hpp"
 
 #include <algorithm>
 #include <cstdarg>
@@ -43,6 +44,13 @@ DevMatrix::~DevMatrix() {
     (void)hipFree(val);
     (void)hipFree(val32);
     (void)hipFree(diag);
+}
+
+Partition::~Partition() {
+    (void)hipFree(row0);
+    (void)hipFree(tile0);
+    (void)hipFree(tile_sys);
+    (void)hipFree(tile_r0);
 }
 
 Operator::~Operator() {
@@ -279,144 +287,14 @@ DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const v
     return make_dia_impl(nrows, ncols, ndiag, offsets, vals, kind, 5, nullptr);
 }
 
-// ---------------------------------------------------------------------------
-// elementwise helpers for the value/vector type combinations
-// ---------------------------------------------------------------------------
-__device__ inline double divv(double x, double d) { return x / d; }
-__device__ inline float divv(float x, float d) { return x / d; }
-template <class R>
-__device__ inline cplx<R> divv(cplx<R> x, R d) { return {x.re / d, x.im / d}; }
-template <class R>
-__device__ inline cplx<R> divv(cplx<R> x, cplx<R> d) { return x / d; }
-
-// ---------------------------------------------------------------------------
-// SpMV epilogues
-//
-// The operands of a row's finish and epilogue (z, the P row, r) do not depend
-// on the row sum, so the kernels load them next to the matrix entries and
-// without a branch around the load: an absent z is replaced by a valid
-// address whose value is ignored, and P components beyond prow re-read the
-// last one and are multiplied by zero.  One memory latency then covers all
-// loads of a row instead of one latency per operand.
-// ---------------------------------------------------------------------------
-template <class VT, class T>
-struct SpmvArgs {
-    int nrows, ncols;
-    const int *rp, *col;
-    const VT *val;
-    int ndiag;
-    const int *offsets;
-    const T *x;
-    T xs;
-    const T *z;
-    T a;
-    int has_seed;      // seed term: y -= seed * x/sdiag (sdiag nullptr: x)
-    const VT *sdiag;
-    T seed;
-    T *y;
-};
-
-// y(row) = xs*(acc - seed*x(row)/sdiag(row)) + a*z(row); zq holds z(row)
-// (ignored without z)
-template <class VT, class T>
-__device__ inline T finish_value(const SpmvArgs<VT, T> &A, long row, T acc, T zq) {
-    if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
-    T yv = A.xs * acc;
-    if (A.z) yv = Tr<T>::fma(A.a, zq, yv);
-    return yv;
-}
-
-// Operands of one row's epilogue: PC components of its P row (PDOT: this
-// lane's components c, c+step, ...), or r(row) (OMEGA).
-template <class T, int EPI, int PC>
-__device__ inline void epi_load(T (&pe)[PC], const EpiArgs<T> &E, long row, int c, int step) {
-    if constexpr (EPI == EPI_PDOT) {
-        const T *p = E.P + row * E.prow;
-        const int last = E.prow - 1;
-#pragma unroll
-        for (int k = 0; k < PC; k++) pe[k] = p[min(c + k * step, last)];
-    } else if constexpr (EPI == EPI_OMEGA) {
-        pe[0] = E.r[row];
-    }
-}
-
-// Accumulates a row into the epilogue sums: PDOT adds P components
-// c, c+step, ... (zero beyond prow) times y into e[0..PC); OMEGA adds
-// (r,r), (y,y), (y,r); NORM adds |y|^2.
-template <class T, int EPI, int NV, int PC>
-__device__ inline void epi_accum(T (&e)[NV], const T (&pe)[PC], const EpiArgs<T> &E, T yv, int c, int step) {
-    using TT = Tr<T>;
-    if constexpr (EPI == EPI_PDOT) {
-#pragma unroll
-        for (int k = 0; k < PC; k++) {
-            T pv = E.conjP ? TT::conj(pe[k]) : pe[k];
-            pv = (c + k * step < E.prow) ? pv : T(0);
-            e[k] = TT::fma(pv, yv, e[k]);
-        }
-    } else if constexpr (EPI == EPI_OMEGA) {
-        const T rr = pe[0];
-        e[0] = TT::fma(TT::conj(rr), rr, e[0]);
-        e[1] = TT::fma(TT::conj(yv), yv, e[1]);
-        e[2] = TT::fma(TT::conj(yv), rr, e[2]);
-    } else if constexpr (EPI == EPI_NORM) {
-        e[0] = e[0] + T(TT::abs2(yv));
-    }
-}
-
-// Block partials of the epilogue sums where every thread holds all NV sums.
-template <class T, int EPI, int NV>
-__device__ inline void epi_store(T (&e)[NV], const EpiArgs<T> &E, T *lds) {
-    if constexpr (EPI != EPI_NONE) {
-        block_sum<T, NV>(e, lds);
-        if (threadIdx.x == 0)
-#pragma unroll
-            for (int i = 0; i < NV; i++) E.partials[(long)i * E.nparts + blockIdx.x] = e[i];
-    }
-}
-
-// Block partials of P^T y when lane g of every G-lane group holds the
-// components g, g+G, ... in e[0..PC): a butterfly over the lanes of equal g
-// inside the wavefront, then one LDS slot per (component, wavefront).
-template <class T, int NV, int G, int PC>
-__device__ inline void epi_store_group(T (&e)[PC], const EpiArgs<T> &E, T *lds) {
-    constexpr int NW = SPMV_THREADS / WAVE;
-#pragma unroll
-    for (int k = 0; k < PC; k++)
-#pragma unroll
-        for (int m = G; m < WAVE; m <<= 1) e[k] = e[k] + shfl_xor(e[k], m);
-    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
-    if (lane < G)
-#pragma unroll
-        for (int k = 0; k < PC; k++) {
-            const int i = lane + k * G;
-            if (i < NV) lds[i * NW + w] = e[k];
-        }
-    __syncthreads();
-    if (threadIdx.x < NV) {
-        T sum = lds[threadIdx.x * NW];
-        for (int v = 1; v < NW; v++) sum = sum + lds[threadIdx.x * NW + v];
-        E.partials[(long)threadIdx.x * E.nparts + blockIdx.x] = sum;
-    }
-}
-
-// Non-temporal loads for the matrix streams (read once per product): they
-// should not push the gathered x entries out of L2.
-template <class V>
-__device__ inline V ldnt(const V *p) { return __builtin_nontemporal_load(p); }
-template <class R>
-__device__ inline cplx<R> ldnt(const cplx<R> *p) {
-    const R *q = reinterpret_cast<const R *>(p);
-    return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
-}
-
 // CRS: a group of G lanes per row (G >= the typical row length), so a
 // wavefront covers 64/G consecutive rows whose entries are one contiguous,
 // coalesced stretch of values and column indices.  The block's row pointers
 // are staged in LDS once per chunk of rows, which leaves two dependent
 // memory latencies per row (column index, then x).  Each group keeps U rows
-// in flight: all their first-pass loads, and the z / P / r operands of the
-// finish and epilogue, are issued before any is used; rows longer than G
-// finish in a strided remainder loop.  The group's partial sums are combined
+// in flight: all their first-pass loads, and the P components of the P^T y
+// epilogue, are issued before any is used; rows longer than G finish in a
+// strided remainder loop.  The group's partial sums are combined
 // by a butterfly inside the group, so every lane of the group holds y(row);
 // for P^T y lane g takes the P components g, g+G, ... of its rows.
 constexpr int CRS_CHUNK = 2048;   // rows whose row pointers sit in LDS at once
@@ -440,7 +318,6 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
     const T *__restrict__ x = A.x;
     const VT *__restrict__ val = A.val;
     const int *__restrict__ col = A.col;
-    const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
     for (long c0 = r0; c0 < r1; c0 += CRS_CHUNK) {
         const long c1 = min(c0 + (long)CRS_CHUNK, r1);
         __syncthreads();
@@ -466,11 +343,13 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
                 vv[u] = ok ? ldnt(&val[j]) : VT(0);
                 cc[u] = ok ? ldnt(&col[j]) : -1;
             }
-            T zq[U], pe[U][PC];
+            // P components of the rows (P^T y) are issued with the matrix
+            // loads; z and r are loaded at the finish, where the register
+            // budget of the U rows in flight is lower
+            T pe[U][PC];
+            if constexpr (EPI == EPI_PDOT) {
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                zq[u] = zp[rq[u]];
-                epi_load<T, EPI, PC>(pe[u], E, rq[u], g, G);
+                for (int u = 0; u < U; u++) epi_load<T, EPI, PC>(pe[u], E, rq[u], g, G);
             }
             T acc[U];
 #pragma unroll
@@ -485,11 +364,13 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_crs(SpmvArgs<VT, T> A, Ep
 #pragma unroll
             for (int u = 0; u < U; u++)
                 if (row[u] < c1) {
-                    const T yv = finish_value(A, row[u], acc[u], zq[u]);
+                    const T zq = A.z ? A.z[row[u]] : T(0);
+                    const T yv = finish_value(A, row[u], acc[u], zq);
                     if (g == 0) A.y[row[u]] = yv;
                     if constexpr (EPI == EPI_PDOT) {
                         epi_accum<T, EPI, PC, PC>(e, pe[u], E, yv, g, G);
                     } else if (g == 0) {
+                        epi_load<T, EPI, PC>(pe[u], E, row[u], 0, 1);
                         epi_accum<T, EPI, NV, PC>(e, pe[u], E, yv, 0, 1);
                     }
                 }
@@ -526,7 +407,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
         T acc = T(0);
         const T zq = zp[row];
         T pe[PC];
-        epi_load<T, EPI, PC>(pe, E, row, 0, 1);
+        if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
         if constexpr (ND > 0) {
             VT vv[ND];
             T xx[ND];
@@ -547,6 +428,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
         }
         const T yv = finish_value(A, row, acc, zq);
         A.y[row] = yv;
+        if constexpr (EPI != EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
         epi_accum<T, EPI, NV, PC>(e, pe, E, yv, 0, 1);
     }
     epi_store<T, EPI, NV>(e, E, s_red);
@@ -597,6 +479,130 @@ static void dispatch_epi(const DevMatrix &K, const SpmvArgs<VT, T> &A, const Epi
     }
 }
 
+// ---------------------------------------------------------------------------
+// batched (tile) SpMV: one workgroup per tile of the block-diagonal matrix's
+// partition.  Same per-row arithmetic and group layout as k_spmv_crs; the
+// epilogue partials are per tile, and a tile whose system has finished
+// returns at once.
+// ---------------------------------------------------------------------------
+struct TileArgs {
+    const int *tile_sys;
+    const long *tile_r0;
+    const int *info;
+    int stride;
+};
+
+template <class VT, class T, int EPI, int NV, int G>
+__global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_crs(SpmvArgs<VT, T> A, EpiArgs<T> E, TileArgs ta) {
+    const int tile = blockIdx.x;
+    if (ta.info && ta.info[(long)ta.tile_sys[tile] * ta.stride] >= 0) return;
+    constexpr int GPB = SPMV_THREADS / G;
+    constexpr int U = 4;
+    constexpr int PC = EPI == EPI_PDOT ? (NV + G - 1) / G : 1;
+    __shared__ int s_rp[BATCH_TILE + 1];
+    __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
+    T e[EPI == EPI_PDOT ? PC : NV];
+#pragma unroll
+    for (int i = 0; i < (EPI == EPI_PDOT ? PC : NV); i++) e[i] = T(0);
+    const int g = threadIdx.x % G, grp = threadIdx.x / G;
+    const long c0 = ta.tile_r0[tile], c1 = ta.tile_r0[tile + 1];
+    const T *__restrict__ x = A.x;
+    const VT *__restrict__ val = A.val;
+    const int *__restrict__ col = A.col;
+    for (long q = c0 + threadIdx.x; q <= c1; q += SPMV_THREADS) s_rp[q - c0] = A.rp[q];
+    __syncthreads();
+    for (long base = c0; base < c1; base += (long)GPB * U) {
+        long row[U], rq[U];
+        int a[U], b[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            row[u] = base + grp + (long)u * GPB;
+            const bool in = row[u] < c1;
+            rq[u] = in ? row[u] : c0;
+            a[u] = in ? s_rp[row[u] - c0] : 0;
+            b[u] = in ? s_rp[row[u] - c0 + 1] : 0;
+        }
+        VT vv[U];
+        int cc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int j = a[u] + g;
+            const bool ok = j < b[u];
+            vv[u] = ok ? ldnt(&val[j]) : VT(0);
+            cc[u] = ok ? ldnt(&col[j]) : -1;
+        }
+        T pe[U][PC];
+        if constexpr (EPI == EPI_PDOT) {
+#pragma unroll
+            for (int u = 0; u < U; u++) epi_load<T, EPI, PC>(pe[u], E, rq[u], g, G);
+        }
+        T acc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) acc[u] = cc[u] >= 0 ? mulv(vv[u], x[cc[u]]) : T(0);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            for (int j = a[u] + g + G; j < b[u]; j += G) acc[u] = fmav(ldnt(&val[j]), x[ldnt(&col[j])], acc[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int m = G / 2; m >= 1; m >>= 1) acc[u] = acc[u] + shfl_xor(acc[u], m);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (row[u] < c1) {
+                const T zq = A.z ? A.z[row[u]] : T(0);
+                const T yv = finish_value(A, row[u], acc[u], zq);
+                if (g == 0) A.y[row[u]] = yv;
+                if constexpr (EPI == EPI_PDOT) {
+                    epi_accum<T, EPI, PC, PC>(e, pe[u], E, yv, g, G);
+                } else if (g == 0) {
+                    epi_load<T, EPI, PC>(pe[u], E, row[u], 0, 1);
+                    epi_accum<T, EPI, NV, PC>(e, pe[u], E, yv, 0, 1);
+                }
+            }
+    }
+    if constexpr (EPI == EPI_PDOT) epi_store_group<T, NV, G, PC>(e, E, s_red);
+    else epi_store<T, EPI, NV>(e, E, s_red);
+}
+
+template <class VT, class T, int EPI, int NV>
+static void launch_bspmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T> E, const TileCtx &tc,
+                         hipStream_t st) {
+    if (K.fmt != DevFmt::CRS) throw std::invalid_argument("batched systems must be stored as CRS");
+    const Partition &P = *tc.part;
+    TileArgs ta{P.tile_sys, P.tile_r0, tc.info, tc.stride};
+    E.nparts = P.ntiles;
+    dim3 gd(P.ntiles), bd(SPMV_THREADS);
+    switch (K.group) {
+    case 2: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 2>), gd, bd, 0, st, A, E, ta); break;
+    case 4: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 4>), gd, bd, 0, st, A, E, ta); break;
+    case 8: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 8>), gd, bd, 0, st, A, E, ta); break;
+    case 16: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 16>), gd, bd, 0, st, A, E, ta); break;
+    case 32: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 32>), gd, bd, 0, st, A, E, ta); break;
+    default: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 64>), gd, bd, 0, st, A, E, ta); break;
+    }
+    LAUNCH_CHECK();
+}
+
+template <class VT, class T>
+static void dispatch_epi_tiles(const DevMatrix &K, const SpmvArgs<VT, T> &A, const EpiArgs<T> &E, int epi,
+                               const TileCtx &tc, hipStream_t st) {
+    if (!tc.part || K.part.get() != tc.part) throw std::invalid_argument("tile launch without the matrix's partition");
+    switch (epi) {
+    case EPI_NONE: launch_bspmv<VT, T, EPI_NONE, 1>(K, A, E, tc, st); break;
+    case EPI_NORM: launch_bspmv<VT, T, EPI_NORM, 1>(K, A, E, tc, st); break;
+    case EPI_OMEGA: launch_bspmv<VT, T, EPI_OMEGA, 3>(K, A, E, tc, st); break;
+    case EPI_PDOT:
+        if (E.prow < 1 || E.prow > 16 || !E.P) throw std::invalid_argument("P^T y epilogue needs 1..16 rows of P");
+        if (E.prow <= 1) launch_bspmv<VT, T, EPI_PDOT, 1>(K, A, E, tc, st);
+        else if (E.prow <= 2) launch_bspmv<VT, T, EPI_PDOT, 2>(K, A, E, tc, st);
+        else if (E.prow <= 4) launch_bspmv<VT, T, EPI_PDOT, 4>(K, A, E, tc, st);
+        else if (E.prow <= 8) launch_bspmv<VT, T, EPI_PDOT, 8>(K, A, E, tc, st);
+        else launch_bspmv<VT, T, EPI_PDOT, 16>(K, A, E, tc, st);
+        break;
+    default: throw std::invalid_argument("bad epilogue");
+    }
+}
+
 template <class T> static T to_T(std::complex<double> c);
 template <> double to_T<double>(std::complex<double> c) { return c.real(); }
 template <> float to_T<float>(std::complex<double> c) { return (float)c.real(); }
@@ -618,7 +624,7 @@ static SpmvArgs<VT, T> base_args(const DevMatrix &K, const VT *val) {
 
 template <class T>
 void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int epi, const EpiArgs<T> &E,
-             const int *info, hipStream_t st) {
+             const int *info, hipStream_t st, const TileCtx *tc) {
     const DevMatrix &K = *op.K;
     const bool cv = op.values_complex();
     const bool has_seed = op.seed != std::complex<double>(0.0, 0.0);
@@ -634,7 +640,8 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
             A.has_seed = has_seed;
             A.sdiag = op.which_precon == 1 ? (const c128 *)op.pdiag : nullptr;
             A.seed = to_T<T>(op.seed);
-            dispatch_epi<c128, T>(K, A, E, epi, info, st);
+            if (tc) dispatch_epi_tiles<c128, T>(K, A, E, epi, *tc, st);
+            else dispatch_epi<c128, T>(K, A, E, epi, info, st);
         } else {
             throw std::invalid_argument("complex matrix values need complex128 vectors");
         }
@@ -646,7 +653,8 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
         A.has_seed = has_seed;
         A.sdiag = op.which_precon == 1 ? (const R *)op.pdiag : nullptr;
         A.seed = to_T<T>(op.seed);
-        dispatch_epi<R, T>(K, A, E, epi, info, st);
+        if (tc) dispatch_epi_tiles<R, T>(K, A, E, epi, *tc, st);
+        else dispatch_epi<R, T>(K, A, E, epi, info, st);
     }
 }
 
@@ -658,29 +666,111 @@ static T coef(const Operator &op, int i) {
 
 template <class T>
 void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const EpiArgs<T> &E,
-              const int *info, hipStream_t st) {
+              const int *info, hipStream_t st, const TileCtx *tc) {
     const int d = op.degree;
     EpiArgs<T> none{};
     if (d == 0) {
         timer_mark(st);
-        op_spmv<T>(op, v, T(1), nullptr, T(0), w, epi, E, info, st);
+        op_spmv<T>(op, v, T(1), nullptr, T(0), w, epi, E, info, st, tc);
         timer_mark(st);
         return;
     }
     // Horner: w = alpha_d v; for i = d-1..0: w = K'(w) + alpha_i v; then K'(w)
     T *cur = w1, *nxt = w2;
     timer_mark(st);
-    op_spmv<T>(op, v, coef<T>(op, d), v, coef<T>(op, d - 1), cur, EPI_NONE, none, info, st);
+    op_spmv<T>(op, v, coef<T>(op, d), v, coef<T>(op, d - 1), cur, EPI_NONE, none, info, st, tc);
     timer_mark(st);
     for (int i = d - 2; i >= 0; i--) {
         timer_mark(st);
-        op_spmv<T>(op, cur, T(1), v, coef<T>(op, i), nxt, EPI_NONE, none, info, st);
+        op_spmv<T>(op, cur, T(1), v, coef<T>(op, i), nxt, EPI_NONE, none, info, st, tc);
         timer_mark(st);
         std::swap(cur, nxt);
     }
     timer_mark(st);
-    op_spmv<T>(op, cur, T(1), nullptr, T(0), w, epi, E, info, st);
+    op_spmv<T>(op, cur, T(1), nullptr, T(0), w, epi, E, info, st, tc);
     timer_mark(st);
+}
+
+// ---------------------------------------------------------------------------
+// block-diagonal batches
+// ---------------------------------------------------------------------------
+// rows [0, n) of one system's CRS arrays into the batch's: row pointers and
+// column indices shifted by the system's first entry and first row
+__global__ void k_shift_rows(long n, const int *__restrict__ rp, int ent0, int *__restrict__ out) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = rp[i] + ent0;
+}
+
+DevMatrix *make_blockdiag(int nsys, const DevMatrix *const *Ks) {
+    if (nsys < 1) throw std::invalid_argument("a batch needs at least one system");
+    const ValKind kind = Ks[0]->kind;
+    long nrows = 0, nnz = 0;
+    for (int b = 0; b < nsys; b++) {
+        const DevMatrix *K = Ks[b];
+        if (!K) throw std::invalid_argument("null system matrix");
+        if (K->fmt != DevFmt::CRS) throw std::invalid_argument("batched systems must be CRS or COO matrices");
+        if (K->nrows != K->ncols || K->nrows < 1) throw std::invalid_argument("batched systems must be square");
+        if (K->kind != kind) throw std::invalid_argument("batched systems must all be real or all complex");
+        nrows += K->nrows;
+        nnz += K->nnz;
+    }
+    if (nrows > INT32_MAX || nnz > INT32_MAX) throw std::invalid_argument("batch too large for int32 indices");
+    std::unique_ptr<DevMatrix> M(new DevMatrix);
+    M->orig_format = 2;
+    M->fmt = DevFmt::CRS;
+    M->kind = kind;
+    M->nrows = M->ncols = (int)nrows;
+    M->nnz = nnz;
+    const size_t vsz = kind == ValKind::Complex ? sizeof(c128) : sizeof(double);
+    HIP_TRY(hipMalloc(&M->row_ptr, sizeof(int) * (nrows + 1)));
+    HIP_TRY(hipMalloc(&M->col, sizeof(int) * (nnz ? nnz : 1)));
+    HIP_TRY(hipMalloc(&M->val, vsz * (nnz ? nnz : 1)));
+    HIP_TRY(hipMalloc(&M->diag, vsz * nrows));
+    std::unique_ptr<Partition> P(new Partition);
+    P->nsys = nsys;
+    P->h_row0.assign(nsys + 1, 0);
+    P->h_tile0.assign(nsys + 1, 0);
+    std::vector<int> tile_sys;
+    std::vector<long> tile_r0;
+    long r0 = 0, e0 = 0;
+    for (int b = 0; b < nsys; b++) {
+        const DevMatrix *K = Ks[b];
+        const long n = K->nrows, nz = K->nnz;
+        const int g = vec_grid(n);
+        // row pointers shifted by e0 (the final entry comes with the last system)
+        hipLaunchKernelGGL(k_shift_rows, dim3(g), dim3(256), 0, 0, n + (b == nsys - 1 ? 1 : 0), K->row_ptr, (int)e0,
+                           M->row_ptr + r0);
+        LAUNCH_CHECK();
+        if (nz > 0) {
+            hipLaunchKernelGGL(k_shift_rows, dim3(vec_grid(nz)), dim3(256), 0, 0, nz, K->col, (int)r0, M->col + e0);
+            LAUNCH_CHECK();
+            HIP_TRY(hipMemcpy((char *)M->val + vsz * e0, K->val, vsz * nz, hipMemcpyDeviceToDevice));
+        }
+        HIP_TRY(hipMemcpy((char *)M->diag + vsz * r0, K->diag, vsz * n, hipMemcpyDeviceToDevice));
+        P->h_row0[b] = r0;
+        P->h_tile0[b] = (int)tile_sys.size();
+        for (long t = r0; t < r0 + n; t += BATCH_TILE) {
+            tile_sys.push_back(b);
+            tile_r0.push_back(t);
+        }
+        r0 += n;
+        e0 += nz;
+    }
+    P->h_row0[nsys] = r0;
+    P->h_tile0[nsys] = (int)tile_sys.size();
+    tile_r0.push_back(r0);
+    P->ntiles = (int)tile_sys.size();
+    HIP_TRY(hipDeviceSynchronize());
+    P->row0 = dev_upload(P->h_row0.data(), P->h_row0.size());
+    P->tile0 = dev_upload(P->h_tile0.data(), P->h_tile0.size());
+    P->tile_sys = dev_upload(tile_sys.data(), tile_sys.size());
+    P->tile_r0 = dev_upload(tile_r0.data(), tile_r0.size());
+    double avg = (double)nnz / (double)nrows;
+    int g = 2;
+    while (g < 64 && g < avg) g *= 2;
+    M->group = g;
+    M->part = std::move(P);
+    return M.release();
 }
 
 // ---------------------------------------------------------------------------
@@ -1030,9 +1120,9 @@ void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> 
 // explicit instantiations
 #define INST(T)                                                                                              \
     template void op_spmv<T>(const Operator &, const T *, T, const T *, T, T *, int, const EpiArgs<T> &,     \
-                             const int *, hipStream_t);                                                      \
+                             const int *, hipStream_t, const TileCtx *);                                     \
     template void op_apply<T>(const Operator &, const T *, T *, T *, T *, int, const EpiArgs<T> &,           \
-                              const int *, hipStream_t);                                                     \
+                              const int *, hipStream_t, const TileCtx *);                                    \
     template void op_precon<T>(const Operator &, const T *, T *, hipStream_t);                               \
     template void op_scaleback<T>(const Operator &, const T *, T *, T *, T *, T *, hipStream_t);             \
     template void user_spmv<T>(const DevMatrix &, const T *, T *, hipStream_t);

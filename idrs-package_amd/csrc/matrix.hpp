@@ -19,6 +19,31 @@ enum class DevFmt { CRS = 0, DIA = 1 };
 constexpr int SPMV_THREADS = 256;
 constexpr int MAX_DIAG = 1024;       // offsets held in LDS by the DIA kernel
 
+// Row partition of a block-diagonal matrix into independent systems, cut
+// into tiles (row ranges inside one system) for the batched kernels: one
+// workgroup per tile, so every block-level partial sum belongs to exactly
+// one system.
+constexpr int BATCH_TILE = 1024;   // rows per tile
+struct Partition {
+    int nsys = 0, ntiles = 0;
+    std::vector<long> h_row0;      // nsys+1: first row of each system
+    std::vector<int> h_tile0;      // nsys+1: first tile of each system
+    long *row0 = nullptr;          // device copies of the above
+    int *tile0 = nullptr;
+    int *tile_sys = nullptr;       // ntiles: system of each tile
+    long *tile_r0 = nullptr;       // ntiles+1: first row of each tile (tiles are contiguous)
+    ~Partition();
+};
+
+// Tile launch of the batched solver: per-tile partials, and per-system
+// early exit once a system has finished (info: the State.info of system 0,
+// stride: ints between consecutive systems' info; nullptr: never skip).
+struct TileCtx {
+    const Partition *part = nullptr;
+    const int *info = nullptr;
+    int stride = 0;
+};
+
 struct DevMatrix {
     int orig_format = 0;     // user_module matrix_format: 2 CRS, 3 COO, 4 CDS, 5 DIA
     DevFmt fmt = DevFmt::CRS;
@@ -39,6 +64,8 @@ struct DevMatrix {
     // main diagonal (real_diagonal / complex_diagonal, zeros -> 1), device, as c128
     // or double depending on kind
     void *diag = nullptr;
+    // set for block-diagonal batches (make_blockdiag)
+    std::unique_ptr<Partition> part;
     ~DevMatrix();
 };
 
@@ -93,15 +120,17 @@ SpmvTimer &spmv_timer();
 // y = xs * (K' x) [+ a z] with epilogue; K' = scaled values when the operator
 // has diagonal scaling.  T is the vector type.  'info' (device) makes the
 // kernel a no-op once a solver has finished (nullptr: always run).
+// With a TileCtx the product runs as one workgroup per tile of the matrix's
+// partition (epilogue partials per tile; finished systems skipped).
 template <class T>
 void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int epi, const EpiArgs<T> &E,
-             const int *info, hipStream_t st);
+             const int *info, hipStream_t st, const TileCtx *tc = nullptr);
 
 // w = A*v (rmatvec: Horner polynomial then system matvec), with the epilogue
 // applied to the last SpMV.  w1, w2: scratch vectors of n.
 template <class T>
 void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const EpiArgs<T> &E,
-              const int *info, hipStream_t st);
+              const int *info, hipStream_t st, const TileCtx *tc = nullptr);
 
 // w = v / op (rprecon): divide by the preconditioner diagonal, or copy.
 template <class T>
@@ -126,6 +155,9 @@ DevMatrix *make_coo(int nrows, int ncols, long nnz, const int *indx1, const int 
                     ValKind kind);
 DevMatrix *make_cds(int nrows, int ncols, int half_band, const void *band, ValKind kind);
 DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const void *vals, ValKind kind);
+// block-diagonal CRS matrix of nsys square CRS systems (copies), with its
+// Partition
+DevMatrix *make_blockdiag(int nsys, const DevMatrix *const *Ks);
 
 // operator construction
 void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shift);

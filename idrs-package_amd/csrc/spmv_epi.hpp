@@ -1,0 +1,140 @@
+// Device helpers shared by the SpMV kernels of the single-system and the
+// batched solvers: value/vector arithmetic, the SpMV argument block, the
+// row finish and the fused epilogues (P^T y, omega dots, norms).
+#pragma once
+
+#include "matrix.hpp"
+
+namespace idrs {
+
+// ---------------------------------------------------------------------------
+// elementwise helpers for the value/vector type combinations
+// ---------------------------------------------------------------------------
+__device__ inline double divv(double x, double d) { return x / d; }
+__device__ inline float divv(float x, float d) { return x / d; }
+template <class R>
+__device__ inline cplx<R> divv(cplx<R> x, R d) { return {x.re / d, x.im / d}; }
+template <class R>
+__device__ inline cplx<R> divv(cplx<R> x, cplx<R> d) { return x / d; }
+
+// ---------------------------------------------------------------------------
+// SpMV epilogues
+//
+// The operands of a row's finish and epilogue (z, the P row, r) do not depend
+// on the row sum, so the kernels load them next to the matrix entries and
+// without a branch around the load: an absent z is replaced by a valid
+// address whose value is ignored, and P components beyond prow re-read the
+// last one and are multiplied by zero.  One memory latency then covers all
+// loads of a row instead of one latency per operand.
+// ---------------------------------------------------------------------------
+template <class VT, class T>
+struct SpmvArgs {
+    int nrows, ncols;
+    const int *rp, *col;
+    const VT *val;
+    int ndiag;
+    const int *offsets;
+    const T *x;
+    T xs;
+    const T *z;
+    T a;
+    int has_seed;      // seed term: y -= seed * x/sdiag (sdiag nullptr: x)
+    const VT *sdiag;
+    T seed;
+    T *y;
+};
+
+// y(row) = xs*(acc - seed*x(row)/sdiag(row)) + a*z(row); zq holds z(row)
+// (ignored without z)
+template <class VT, class T>
+__device__ inline T finish_value(const SpmvArgs<VT, T> &A, long row, T acc, T zq) {
+    if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
+    T yv = A.xs * acc;
+    if (A.z) yv = Tr<T>::fma(A.a, zq, yv);
+    return yv;
+}
+
+// Operands of one row's epilogue: PC components of its P row (PDOT: this
+// lane's components c, c+step, ...), or r(row) (OMEGA).
+template <class T, int EPI, int PC>
+__device__ inline void epi_load(T (&pe)[PC], const EpiArgs<T> &E, long row, int c, int step) {
+    if constexpr (EPI == EPI_PDOT) {
+        const T *p = E.P + row * E.prow;
+        const int last = E.prow - 1;
+#pragma unroll
+        for (int k = 0; k < PC; k++) pe[k] = p[min(c + k * step, last)];
+    } else if constexpr (EPI == EPI_OMEGA) {
+        pe[0] = E.r[row];
+    }
+}
+
+// Accumulates a row into the epilogue sums: PDOT adds P components
+// c, c+step, ... (zero beyond prow) times y into e[0..PC); OMEGA adds
+// (r,r), (y,y), (y,r); NORM adds |y|^2.
+template <class T, int EPI, int NV, int PC>
+__device__ inline void epi_accum(T (&e)[NV], const T (&pe)[PC], const EpiArgs<T> &E, T yv, int c, int step) {
+    using TT = Tr<T>;
+    if constexpr (EPI == EPI_PDOT) {
+#pragma unroll
+        for (int k = 0; k < PC; k++) {
+            T pv = E.conjP ? TT::conj(pe[k]) : pe[k];
+            pv = (c + k * step < E.prow) ? pv : T(0);
+            e[k] = TT::fma(pv, yv, e[k]);
+        }
+    } else if constexpr (EPI == EPI_OMEGA) {
+        const T rr = pe[0];
+        e[0] = TT::fma(TT::conj(rr), rr, e[0]);
+        e[1] = TT::fma(TT::conj(yv), yv, e[1]);
+        e[2] = TT::fma(TT::conj(yv), rr, e[2]);
+    } else if constexpr (EPI == EPI_NORM) {
+        e[0] = e[0] + T(TT::abs2(yv));
+    }
+}
+
+// Block partials of the epilogue sums where every thread holds all NV sums.
+template <class T, int EPI, int NV>
+__device__ inline void epi_store(T (&e)[NV], const EpiArgs<T> &E, T *lds) {
+    if constexpr (EPI != EPI_NONE) {
+        block_sum<T, NV>(e, lds);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int i = 0; i < NV; i++) E.partials[(long)i * E.nparts + blockIdx.x] = e[i];
+    }
+}
+
+// Block partials of P^T y when lane g of every G-lane group holds the
+// components g, g+G, ... in e[0..PC): a butterfly over the lanes of equal g
+// inside the wavefront, then one LDS slot per (component, wavefront).
+template <class T, int NV, int G, int PC>
+__device__ inline void epi_store_group(T (&e)[PC], const EpiArgs<T> &E, T *lds) {
+    constexpr int NW = SPMV_THREADS / WAVE;
+#pragma unroll
+    for (int k = 0; k < PC; k++)
+#pragma unroll
+        for (int m = G; m < WAVE; m <<= 1) e[k] = e[k] + shfl_xor(e[k], m);
+    const int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    if (lane < G)
+#pragma unroll
+        for (int k = 0; k < PC; k++) {
+            const int i = lane + k * G;
+            if (i < NV) lds[i * NW + w] = e[k];
+        }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        T sum = lds[threadIdx.x * NW];
+        for (int v = 1; v < NW; v++) sum = sum + lds[threadIdx.x * NW + v];
+        E.partials[(long)threadIdx.x * E.nparts + blockIdx.x] = sum;
+    }
+}
+
+// Non-temporal loads for the matrix streams (read once per product): they
+// should not push the gathered x entries out of L2.
+template <class V>
+__device__ inline V ldnt(const V *p) { return __builtin_nontemporal_load(p); }
+template <class R>
+__device__ inline cplx<R> ldnt(const cplx<R> *p) {
+    const R *q = reinterpret_cast<const R *>(p);
+    return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+
+}  // namespace idrs

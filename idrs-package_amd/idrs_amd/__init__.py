@@ -6,10 +6,13 @@ the IDRS solver); see api.py.
 """
 from ._lib import IdrsError, load as load_library  # noqa: F401
 from .api import (  # noqa: F401
+    BatchResult,
     IdrsResult,
     KrylovResult,
     Matrix,
     UserMatrix,
+    batch_format,
+    batch_info,
     cds_format,
     complex_diagonal,
     complex_preconditioner,
@@ -17,6 +20,7 @@ from .api import (  # noqa: F401
     crs_format,
     dia_format,
     idrs,
+    idrs_batched,
     make_p,
     msidrs,
     msqmridr,

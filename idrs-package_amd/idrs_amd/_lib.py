@@ -139,6 +139,10 @@ _PROTOS = {
                                   c_void_p, c_void_p, POINTER(idrs_stats_t)]),
     "idrs_msqmridr_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
                                    c_void_p, c_void_p, POINTER(idrs_stats_t)]),
+    "idrs_batch_format": (c_int, [c_int, c_void_p, POINTER(c_void_p)]),
+    "idrs_batch_info": (c_int, [c_void_p, POINTER(c_int), c_void_p]),
+    "idrs_idrs_batched_f64": (c_int, [c_void_p, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, c_void_p]),
+    "idrs_idrs_batched_c128": (c_int, [c_void_p, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, c_void_p]),
     "idrs_make_p_f64": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_uint64, c_void_p]),
     "idrs_make_p_c128": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_uint64, c_void_p]),
 }

@@ -25,6 +25,8 @@ IdrsResult = namedtuple("IdrsResult", "x flag relres iterations resvec solve_ms 
 # relres: per-shift relative residuals (array) for the multishift solvers;
 # colfac: MSIDRS collinearity factors when asked for, else None
 KrylovResult = namedtuple("KrylovResult", "x flag relres iterations resvec solve_ms colfac")
+# per-system arrays flag, relres, iterations; solve_ms of the whole batch
+BatchResult = namedtuple("BatchResult", "x flag relres iterations solve_ms")
 
 
 def _ptr(a):
@@ -488,3 +490,95 @@ def msqmridr(A, b, sigma, s, M1=None, tolerance=None, maximum_iterations=None, i
                    maximum_iterations=maximum_iterations, inner_s=inner_s, inner_tolerance=inner_tolerance,
                    inner_iterations=inner_iterations, omega=omega, resvec=resvec, PT_raw=PT_raw, p_seed=p_seed,
                    out=out)
+
+
+def batch_format(mats):
+    """Block-diagonal batch of square CRS/COO user matrices (copies).  Vectors
+    of the batch are the systems' vectors concatenated in order."""
+    mats = list(mats)
+    if not mats:
+        raise ValueError("a batch needs at least one matrix")
+    cplx = any(m.is_complex for m in mats)
+    arr = (ctypes.c_void_p * len(mats))(*[m.handle.value for m in mats])
+    h = ctypes.c_void_p()
+    check(_lib.load().idrs_batch_format(len(mats), arr, ctypes.byref(h)))
+    um = UserMatrix(h.value, cplx)
+    um._keep = mats
+    return um
+
+
+def batch_info(K):
+    """(nsys, row0) of a batch user matrix; row0[k] = first row of system k."""
+    lib = _lib.load()
+    ns = ctypes.c_int()
+    check(lib.idrs_batch_info(K.handle, ctypes.byref(ns), None))
+    row0 = np.zeros(ns.value + 1, dtype=np.int64)
+    if ns.value:
+        check(lib.idrs_batch_info(K.handle, ctypes.byref(ns), row0.ctypes.data))
+    return ns.value, row0
+
+
+def idrs_batched(A, b, s, tolerance=None, maximum_iterations=None, variant=None, x0=None, omega=None,
+                 PT_raw=None, p_seed=None, out=None):
+    """x_k = IDRS(A_k, b_k, s, ...) for every system of a batch (A built on a
+    batch_format user matrix).  b (and x0, PT_raw, out) are the systems'
+    vectors concatenated: numpy arrays, or CUDA tensors for a
+    device-resident batch.  Returns BatchResult with per-system arrays."""
+    lib = _lib.load()
+    nsys, row0 = batch_info(A.K)
+    if nsys == 0:
+        raise ValueError("A is not built on a batch matrix")
+    device = _is_torch(b)
+    n = int(b.shape[0])
+    if n != row0[-1]:
+        raise ValueError(f"b has {n} entries, the batch {row0[-1]}")
+    o = idrs_opts_t()
+    keep = []
+    if tolerance is not None:
+        if tolerance < 0:
+            raise ValueError("Illegal value for parameter tolerance")
+        o.has_tolerance, o.tolerance = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    if device:
+        import torch
+
+        cplx = b.is_complex()
+        dt = torch.complex128 if cplx else torch.float64
+        if b.dtype != dt or not b.is_contiguous():
+            raise TypeError("device b must be a contiguous float64/complex128 CUDA tensor")
+        x = out if out is not None else torch.empty_like(b)
+        o.device_pointers = 1
+        for name, arr in (("x0", x0), ("PT_raw", PT_raw)):
+            if arr is not None:
+                if not (_is_torch(arr) and arr.is_cuda and arr.dtype == dt and arr.is_contiguous()):
+                    raise TypeError(f"{name} must be a contiguous CUDA tensor of dtype {dt}")
+                setattr(o, name, arr.data_ptr())
+        bp, xp = ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr())
+        npdt = np.complex128 if cplx else np.float64
+        torch.cuda.current_stream().synchronize()
+    else:
+        cplx = A.is_complex or np.iscomplexobj(b)
+        npdt = np.complex128 if cplx else np.float64
+        b = np.ascontiguousarray(b, dtype=npdt)
+        x = np.empty_like(b)
+        for name, arr in (("x0", x0), ("PT_raw", PT_raw)):
+            if arr is not None:
+                a = np.ascontiguousarray(arr, dtype=npdt)
+                keep.append(a)
+                setattr(o, name, a.ctypes.data)
+        bp, xp = _ptr(b), _ptr(x)
+    if omega is not None:
+        om = np.ascontiguousarray(np.atleast_1d(omega), dtype=npdt)
+        keep.append(om)
+        o.omega, o.n_omega = om.ctypes.data, len(om)
+    o.p_seed = int(p_seed) if p_seed is not None else 0
+    st = (idrs_stats_t * nsys)()
+    f = lib.idrs_idrs_batched_c128 if cplx else lib.idrs_idrs_batched_f64
+    check(f(A.handle, bp, int(s), ctypes.byref(o), xp, st))
+    flag = np.array([q.flag for q in st])
+    relres = np.array([q.relres for q in st])
+    its = np.array([q.iterations for q in st])
+    return BatchResult(x, flag, relres, its, st[0].solve_ms if nsys else 0.0)

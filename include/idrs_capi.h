@@ -247,6 +247,29 @@ int idrs_msqmridr_f64(idrs_matrix_t A, int n, const double *b, int n_sigma, cons
 int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
                        const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats);
 
+/* ---------------------------------------------------------------------
+ * batches of independent systems (no counterpart in the reference, which
+ * solves one system per call; these run IDRS, idrs.inc:1-311, on every
+ * system of a batch at once)
+ * ------------------------------------------------------------------- */
+
+/* A = block-diagonal matrix of nsys square CRS/COO user matrices (copies);
+   vectors of a batch are the systems' vectors concatenated in order. */
+int idrs_batch_format(int nsys, const idrs_user_matrix_t *K, idrs_user_matrix_t *A);
+/* nsys of a batch matrix and, if row0 != NULL, the first row of every
+   system (row0[nsys] = total rows); nsys = 0 for an ordinary matrix. */
+int idrs_batch_info(idrs_user_matrix_t A, int *nsys, int64_t *row0);
+/* x = IDRS(A_k, b_k, s, ...) for every system k of the batch matrix behind
+   A (created from a batch user matrix; diagonal scaling and polynomial
+   preconditioners apply to every system).  Options as idrs_idrs_f64 except
+   M1, U0 and resvec (rejected); without maximum_iterations each system
+   uses min(2 n_k, 1000).  stats: one per system (solve_ms: whole batch). */
+int idrs_idrs_batched_f64(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
+                          idrs_stats_t *stats);
+/* complex128 form of idrs_idrs_batched_f64 */
+int idrs_idrs_batched_c128(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
+                           idrs_stats_t *stats);
+
 /* PT = MAKE_P(...) for inspection: the orthonormalised shadow space PT(s,n)
    (RMAKE_P / CMAKE_P, idrs_module.f90:839-937).  method 1 Gram-Schmidt of
    PT_raw (or the library stream when NULL); method 2 PT = conj(r). Host arrays. */

@@ -1,0 +1,139 @@
+"""GPU parity of the batched IDR(s) solve (idrs_idrs_batched_*): every system
+of a batch must behave as its own IDRS solve (MODULES/IDRS/idrs.inc) on the
+oracle, with the same default shadow-space stream per system.
+
+Bar (FP64): at the convergence tolerance each system's flag equals the
+oracle's and its iteration count is within +-2; at an equal, fixed
+iteration count each system's solution agrees within 1e-10 relative.
+Systems that finish early stay frozen while the others continue.
+"""
+import numpy as np
+import pytest
+
+import idrs_amd
+import orc
+from idrs_amd import problems as P
+
+from helpers import oracle_crs, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems(specs, seed=0):
+    """2-D 5-point convection-diffusion systems (m, Pe, angle) and random b."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m, pe, ang in specs:
+        crs, _ = P.convection_diffusion_2d(m, pe, angle=ang)
+        out.append((crs, rng.uniform(-1.0, 1.0, crs.n)))
+    return out
+
+
+def _batch(systems, diag_precon=False, poly=None):
+    Ks = [idrs_amd.crs_format(c.n, c.n, c.row_ptr, c.col_ind, rval=c.val) for c, _ in systems]
+    KB = idrs_amd.batch_format(Ks)
+    A = idrs_amd.Matrix(KB)
+    if diag_precon:
+        A.set_precon(1)
+    if poly is not None:
+        kind, degree, foci = poly
+        idrs_amd.real_preconditioner(A, kind, degree, foci=foci)
+    b = np.concatenate([bb for _, bb in systems])
+    return A, b
+
+
+SPECS = [(5, 1.0, 0.3), (9, 30.0, 1.1), (17, 5.0, 2.0), (33, 80.0, 4.0), (40, 12.0, 5.5), (12, 60.0, 0.7)]
+
+
+def test_batch_partition(gpu):
+    systems = _systems(SPECS)
+    A, b = _batch(systems)
+    nsys, row0 = idrs_amd.batch_info(A.K)
+    assert nsys == len(SPECS)
+    assert list(np.diff(row0)) == [c.n for c, _ in systems]
+    # the block-diagonal product is every system's own product
+    v = np.random.default_rng(1).standard_normal(row0[-1])
+    w = A.K * v
+    for k, (c, _) in enumerate(systems):
+        assert np.allclose(w[row0[k]:row0[k + 1]], c.matvec(v[row0[k]:row0[k + 1]]), rtol=1e-13, atol=1e-10)
+
+
+@pytest.mark.parametrize("s,variant", [(2, 1), (4, 1), (1, 2)])
+def test_batch_matches_single_solves(gpu, s, variant):
+    systems = _systems(SPECS)
+    A, b = _batch(systems)
+    _, row0 = idrs_amd.batch_info(A.K)
+    r = idrs_amd.idrs_batched(A, b, s, tolerance=1e-8, maximum_iterations=3000, variant=variant)
+    for k, (c, bb) in enumerate(systems):
+        xo, fo, ro, io, _ = orc.idrs(oracle_crs(c), bb, s, tolerance=1e-8, maximum_iterations=3000, variant=variant)
+        xk = r.x[row0[k]:row0[k + 1]]
+        print(f"sys {k} n={c.n}: batch it={r.iterations[k]} flag={r.flag[k]}  oracle it={io} flag={fo}")
+        assert r.flag[k] == fo == 0 and abs(r.iterations[k] - io) <= 2
+        assert r.relres[k] <= 1e-8
+        assert np.linalg.norm(bb - c.matvec(xk)) <= 1.01e-8 * np.linalg.norm(bb)
+    # equal fixed iteration count: every system's iterate
+    g = idrs_amd.idrs_batched(A, b, s, tolerance=0.0, maximum_iterations=14, variant=variant)
+    assert np.all(g.iterations == 14) and np.all(g.flag == 1)
+    for k, (c, bb) in enumerate(systems):
+        xo = orc.idrs(oracle_crs(c), bb, s, tolerance=0.0, maximum_iterations=14, variant=variant)[0]
+        assert rel(g.x[row0[k]:row0[k + 1]], xo) < 1e-10
+
+
+def test_batch_chebyshev_and_scaling(gpu):
+    """Diagonal scaling and a Chebyshev polynomial on every system (the
+    pp_idrs operator), against the oracle's per-system operator."""
+    systems = _systems(SPECS[:4], seed=3)
+    poly = (2, 4, (0.15, 1.85))
+    A, b = _batch(systems, diag_precon=True, poly=poly)
+    _, row0 = idrs_amd.batch_info(A.K)
+    g = idrs_amd.idrs_batched(A, b, 4, tolerance=0.0, maximum_iterations=10)
+    for k, (c, bb) in enumerate(systems):
+        Ao = oracle_crs(c, diag_precon=True, poly=poly)
+        xo = orc.idrs(Ao, bb, 4, tolerance=0.0, maximum_iterations=10)[0]
+        assert rel(g.x[row0[k]:row0[k + 1]], xo) < 1e-10
+
+
+def test_batch_x0_pt_raw_and_device(gpu):
+    import torch
+
+    systems = _systems(SPECS[:3], seed=5)
+    A, b = _batch(systems)
+    _, row0 = idrs_amd.batch_info(A.K)
+    rng = np.random.default_rng(6)
+    x0 = rng.standard_normal(row0[-1])
+    PT = rng.standard_normal((row0[-1], 2))
+    h = idrs_amd.idrs_batched(A, b, 2, tolerance=0.0, maximum_iterations=9, x0=x0, PT_raw=PT)
+    for k, (c, bb) in enumerate(systems):
+        sl = slice(row0[k], row0[k + 1])
+        xo = orc.idrs(oracle_crs(c), bb, 2, tolerance=0.0, maximum_iterations=9, x0=x0[sl], PT_raw=PT[sl])[0]
+        assert rel(h.x[sl], xo) < 1e-10
+    dev = torch.device("cuda:0")
+    d = idrs_amd.idrs_batched(A, torch.from_numpy(b).to(dev), 2, tolerance=0.0, maximum_iterations=9,
+                              x0=torch.from_numpy(x0).to(dev), PT_raw=torch.from_numpy(PT).to(dev))
+    assert rel(d.x.cpu().numpy(), h.x) < 1e-14
+
+
+def test_batch_frozen_systems(gpu):
+    """One easy and one hard system: the easy one stops at its own count and
+    its solution stays what its own solve returns."""
+    systems = _systems([(6, 0.0, 0.0), (48, 90.0, 0.4)], seed=8)
+    A, b = _batch(systems)
+    _, row0 = idrs_amd.batch_info(A.K)
+    r = idrs_amd.idrs_batched(A, b, 2, tolerance=1e-10, maximum_iterations=4000)
+    its = []
+    for k, (c, bb) in enumerate(systems):
+        xo, fo, ro, io, _ = orc.idrs(oracle_crs(c), bb, 2, tolerance=1e-10, maximum_iterations=4000)
+        its.append(io)
+        assert r.flag[k] == fo == 0 and abs(r.iterations[k] - io) <= 2
+        assert rel(r.x[row0[k]:row0[k + 1]], xo) < 1e-8
+    assert its[0] < its[1]
+
+
+def test_batch_rejects(gpu):
+    crs, _ = P.convection_diffusion_2d(6, 1.0)
+    K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    with pytest.raises(ValueError):
+        idrs_amd.idrs_batched(idrs_amd.Matrix(K), np.ones(crs.n), 2)
+    A, b = _batch(_systems(SPECS[:2]))
+    with pytest.raises(idrs_amd.IdrsError):
+        idrs_amd.idrs_batched(A, b, 0)
