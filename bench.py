@@ -9,8 +9,9 @@ Gaussian shadow space (seed 0); ||r||/||b|| <= 1e-8.  A "step" is one full
 solve with b, x and the matrix already resident in HBM.
 
 Prints one JSON line.  roofline: the SpMV kernel (the dominant kernel),
-achieved = algorithmic bytes of the timed SpMV launches / their summed HIP
-event time, against the 8 TB/s HBM peak.  cpu_baseline: the oracle (the
+achieved = algorithmic bytes of the SpMV launches of one solve / their summed
+HIP event time (an extra solve after the timed steps, so the events do not
+slow the timed ones), against the 8 TB/s HBM peak.  cpu_baseline: the oracle (the
 project's scalar C restatement of the reference) on a bounded sample of the
 same workload, extrapolated to the GPU's iteration count.
 """
@@ -216,6 +217,123 @@ def bench_batched(args):
     print(json.dumps(out))
 
 
+def bench_mixed(args):
+    """The 200^3 IDR(4)+Chebyshev(8) solve to 1e-8 with single-precision inner
+    solves and FP64 residual replacement, against the FP64 solve of the same
+    system (DIA storage unless --format crs)."""
+    import numpy as np
+    import torch
+
+    import idrs_amd
+    from idrs_amd import problems as P
+
+    m, s, d = args.grid, args.s, args.degree
+    crs, dia = P.convection_diffusion_3d(m, 50.0, reaction=1.0)
+    n = crs.n
+    b = torch.from_numpy(np.random.default_rng(0).uniform(-1.0, 1.0, n)).to("cuda:0")
+    x = torch.empty_like(b)
+    fmt = "crs" if args.format == "crs" else "dia"
+    K = (idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val) if fmt == "crs"
+         else idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals))
+    A = idrs_amd.Matrix(K)
+    A.set_precon(1)
+    idrs_amd.real_preconditioner(A, 2, d, foci=(0.15, 1.85))
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t) * 1e3 / args.steps
+
+    rf, f_ms = timed(lambda: idrs_amd.idrs(A, b, s, tolerance=1e-8, maximum_iterations=20000, out=x))
+    rm, m_ms = timed(lambda: idrs_amd.idrs_mixed(A, b, s, tolerance=1e-8, maximum_iterations=20000,
+                                                 inner_tolerance=args.inner_tol, out=x))
+    # the operator's true residual of the mixed solution, in FP64 on the device
+    print(f"# fp64: {f_ms:.1f} ms, {rf.iterations} its; mixed: {m_ms:.1f} ms, {rm.iterations} inner its, "
+          f"{rm.outer} outer, relres {rm.relres:.3e}", file=sys.stderr, flush=True)
+    assert rm.flag == 0, f"mixed solve flag {rm.flag}"
+    out = {
+        "metric": "mixed-precision time-to-solution (ms) at rel. residual 1e-8 vs the FP64 solve",
+        "value": round(m_ms, 2), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(m_ms, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 residual", "data": "synthetic",
+        "config": {"workload": f"{m}^3 convection-diffusion-reaction, IDR({s}) + Chebyshev({d}) with diagonal "
+                               f"scaling, tol 1e-8 (true FP64 residual), storage {fmt.upper()}",
+                   "fp64_ms": round(f_ms, 2), "fp64_iterations": rf.iterations, "speedup_vs_fp64": round(f_ms / m_ms, 2),
+                   "inner_iterations": rm.iterations, "outer_steps": rm.outer, "relres": rm.relres,
+                   "inner_tolerance": args.inner_tol},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def bench_multishift(args):
+    """Shifted Helmholtz systems (A - sigma_k I) x_k = b, k = 1..16, complex128:
+    MSIDRS(4) and MSQMRIDR(8) (one Krylov space for all shifts) against a
+    loop of single-shift IDR(4) solves (the operator's seed set to sigma_k)."""
+    import numpy as np
+    import torch
+
+    import idrs_amd
+    from idrs_amd import problems as P
+
+    m = args.ms_grid
+    crs, _ = P.helmholtz_3d(m)
+    n = crs.n
+    sig = P.helmholtz_shifts(m, nshift=args.nshift)
+    bh = np.zeros(n, dtype=np.complex128)
+    bh[(m // 2) * (1 + m + m * m)] = 1.0
+    b = torch.from_numpy(bh).to("cuda:0")
+    K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    A = idrs_amd.Matrix(K)
+    tol, maxit = 1e-8, args.ms_maxit
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t) * 1e3
+
+    rm, ms_idrs = timed(lambda: idrs_amd.msidrs(A, b, sig, 4, tolerance=tol, maximum_iterations=maxit))
+    print(f"# msidrs(4): {ms_idrs:.1f} ms, {rm.iterations} its, flag {rm.flag}, max relres {rm.relres.max():.2e}",
+          file=sys.stderr, flush=True)
+    rq, ms_qmr = timed(lambda: idrs_amd.msqmridr(A, b, sig, 8, tolerance=tol, maximum_iterations=maxit))
+    print(f"# msqmridr(8): {ms_qmr:.1f} ms, {rq.iterations} its, flag {rq.flag}, max relres {rq.relres.max():.2e}",
+          file=sys.stderr, flush=True)
+    # per-shift loop: IDR(4) on K - sigma_k I via the operator's seed
+    loop_its, x = [], torch.empty_like(b)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for sk in sig:
+        A.set_seed(complex(sk))
+        r = idrs_amd.idrs(A, b, 4, tolerance=tol, maximum_iterations=maxit, out=x)
+        loop_its.append(r.iterations)
+    torch.cuda.synchronize()
+    loop_ms = (time.perf_counter() - t) * 1e3
+    A.set_seed(0j)
+    print(f"# per-shift IDR(4) loop: {loop_ms:.1f} ms, iterations {loop_its}", file=sys.stderr, flush=True)
+    best = min(ms_idrs, ms_qmr)
+    out = {
+        "metric": f"multishift time-to-solution (ms), {args.nshift} shifted Helmholtz systems, vs a per-shift IDR(4) loop",
+        "value": round(best, 2), "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(best, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"{m}^3 h^2-scaled 7-point Laplacian, sigma_k = (2 pi f_k h)^2 (1 - 0.05 i), "
+                               f"ppw 4..30, b = centre unit vector, tol 1e-8, complex128",
+                   "msidrs4_ms": round(ms_idrs, 2), "msidrs4_iterations": rm.iterations, "msidrs4_flag": rm.flag,
+                   "msqmridr8_ms": round(ms_qmr, 2), "msqmridr8_iterations": rq.iterations, "msqmridr8_flag": rq.flag,
+                   "per_shift_loop_ms": round(loop_ms, 2), "per_shift_iterations": loop_its,
+                   "speedup_vs_loop": round(loop_ms / best, 2)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,10 +344,14 @@ def main():
     ap.add_argument("--degree", type=int, default=8)
     ap.add_argument("--format", choices=["both", "crs", "dia"], default="both",
                     help="storage: both (headline DIA, then CRS), or one of them")
-    ap.add_argument("--form", choices=["solve", "batched"], default="solve",
+    ap.add_argument("--form", choices=["solve", "batched", "mixed", "multishift"], default="solve",
                     help="solve: the 200^3 IDR(4)+Chebyshev solve (default); batched: many small systems")
     ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
     ap.add_argument("--batch-maxit", type=int, default=4000)
+    ap.add_argument("--inner-tol", type=float, default=1e-4, help="inner tolerance of the mixed form")
+    ap.add_argument("--ms-grid", type=int, default=128, help="grid side of the multishift form")
+    ap.add_argument("--nshift", type=int, default=16)
+    ap.add_argument("--ms-maxit", type=int, default=20000)
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -244,6 +366,12 @@ def main():
         sys.exit(2)
     if args.form == "batched":
         bench_batched(args)
+        return
+    if args.form == "mixed":
+        bench_mixed(args)
+        return
+    if args.form == "multishift":
+        bench_multishift(args)
         return
 
     import numpy as np
@@ -289,16 +417,21 @@ def main():
         t_start = time.perf_counter()
         results = []
         for i in range(args.steps):
-            r = step(True)
+            r = step(False)
             results.append(r)
-            print(f"# {fmt} step {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} "
-                  f"solve={r.solve_ms:.1f} ms spmv={r.spmv_ms:.1f} ms / {r.spmv_launches} launches",
-                  file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_start
         ms_per_step = wall * 1e3 / args.steps
+        for i, r in enumerate(results):
+            print(f"# {fmt} step {i}: iterations={r.iterations} flag={r.flag} relres={r.relres:.3e} "
+                  f"solve={r.solve_ms:.1f} ms", file=sys.stderr, flush=True)
         last = results[-1]
         assert last.flag == 0, f"solve did not converge: flag {last.flag}"
+        # the SpMV roofline from one more solve, outside the timed steps, with
+        # HIP events around every SpMV launch on the solver's stream
+        results = [step(True)]
+        print(f"# {fmt} event-timed solve: spmv={results[0].spmv_ms:.1f} ms / {results[0].spmv_launches} launches",
+              file=sys.stderr, flush=True)
         # roofline of the SpMV kernel over the timed launches
         tot_bytes, launches = solve_spmv_bytes(n, nnz, s, d, last.iterations, s)
         if fmt == "dia":

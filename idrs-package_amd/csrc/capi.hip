@@ -4,6 +4,7 @@
 #include "idrs.hpp"
 #include "krylov.hpp"
 #include "batch.hpp"
+#include "mixed.hpp"
 
 #include <cstdarg>
 #include <cstring>
@@ -270,6 +271,56 @@ static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opt
             stats[k].spmv_launches = 0;
             stats[k].reserved = 0;
         }
+    });
+}
+
+template <class T>
+static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double inner_tol,
+                       int max_outer, double *x, idrs_stats_t *stats, int *outer_steps) {
+    return guarded([&] {
+        need(A && b && x, "null argument");
+        need(s >= 1 && s <= SMAX, "s must be in 1..16");
+        hipStream_t st = lib_stream();
+        idrs_opts_t o{};
+        if (opts) o = *opts;
+        need(!o.has_M1 && !o.U0 && !o.resvec && !o.omega, "M1, U0, omega and resvec are not offered here");
+        SolveOpts<T> so;
+        if (o.has_tolerance) so.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            so.has_maxit = true;
+            so.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) so.variant = o.variant;
+        so.p_seed = o.p_seed ? o.p_seed : 1;
+        const int prow = so.variant == 1 ? s : 1;
+        MixedStats ms;
+        if (o.device_pointers) {
+            so.x0 = (const T *)o.x0;
+            so.PT_raw = (const T *)o.PT_raw;
+            mixed_solve<T>(A->op, n, s, (const T *)b, (T *)x, so, inner_tol, max_outer, ms, st);
+        } else {
+            DVec<T> db((const T *)b, n, st), dx(n), dx0, dP;
+            if (o.x0) {
+                dx0 = DVec<T>((const T *)o.x0, n, st);
+                so.x0 = dx0.p;
+            }
+            if (o.PT_raw) {
+                dP = DVec<T>((const T *)o.PT_raw, (size_t)n * prow, st);
+                so.PT_raw = dP.p;
+            }
+            mixed_solve<T>(A->op, n, s, db.p, dx.p, so, inner_tol, max_outer, ms, st);
+            dx.download((T *)x, st);
+        }
+        if (stats) {
+            stats->flag = ms.flag;
+            stats->iterations = ms.iterations;
+            stats->relres = ms.relres;
+            stats->solve_ms = ms.solve_ms;
+            stats->spmv_ms = 0.0;
+            stats->spmv_launches = 0;
+            stats->reserved = 0;
+        }
+        if (outer_steps) *outer_steps = ms.outer;
     });
 }
 
@@ -603,6 +654,57 @@ int idrs_msqmridr_f64(idrs_matrix_t A, int n, const double *b, int n_sigma, cons
 int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, const double *sigma, int s,
                        const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats) {
     return krylov_entry<c128>(Kry::MSQMRIDR, A, n, b, n_sigma, sigma, s, opts, X, relres, stats);
+}
+
+int idrs_idrs_mixed_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
+                        double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps) {
+    return mixed_entry<double>(A, n, b, s, opts, inner_tolerance, max_outer, x, stats, outer_steps);
+}
+int idrs_idrs_mixed_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
+                         double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps) {
+    return mixed_entry<c128>(A, n, b, s, opts, inner_tolerance, max_outer, x, stats, outer_steps);
+}
+
+int idrs_crs_transpose(idrs_user_matrix_t K, idrs_user_matrix_t *KT) {
+    return guarded([&] {
+        need(K && KT, "null argument");
+        *KT = nullptr;
+        std::unique_ptr<DevMatrix> T(make_transpose(*K->M));
+        *KT = new idrs_user_matrix_s{T.release()};
+    });
+}
+int idrs_crs_transpose_refresh(idrs_user_matrix_t K, idrs_user_matrix_t KT) {
+    return guarded([&] {
+        need(K && KT, "null argument");
+        hipStream_t st = lib_stream();
+        transpose_refresh(*K->M, *KT->M, st);
+        HIP_TRY(hipStreamSynchronize(st));
+    });
+}
+int idrs_user_matrix_set_values_f64(idrs_user_matrix_t K, const double *vals, int device_pointer) {
+    return guarded([&] {
+        need(K && (vals || K->M->nnz == 0), "null argument");
+        need(K->M->kind != ValKind::Complex, "complex matrix: real values given");
+        hipStream_t st = lib_stream();
+        set_values(*K->M, vals, device_pointer != 0, st);
+        HIP_TRY(hipStreamSynchronize(st));
+    });
+}
+int idrs_crs_pattern_grad_f64(idrs_user_matrix_t K, const double *lambda, const double *x, double *grad,
+                              int device_pointers) {
+    return guarded([&] {
+        need(K && lambda && x && (grad || K->M->nnz == 0), "null argument");
+        hipStream_t st = lib_stream();
+        const DevMatrix &M = *K->M;
+        if (device_pointers) {
+            pattern_grad<double>(M, lambda, x, grad, st);
+            HIP_TRY(hipStreamSynchronize(st));
+        } else {
+            DVec<double> dl(lambda, M.nrows, st), dx(x, M.ncols, st), dg((size_t)M.nnz);
+            pattern_grad<double>(M, dl.p, dx.p, dg.p, st);
+            dg.download(grad, st);
+        }
+    });
 }
 
 int idrs_batch_format(int nsys, const idrs_user_matrix_t *K, idrs_user_matrix_t *A) {

@@ -692,6 +692,10 @@ template int idrs_solve<double>(const Operator &, long, int, const double *, dou
                                 SolveStats &, hipStream_t);
 template int idrs_solve<c128>(const Operator &, long, int, const c128 *, c128 *, const SolveOpts<c128> &,
                               SolveStats &, hipStream_t);
+template int idrs_solve<float>(const Operator &, long, int, const float *, float *, const SolveOpts<float> &,
+                               SolveStats &, hipStream_t);
+template int idrs_solve<c64>(const Operator &, long, int, const c64 *, c64 *, const SolveOpts<c64> &, SolveStats &,
+                             hipStream_t);
 template void make_p_device<double>(long, int, int, const double *, const double *, double *, unsigned long long,
                                     hipStream_t);
 template void make_p_device<c128>(long, int, int, const c128 *, const c128 *, c128 *, unsigned long long, hipStream_t);

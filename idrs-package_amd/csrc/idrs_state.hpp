@@ -24,11 +24,17 @@ struct State {
 };
 
 __device__ inline double sdiv(double a, double b) { return a / b; }
+__device__ inline float sdiv(float a, float b) { return a / b; }
 __device__ inline c128 sdiv(c128 a, c128 b) { return a / b; }
+__device__ inline c64 sdiv(c64 a, c64 b) { return a / b; }
 __device__ inline double sabs(double a) { return fabs(a); }
+__device__ inline double sabs(float a) { return fabs((double)a); }
 __device__ inline double sabs(c128 a) { return hypot(a.re, a.im); }
+__device__ inline double sabs(c64 a) { return hypot((double)a.re, (double)a.im); }
 __device__ inline double sre(double a) { return a; }
+__device__ inline double sre(float a) { return a; }
 __device__ inline double sre(c128 a) { return a.re; }
+__device__ inline double sre(c64 a) { return a.re; }
 
 template <class T>
 __device__ inline T fmaT(T a, T b, T c) { return Tr<T>::fma(a, b, c); }

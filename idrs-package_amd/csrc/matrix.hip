@@ -44,6 +44,7 @@ DevMatrix::~DevMatrix() {
     (void)hipFree(val);
     (void)hipFree(val32);
     (void)hipFree(diag);
+    (void)hipFree(tperm);
 }
 
 Partition::~Partition() {
@@ -772,6 +773,130 @@ DevMatrix *make_blockdiag(int nsys, const DevMatrix *const *Ks) {
     M->part = std::move(P);
     return M.release();
 }
+
+// ---------------------------------------------------------------------------
+// transposes, value updates and the pattern gradient (differentiable solves)
+// ---------------------------------------------------------------------------
+DevMatrix *make_transpose(const DevMatrix &K) {
+    if (K.fmt != DevFmt::CRS) throw std::invalid_argument("transpose needs a CRS/COO matrix");
+    if (K.part) throw std::invalid_argument("transpose of a batch matrix is not offered");
+    const int n = K.nrows, m = K.ncols;
+    const long nnz = K.nnz;
+    std::vector<int> rp(n + 1), col((size_t)nnz);
+    HIP_TRY(hipMemcpy(rp.data(), K.row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    if (nnz) HIP_TRY(hipMemcpy(col.data(), K.col, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+    // counting sort by column; rows visited in order keep each new row sorted
+    std::vector<int> trp(m + 1, 0), tcol((size_t)nnz), perm((size_t)nnz);
+    for (long k = 0; k < nnz; k++) trp[col[k] + 1]++;
+    for (int j = 0; j < m; j++) trp[j + 1] += trp[j];
+    std::vector<int> pos(trp.begin(), trp.end() - 1);
+    for (int i = 0; i < n; i++)
+        for (int k = rp[i]; k < rp[i + 1]; k++) {
+            const int p = pos[col[k]]++;
+            tcol[p] = i;
+            perm[p] = k;
+        }
+    std::unique_ptr<DevMatrix> T(new DevMatrix);
+    T->orig_format = 2;
+    T->fmt = DevFmt::CRS;
+    T->kind = K.kind;
+    T->nrows = m;
+    T->ncols = n;
+    T->nnz = nnz;
+    T->row_ptr = dev_upload(trp.data(), trp.size());
+    T->col = dev_upload(tcol.data(), (size_t)nnz);
+    T->tperm = dev_upload(perm.data(), (size_t)nnz);
+    const size_t vsz = K.kind == ValKind::Complex ? sizeof(c128) : sizeof(double);
+    HIP_TRY(hipMalloc(&T->val, vsz * (nnz ? nnz : 1)));
+    HIP_TRY(hipMalloc(&T->diag, vsz * (m ? m : 1)));
+    double avg = m > 0 ? (double)nnz / m : 1.0;
+    int g = 2;
+    while (g < 64 && g < avg) g *= 2;
+    T->group = g;
+    transpose_refresh(K, *T, 0);
+    HIP_TRY(hipDeviceSynchronize());
+    return T.release();
+}
+
+template <class V>
+__global__ void k_gather(long n, const int *__restrict__ perm, const V *__restrict__ src, V *__restrict__ dst) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[perm[i]];
+}
+
+// d[i] = sum of the stored entries (i, i), zero -> 1 (real_diagonal /
+// complex_diagonal, user_module.f90:960-1161)
+template <class V>
+__global__ void k_crs_diag(int n, const int *__restrict__ rp, const int *__restrict__ col, const V *__restrict__ val,
+                           V *__restrict__ d) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        V acc = V(0);
+        for (int k = rp[i]; k < rp[i + 1]; k++)
+            if (col[k] == i) acc = acc + val[k];
+        if (Tr<V>::absv(acc) == 0.0) acc = V(1.0);
+        d[i] = acc;
+    }
+}
+
+static void refresh_diag(DevMatrix &K, hipStream_t st) {
+    const int g = vec_grid(K.nrows);
+    if (K.kind == ValKind::Complex)
+        hipLaunchKernelGGL(k_crs_diag<c128>, dim3(g), dim3(256), 0, st, K.nrows, K.row_ptr, K.col,
+                           (const c128 *)K.val, (c128 *)K.diag);
+    else
+        hipLaunchKernelGGL(k_crs_diag<double>, dim3(g), dim3(256), 0, st, K.nrows, K.row_ptr, K.col,
+                           (const double *)K.val, (double *)K.diag);
+    LAUNCH_CHECK();
+}
+
+void transpose_refresh(const DevMatrix &K, DevMatrix &KT, hipStream_t st) {
+    if (!KT.tperm || KT.nnz != K.nnz || KT.nrows != K.ncols || KT.kind != K.kind)
+        throw std::invalid_argument("not a transpose of this matrix");
+    if (K.nnz) {
+        const int g = vec_grid(K.nnz);
+        if (K.kind == ValKind::Complex)
+            hipLaunchKernelGGL(k_gather<c128>, dim3(g), dim3(256), 0, st, K.nnz, KT.tperm, (const c128 *)K.val,
+                               (c128 *)KT.val);
+        else
+            hipLaunchKernelGGL(k_gather<double>, dim3(g), dim3(256), 0, st, K.nnz, KT.tperm, (const double *)K.val,
+                               (double *)KT.val);
+        LAUNCH_CHECK();
+    }
+    if (KT.nrows) refresh_diag(KT, st);
+}
+
+void set_values(DevMatrix &K, const void *vals, bool device, hipStream_t st) {
+    if (K.fmt != DevFmt::CRS) throw std::invalid_argument("set_values needs a CRS/COO matrix");
+    if (K.orig_format == 3) throw std::invalid_argument("COO matrices are stored row-sorted; set CRS values");
+    if (K.tperm) throw std::invalid_argument("refresh a transpose from its source matrix instead");
+    const size_t vsz = K.kind == ValKind::Complex ? sizeof(c128) : sizeof(double);
+    if (K.nnz)
+        HIP_TRY(hipMemcpyAsync(K.val, vals, vsz * K.nnz, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+    (void)hipFree(K.val32);
+    K.val32 = nullptr;
+    if (K.nrows) refresh_diag(K, st);
+}
+
+template <class T, class V>
+__global__ void k_pattern_grad(int n, const int *__restrict__ rp, const int *__restrict__ col,
+                               const T *__restrict__ lam, const T *__restrict__ x, T *__restrict__ grad) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const T li = lam[i];
+        for (int k = rp[i]; k < rp[i + 1]; k++) grad[k] = -(li * x[col[k]]);
+    }
+}
+
+template <class T>
+void pattern_grad(const DevMatrix &K, const T *lambda, const T *x, T *grad, hipStream_t st) {
+    if (K.fmt != DevFmt::CRS) throw std::invalid_argument("pattern gradient needs a CRS/COO matrix");
+    if (K.orig_format == 3) throw std::invalid_argument("pattern gradient is in CRS entry order; use a CRS matrix");
+    if (!K.nrows) return;
+    hipLaunchKernelGGL((k_pattern_grad<T, T>), dim3(vec_grid(K.nrows)), dim3(256), 0, st, K.nrows, K.row_ptr, K.col,
+                       lambda, x, grad);
+    LAUNCH_CHECK();
+}
+template void pattern_grad<double>(const DevMatrix &, const double *, const double *, double *, hipStream_t);
+template void pattern_grad<c128>(const DevMatrix &, const c128 *, const c128 *, c128 *, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // elementwise kernels: w = v / D  (rprecon), w = c*v

@@ -66,6 +66,8 @@ struct DevMatrix {
     void *diag = nullptr;
     // set for block-diagonal batches (make_blockdiag)
     std::unique_ptr<Partition> part;
+    // set for transposes (make_transpose): val[p] = source val[tperm[p]]
+    int *tperm = nullptr;
     ~DevMatrix();
 };
 
@@ -158,6 +160,17 @@ DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const v
 // block-diagonal CRS matrix of nsys square CRS systems (copies), with its
 // Partition
 DevMatrix *make_blockdiag(int nsys, const DevMatrix *const *Ks);
+// CRS of K^T (rows of K^T list K's entries of that column in increasing row
+// order), with the value permutation kept for transpose_refresh
+DevMatrix *make_transpose(const DevMatrix &K);
+// KT.val = K.val permuted (after K's values changed)
+void transpose_refresh(const DevMatrix &K, DevMatrix &KT, hipStream_t st);
+// replaces the stored values of a CRS matrix (same pattern) and recomputes
+// its diagonal; vals is a device or host array of nnz values
+void set_values(DevMatrix &K, const void *vals, bool device, hipStream_t st);
+// grad[k] = -lambda[i] * x[j] for every stored entry k = (i, j) of K
+template <class T>
+void pattern_grad(const DevMatrix &K, const T *lambda, const T *x, T *grad, hipStream_t st);
 
 // operator construction
 void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shift);

@@ -1,0 +1,188 @@
+// Mixed-precision IDR(s) (FP32 inner Krylov solve, FP64 defect correction).
+//
+// Each outer step: r = b - A x in FP64 (the true residual, A applied with
+// FP64 values), stop if ||r|| <= tol ||b||; else solve A d = r / ||r|| by
+// IDR(s) in single precision -- every SpMV reads the FP32 copy of the
+// (scaled) values and FP32 vectors, every vector kernel moves FP32 -- to a
+// relative tolerance inner_tol (no tighter than the remaining FP64 gap
+// needs), and update x += ||r|| d in FP64.  The inner solve is the same
+// device IDR(s) as the FP64 one (idrs.inc:1-311), instantiated for float /
+// complex<float>.  The FP32 operator differs from A by O(1e-7) relative, so
+// each outer step contracts the error by about max(inner_tol, cond * 1e-7).
+#include "mixed.hpp"
+
+#include <cmath>
+#include <stdexcept>
+
+namespace idrs {
+namespace {
+
+template <class T> struct Lo;
+template <> struct Lo<double> { using type = float; };
+template <> struct Lo<c128> { using type = c64; };
+
+__device__ inline float lo(double v) { return (float)v; }
+__device__ inline c64 lo(c128 v) { return c64((float)v.re, (float)v.im); }
+__device__ inline double hi(float v) { return (double)v; }
+__device__ inline c128 hi(c64 v) { return c128((double)v.re, (double)v.im); }
+
+// r32 = r64 * c  (c = 1/||r||)
+template <class T64, class T32>
+__global__ void k_to_lo(long n, const T64 *__restrict__ r, double c, T32 *__restrict__ r32) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        if constexpr (Tr<T64>::is_complex) r32[q] = lo(T64(r[q].re * c, r[q].im * c));
+        else r32[q] = lo(r[q] * c);
+    }
+}
+
+// x = x + c * d32
+template <class T64, class T32>
+__global__ void k_add_hi(long n, T64 *__restrict__ x, double c, const T32 *__restrict__ d32) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const T64 d = hi(d32[q]);
+        if constexpr (Tr<T64>::is_complex) x[q] = T64(fma(c, d.re, x[q].re), fma(c, d.im, x[q].im));
+        else x[q] = fma(c, d, x[q]);
+    }
+}
+
+// r = b - t, partials |r|^2
+template <class T>
+__global__ __launch_bounds__(256) void k_res(long n, const T *__restrict__ b, const T *__restrict__ t,
+                                             T *__restrict__ r, T *partials) {
+    __shared__ T lds[4];
+    T e[1] = {T(0)};
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const T rq = t ? b[q] - t[q] : b[q];
+        r[q] = rq;
+        e[0] = e[0] + T(Tr<T>::abs2(rq));
+    }
+    block_sum<T, 1>(e, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = e[0];
+}
+
+template <class T>
+__global__ __launch_bounds__(FIN_THREADS) void k_sum(const T *partials, int nparts, T *out) {
+    __shared__ T red[1];
+    reduce_partials(partials, nparts, 1, red);
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+struct Dev {
+    void *p = nullptr;
+    explicit Dev(size_t bytes) { HIP_TRY(hipMalloc(&p, bytes ? bytes : 1)); }
+    ~Dev() { (void)hipFree(p); }
+    Dev(const Dev &) = delete;
+    Dev &operator=(const Dev &) = delete;
+    template <class V> V *as() const { return (V *)p; }
+};
+
+inline double real_part(double v) { return v; }
+inline double real_part(c128 v) { return v.re; }
+
+}  // namespace
+
+template <class T64>
+void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const SolveOpts<T64> &o, double inner_tol,
+                 int max_outer, MixedStats &st, hipStream_t stream) {
+    using T32 = typename Lo<T64>::type;
+    if (n < 1) throw std::invalid_argument("empty system");
+    if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
+    if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
+    if (!(inner_tol > 0 && inner_tol < 1)) throw std::invalid_argument("inner tolerance must be in (0, 1)");
+    if (max_outer < 1) throw std::invalid_argument("at least one outer step");
+    if (o.has_m1 || o.U0 || o.resvec || o.omega) throw std::invalid_argument("M1, U0, omega and resvec are not offered");
+    if (A.values_complex()) throw std::invalid_argument("mixed precision needs real matrix values");
+    if (A.seed != std::complex<double>(0.0, 0.0)) throw std::invalid_argument("mixed precision without a seed shift");
+    const int maxit = o.has_maxit ? o.maxit : (int)std::min<long>(2 * n, 1000);
+    op_make_lowprec(const_cast<Operator &>(A));
+
+    const size_t v64 = sizeof(T64) * (size_t)n, v32 = sizeof(T32) * (size_t)n;
+    const int prow = o.variant == 1 ? s : 1;
+    Dev br(v64), bt(v64), bw1(v64), bw2(v64), br32(v32), bd32(v32), bpart(sizeof(T64) * MAX_PARTS), bsum(sizeof(T64));
+    std::unique_ptr<Dev> bP32;
+    T64 *r = br.as<T64>(), *t = bt.as<T64>(), *w1 = bw1.as<T64>(), *w2 = bw2.as<T64>(), *part = bpart.as<T64>();
+    T32 *r32 = br32.as<T32>(), *d32 = bd32.as<T32>();
+    const int gv = vec_grid(n);
+    EpiArgs<T64> none{};
+    if (o.PT_raw) {
+        bP32.reset(new Dev(v32 * prow));
+        hipLaunchKernelGGL((k_to_lo<T64, T32>), dim3(vec_grid(n * prow)), dim3(256), 0, stream, n * (long)prow,
+                           o.PT_raw, 1.0, bP32->as<T32>());
+        LAUNCH_CHECK();
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, stream));
+    auto norm_of = [&](const T64 *bv, const T64 *tv, T64 *rv) {
+        hipLaunchKernelGGL((k_res<T64>), dim3(gv), dim3(256), 0, stream, n, bv, tv, rv, part);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_sum<T64>), dim3(1), dim3(FIN_THREADS), 0, stream, (const T64 *)part, gv, bsum.as<T64>());
+        LAUNCH_CHECK();
+        T64 h;
+        HIP_TRY(hipMemcpyAsync(&h, bsum.p, sizeof(T64), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return std::sqrt(real_part(h));
+    };
+    const double normb = norm_of(b, nullptr, r);
+    const double tolb = o.tol * normb;
+    if (o.x0) {
+        HIP_TRY(hipMemcpyAsync(x, o.x0, v64, hipMemcpyDeviceToDevice, stream));
+        op_apply<T64>(A, x, t, w1, w2, EPI_NONE, none, nullptr, stream);
+    } else {
+        HIP_TRY(hipMemsetAsync(x, 0, v64, stream));
+    }
+    double normr = norm_of(b, o.x0 ? t : nullptr, r);
+    int info = -1, its = 0, outer = 0;
+    while (true) {
+        if (normr <= tolb) {
+            info = 0;
+            break;
+        }
+        if (outer == max_outer || its >= maxit) {
+            info = 1;
+            break;
+        }
+        hipLaunchKernelGGL((k_to_lo<T64, T32>), dim3(gv), dim3(256), 0, stream, n, (const T64 *)r, 1.0 / normr, r32);
+        LAUNCH_CHECK();
+        SolveOpts<T32> io;
+        // no tighter than the FP64 target needs (half of it, for the FP32 error)
+        io.tol = std::max(inner_tol, 0.5 * tolb / normr);
+        io.has_maxit = true;
+        io.maxit = maxit - its;
+        io.variant = o.variant;
+        io.PT_raw = bP32 ? bP32->as<T32>() : nullptr;
+        io.p_seed = o.p_seed;
+        SolveStats is;
+        idrs_solve<T32>(A, n, s, r32, d32, io, is, stream);
+        its += is.iterations;
+        outer++;
+        hipLaunchKernelGGL((k_add_hi<T64, T32>), dim3(gv), dim3(256), 0, stream, n, x, normr, (const T32 *)d32);
+        LAUNCH_CHECK();
+        op_apply<T64>(A, x, t, w1, w2, EPI_NONE, none, nullptr, stream);
+        const double prev = normr;
+        normr = norm_of(b, t, r);
+        if (is.flag == 3 || !(normr < prev)) {     // inner breakdown, or no progress
+            info = normr <= tolb ? 0 : 3;
+            break;
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.flag = info;
+    st.iterations = its;
+    st.outer = outer;
+    st.relres = normr / normb;
+    st.solve_ms = ms;
+}
+
+template void mixed_solve<double>(const Operator &, long, int, const double *, double *, const SolveOpts<double> &,
+                                  double, int, MixedStats &, hipStream_t);
+template void mixed_solve<c128>(const Operator &, long, int, const c128 *, c128 *, const SolveOpts<c128> &, double,
+                                int, MixedStats &, hipStream_t);
+
+}  // namespace idrs

@@ -9,6 +9,7 @@ from .api import (  # noqa: F401
     BatchResult,
     IdrsResult,
     KrylovResult,
+    MixedResult,
     Matrix,
     UserMatrix,
     batch_format,
@@ -21,6 +22,7 @@ from .api import (  # noqa: F401
     dia_format,
     idrs,
     idrs_batched,
+    idrs_mixed,
     make_p,
     msidrs,
     msqmridr,
@@ -29,3 +31,4 @@ from .api import (  # noqa: F401
     real_preconditioner,
     scaleback,
 )
+from .autograd import SparseSystem, idrs_solve  # noqa: F401,E402

@@ -27,6 +27,8 @@ IdrsResult = namedtuple("IdrsResult", "x flag relres iterations resvec solve_ms 
 KrylovResult = namedtuple("KrylovResult", "x flag relres iterations resvec solve_ms colfac")
 # per-system arrays flag, relres, iterations; solve_ms of the whole batch
 BatchResult = namedtuple("BatchResult", "x flag relres iterations solve_ms")
+# iterations: inner (single-precision) iterations; outer: FP64 residual replacements
+MixedResult = namedtuple("MixedResult", "x flag relres iterations outer solve_ms")
 
 
 def _ptr(a):
@@ -582,3 +584,55 @@ def idrs_batched(A, b, s, tolerance=None, maximum_iterations=None, variant=None,
     relres = np.array([q.relres for q in st])
     its = np.array([q.iterations for q in st])
     return BatchResult(x, flag, relres, its, st[0].solve_ms if nsys else 0.0)
+
+
+def idrs_mixed(A, b, s, tolerance=None, maximum_iterations=None, variant=None, x0=None, PT_raw=None, p_seed=None,
+               inner_tolerance=1e-5, max_outer=10, out=None):
+    """x = A^-1 b to the FP64 tolerance with single-precision inner IDR(s)
+    solves and FP64 residual replacement (idrs_idrs_mixed_*).  b: numpy
+    array or CUDA tensor as for idrs()."""
+    lib = _lib.load()
+    device = _is_torch(b)
+    n = int(b.shape[0])
+    o = idrs_opts_t()
+    keep = []
+    if tolerance is not None:
+        if tolerance < 0:
+            raise ValueError("Illegal value for parameter tolerance")
+        o.has_tolerance, o.tolerance = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    if device:
+        import torch
+
+        cplx = b.is_complex()
+        dt = torch.complex128 if cplx else torch.float64
+        if b.dtype != dt or not b.is_contiguous():
+            raise TypeError("device b must be a contiguous float64/complex128 CUDA tensor")
+        x = out if out is not None else torch.empty_like(b)
+        o.device_pointers = 1
+        for name, arr in (("x0", x0), ("PT_raw", PT_raw)):
+            if arr is not None:
+                setattr(o, name, arr.data_ptr())
+        bp, xp = ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr())
+        torch.cuda.current_stream().synchronize()
+    else:
+        cplx = np.iscomplexobj(b)
+        npdt = np.complex128 if cplx else np.float64
+        b = np.ascontiguousarray(b, dtype=npdt)
+        x = np.empty_like(b)
+        for name, arr in (("x0", x0), ("PT_raw", PT_raw)):
+            if arr is not None:
+                a = np.ascontiguousarray(arr, dtype=npdt)
+                keep.append(a)
+                setattr(o, name, a.ctypes.data)
+        bp, xp = _ptr(b), _ptr(x)
+    o.p_seed = int(p_seed) if p_seed is not None else 0
+    st = idrs_stats_t()
+    outer = ctypes.c_int()
+    f = lib.idrs_idrs_mixed_c128 if cplx else lib.idrs_idrs_mixed_f64
+    check(f(A.handle, n, bp, int(s), ctypes.byref(o), float(inner_tolerance), int(max_outer), xp, ctypes.byref(st),
+            ctypes.byref(outer)))
+    return MixedResult(x, st.flag, st.relres, st.iterations, outer.value, st.solve_ms)

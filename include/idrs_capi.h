@@ -248,6 +248,45 @@ int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, con
                        const idrs_krylov_opts_t *opts, double *X, double *relres, idrs_stats_t *stats);
 
 /* ---------------------------------------------------------------------
+ * mixed precision (no counterpart in the reference, which is FP64 only)
+ * ------------------------------------------------------------------- */
+
+/* x = IDRS(A, b, s, ...) to the FP64 tolerance opts->tolerance with
+   single-precision inner solves: r = b - A x in FP64, then IDR(s) with FP32
+   values and vectors on A d = r/||r|| to inner_tolerance, x += ||r|| d,
+   repeated (at most max_outer times) until the true FP64 residual meets the
+   tolerance.  opts->maximum_iterations bounds the inner iterations in total;
+   x0, PT_raw, variant and device_pointers as for idrs_idrs_f64 (M1, U0,
+   omega, resvec rejected).  Matrix values must be real and the operator
+   without a seed shift.  stats->iterations: inner iterations; *outer_steps:
+   FP64 residual replacements. */
+int idrs_idrs_mixed_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
+                        double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps);
+/* complex128 vectors (complex64 inner solves) on a real-valued operator */
+int idrs_idrs_mixed_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
+                         double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps);
+
+/* ---------------------------------------------------------------------
+ * differentiable solves (no counterpart in the reference): the adjoint
+ * system A^T lambda = dL/dx is solved by IDRS on the transposed matrix, and
+ * dL/dA on A's sparsity pattern is -lambda_i x_j.
+ * ------------------------------------------------------------------- */
+
+/* KT = transpose of a CRS/COO user matrix (CRS; keeps the value map) */
+int idrs_crs_transpose(idrs_user_matrix_t K, idrs_user_matrix_t *KT);
+/* KT's values = K's values after K's values changed */
+int idrs_crs_transpose_refresh(idrs_user_matrix_t K, idrs_user_matrix_t KT);
+/* replaces the nnz stored values of a CRS matrix (same pattern, CRS entry
+   order); vals is a device pointer if device_pointer != 0.  The diagonal
+   is recomputed; operators built with diagonal scaling or low-precision
+   copies on K must be rebuilt. */
+int idrs_user_matrix_set_values_f64(idrs_user_matrix_t K, const double *vals, int device_pointer);
+/* grad[k] = -lambda[i] * x[j] for every stored entry k = (i, j) of the CRS
+   matrix K, in CRS entry order (device pointers if device_pointers != 0) */
+int idrs_crs_pattern_grad_f64(idrs_user_matrix_t K, const double *lambda, const double *x, double *grad,
+                              int device_pointers);
+
+/* ---------------------------------------------------------------------
  * batches of independent systems (no counterpart in the reference, which
  * solves one system per call; these run IDRS, idrs.inc:1-311, on every
  * system of a batch at once)
