@@ -27,15 +27,16 @@ namespace {
 struct BV {
     int nsys, ntiles;
     const int *tile_sys;   // ntiles
-    const long *tile_r0;   // ntiles+1
-    const int *tile0;      // nsys+1
+    const long *tile_r0;   // ntiles: first row
+    const long *tile_r1;   // ntiles: end row
+    const int *tile0;      // nsys+1: tiles [tile0[k], tile0[k+1]) belong to system k
     const long *row0;      // nsys+1
 };
 
 constexpr int BT = 256;    // threads of the tile and per-system kernels
 
 #define TILE_ROWS(q)                                                                                  \
-    for (long q = bv.tile_r0[blockIdx.x] + threadIdx.x, q##_end = bv.tile_r0[blockIdx.x + 1]; q < q##_end; \
+    for (long q = bv.tile_r0[blockIdx.x] + threadIdx.x, q##_end = bv.tile_r1[blockIdx.x]; q < q##_end; \
          q += BT)
 
 // Sum of partials[q * ntiles + t] over the tiles t of system blockIdx.x, for
@@ -268,14 +269,19 @@ __global__ __launch_bounds__(BT) void k_bfin_cycle(BV bv, State<T> *st, const T 
     if (threadIdx.x == 0) scalar_cycle(S, red, (double *)nullptr);
 }
 
-// flag = 1 when every system has finished
+// flag[0] = 1 when every system has finished; flag[1 + k] = 1 while system
+// k runs
 template <class T>
 __global__ __launch_bounds__(BT) void k_bdone(int nsys, const State<T> *st, int *flag) {
     __shared__ int open;
     if (threadIdx.x == 0) open = 0;
     __syncthreads();
     int mine = 0;
-    for (int b = threadIdx.x; b < nsys; b += BT) mine |= (st[b].info < 0) ? 1 : 0;
+    for (int b = threadIdx.x; b < nsys; b += BT) {
+        const int live = (st[b].info < 0) ? 1 : 0;
+        flag[1 + b] = live;
+        mine |= live;
+    }
     if (mine) atomicOr(&open, 1);
     __syncthreads();
     if (threadIdx.x == 0) flag[0] = open ? 0 : 1;
@@ -431,7 +437,13 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
 
     const size_t vb = sizeof(T) * (size_t)ntot;
     DBuf bG(vb * s), bU(vb * s), br(vb), bt(vb), bw1(vb), bw2(vb), btmp(vb), bP(vb * prow);
-    DBuf bpart(sizeof(T) * (size_t)MAX_NV * ntiles), bst(sizeof(State<T>) * nsys), bflag(sizeof(int) * 2);
+    DBuf bpart(sizeof(T) * (size_t)MAX_NV * ntiles), bst(sizeof(State<T>) * nsys), bflag(sizeof(int) * (nsys + 1));
+    // two sets of compacted tile lists (tiles of unfinished systems only),
+    // alternated so a set is rewritten only when no queued launch reads it
+    DBuf ctile_sys[2] = {DBuf(sizeof(int) * ntiles), DBuf(sizeof(int) * ntiles)};
+    DBuf ctile_r0[2] = {DBuf(sizeof(long) * ntiles), DBuf(sizeof(long) * ntiles)};
+    DBuf ctile_r1[2] = {DBuf(sizeof(long) * ntiles), DBuf(sizeof(long) * ntiles)};
+    DBuf ctile0[2] = {DBuf(sizeof(int) * (nsys + 1)), DBuf(sizeof(int) * (nsys + 1))};
     T *G = bG.as<T>(), *U = bU.as<T>(), *r = br.as<T>(), *t = bt.as<T>(), *w1 = bw1.as<T>(), *w2 = bw2.as<T>();
     T *tmp = btmp.as<T>(), *P = bP.as<T>(), *part_d = bpart.as<T>();
     State<T> *st = bst.as<State<T>>();
@@ -443,8 +455,10 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
         omega_d = bom->as<T>();
         HIP_TRY(hipMemcpyAsync(omega_d, o.omega, sizeof(T) * o.n_omega, hipMemcpyHostToDevice, stream));
     }
-    BV bv{nsys, ntiles, part->tile_sys, part->tile_r0, part->tile0, part->row0};
-    const dim3 gt(ntiles), gsys(nsys);
+    const BV bv_all{nsys, ntiles, part->tile_sys, part->tile_r0, part->tile_r0 + 1, part->tile0, part->row0};
+    BV bv = bv_all;
+    dim3 gt(ntiles);
+    const dim3 gsys(nsys);
     TileCtx all{part, nullptr, 0};
     TileCtx live{part, &st[0].info, (int)(sizeof(State<T>) / sizeof(int))};
     static_assert(sizeof(State<T>) % sizeof(int) == 0, "State stride in ints");
@@ -533,8 +547,45 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
         LAUNCH_CHECK();
     };
 
+    // h_flag: two slots of nsys+1 ints (done flag, then one live flag per system)
     int *h_flag = nullptr;
-    HIP_TRY(hipHostMalloc((void **)&h_flag, 2 * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&h_flag, 2 * sizeof(int) * (nsys + 1), hipHostMallocDefault));
+    int cset = 0;            // compacted set the next rewrite uses
+    // Restricts every later tile launch to the unfinished systems' tiles once
+    // they are at most 3/4 of the tiles launched so far.  Finished systems
+    // keep an empty tile range (their reduction kernels return at once).
+    auto compact = [&](const int *liveflag) {
+        std::vector<int> tsys, t0(nsys + 1);
+        std::vector<long> r0v, r1v;
+        for (int k = 0; k < nsys; k++) {
+            t0[k] = (int)tsys.size();
+            if (!liveflag[k]) continue;
+            for (int t = part->h_tile0[k]; t < part->h_tile0[k + 1]; t++) {
+                tsys.push_back(k);
+                const long a = part->h_row0[k] + (long)(t - part->h_tile0[k]) * BATCH_TILE;
+                r0v.push_back(a);
+                r1v.push_back(std::min(a + (long)BATCH_TILE, part->h_row0[k + 1]));
+            }
+        }
+        t0[nsys] = (int)tsys.size();
+        const int nt = (int)tsys.size();
+        if (nt == 0 || 4L * nt > 3L * bv.ntiles) return;
+        HIP_TRY(hipMemcpy(ctile_sys[cset].p, tsys.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctile_r0[cset].p, r0v.data(), sizeof(long) * nt, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctile_r1[cset].p, r1v.data(), sizeof(long) * nt, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(ctile0[cset].p, t0.data(), sizeof(int) * (nsys + 1), hipMemcpyHostToDevice));
+        bv.ntiles = nt;
+        bv.tile_sys = ctile_sys[cset].as<int>();
+        bv.tile_r0 = ctile_r0[cset].as<long>();
+        bv.tile_r1 = ctile_r1[cset].as<long>();
+        bv.tile0 = ctile0[cset].as<int>();
+        live.ntiles = nt;
+        live.tile_sys = bv.tile_sys;
+        live.tile_r0 = bv.tile_r0;
+        live.tile_r1 = bv.tile_r1;
+        gt = dim3(nt);
+        cset ^= 1;
+    };
     hipEvent_t pe[2];
     HIP_TRY(hipEventCreateWithFlags(&pe[0], hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&pe[1], hipEventDisableTiming));
@@ -550,14 +601,19 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
         bool done = false;
         for (long c = 0; c < max_cycles && !done; c++) {
             enqueue_cycle();
-            HIP_TRY(hipMemcpyAsync(&h_flag[c % 2], flag, sizeof(int), hipMemcpyDeviceToHost, stream));
+            int *slot = h_flag + (c % 2) * (nsys + 1);
+            HIP_TRY(hipMemcpyAsync(slot, flag, sizeof(int) * (nsys + 1), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipEventRecord(pe[c % 2], stream));
             if (c >= 1) {
                 HIP_TRY(hipEventSynchronize(pe[(c - 1) % 2]));
-                if (h_flag[(c - 1) % 2]) done = true;
+                const int *prev = h_flag + ((c - 1) % 2) * (nsys + 1);
+                if (prev[0]) done = true;
+                else compact(prev + 1);
             }
         }
-        // final true residuals (idrs.inc:304-310)
+        // final true residuals over every system (idrs.inc:304-310)
+        bv = bv_all;
+        gt = dim3(ntiles);
         op_apply<T>(A, x, tmp, w1, w2, EPI_NONE, none, nullptr, stream, &all);
         hipLaunchKernelGGL((k_bresidual<T>), gt, dim3(BT), 0, stream, bv, b, (const T *)tmp, r, part_d);
         LAUNCH_CHECK();

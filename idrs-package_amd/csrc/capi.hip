@@ -439,6 +439,15 @@ int idrs_user_matrix_destroy(idrs_user_matrix_t A) {
     });
 }
 
+int idrs_user_matrix_autotune(idrs_user_matrix_t K, int reps, int *group) {
+    return guarded([&] {
+        need(K != nullptr, "null handle");
+        need(reps >= 1, "reps must be positive");
+        const int g = autotune_group(*K->M, reps, lib_stream(), nullptr);
+        if (group) *group = g;
+    });
+}
+
 int idrs_user_matrix_info(idrs_user_matrix_t A, int *nrows, int *ncols, int64_t *nnz, int *format) {
     return guarded([&] {
         need(A != nullptr, "null handle");

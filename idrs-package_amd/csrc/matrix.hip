@@ -488,7 +488,7 @@ static void dispatch_epi(const DevMatrix &K, const SpmvArgs<VT, T> &A, const Epi
 // ---------------------------------------------------------------------------
 struct TileArgs {
     const int *tile_sys;
-    const long *tile_r0;
+    const long *tile_r0, *tile_r1;
     const int *info;
     int stride;
 };
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_crs(SpmvArgs<VT, T> A, E
 #pragma unroll
     for (int i = 0; i < (EPI == EPI_PDOT ? PC : NV); i++) e[i] = T(0);
     const int g = threadIdx.x % G, grp = threadIdx.x / G;
-    const long c0 = ta.tile_r0[tile], c1 = ta.tile_r0[tile + 1];
+    const long c0 = ta.tile_r0[tile], c1 = ta.tile_r1[tile];
     const T *__restrict__ x = A.x;
     const VT *__restrict__ val = A.val;
     const int *__restrict__ col = A.col;
@@ -570,9 +570,12 @@ static void launch_bspmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T
                          hipStream_t st) {
     if (K.fmt != DevFmt::CRS) throw std::invalid_argument("batched systems must be stored as CRS");
     const Partition &P = *tc.part;
-    TileArgs ta{P.tile_sys, P.tile_r0, tc.info, tc.stride};
-    E.nparts = P.ntiles;
-    dim3 gd(P.ntiles), bd(SPMV_THREADS);
+    TileArgs ta = tc.tile_sys ? TileArgs{tc.tile_sys, tc.tile_r0, tc.tile_r1, tc.info, tc.stride}
+                              : TileArgs{P.tile_sys, P.tile_r0, P.tile_r0 + 1, tc.info, tc.stride};
+    const int nt = tc.tile_sys ? tc.ntiles : P.ntiles;
+    if (nt < 1) return;
+    E.nparts = nt;
+    dim3 gd(nt), bd(SPMV_THREADS);
     switch (K.group) {
     case 2: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 2>), gd, bd, 0, st, A, E, ta); break;
     case 4: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 4>), gd, bd, 0, st, A, E, ta); break;
@@ -973,6 +976,62 @@ void user_spmv(const DevMatrix &K, const T *v, T *w, hipStream_t st) {
     EpiArgs<T> none{};
     op_spmv<T>(tmp, v, T(1), nullptr, T(0), w, EPI_NONE, none, nullptr, st);
     tmp.K = nullptr;
+}
+
+// Autotuner of the CRS kernel's lanes per row: times the product with the
+// candidate group sizes around the average-row-length choice (reps launches
+// each, HIP events on the library stream) and keeps the fastest.  Explicit,
+// so results stay reproducible: a group size changes the order in which a
+// row's partial sums are combined, i.e. the rounding.
+int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair<int, float>> *times) {
+    if (K.fmt != DevFmt::CRS || K.nrows < 1) return K.group;
+    const bool cplx = K.kind == ValKind::Complex;
+    const size_t vsz = cplx ? sizeof(c128) : sizeof(double);
+    void *x = nullptr, *y = nullptr;
+    HIP_TRY(hipMalloc(&x, vsz * (K.ncols ? K.ncols : 1)));
+    HIP_TRY(hipMalloc(&y, vsz * K.nrows));
+    HIP_TRY(hipMemsetAsync(x, 0, vsz * (K.ncols ? K.ncols : 1), st));
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    const int g0 = K.group;
+    int best = g0;
+    float best_ms = 0.f;
+    try {
+        for (int g : {g0 / 2, g0, g0 * 2, g0 * 4}) {
+            if (g < 2 || g > 64) continue;
+            K.group = g;
+            if (cplx) user_spmv<c128>(K, (const c128 *)x, (c128 *)y, st);   // warm-up
+            else user_spmv<double>(K, (const double *)x, (double *)y, st);
+            HIP_TRY(hipEventRecord(a, st));
+            for (int r = 0; r < reps; r++) {
+                if (cplx) user_spmv<c128>(K, (const c128 *)x, (c128 *)y, st);
+                else user_spmv<double>(K, (const double *)x, (double *)y, st);
+            }
+            HIP_TRY(hipEventRecord(b, st));
+            HIP_TRY(hipEventSynchronize(b));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, a, b));
+            if (times) times->push_back({g, ms / reps});
+            if (best_ms == 0.f || ms < best_ms) {
+                best_ms = ms;
+                best = g;
+            }
+        }
+    } catch (...) {
+        K.group = g0;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        (void)hipFree(x);
+        (void)hipFree(y);
+        throw;
+    }
+    K.group = best;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(x);
+    (void)hipFree(y);
+    return best;
 }
 
 // ---------------------------------------------------------------------------

@@ -42,6 +42,11 @@ struct TileCtx {
     const Partition *part = nullptr;
     const int *info = nullptr;
     int stride = 0;
+    // optional subset of the partition's tiles (e.g. those of unfinished
+    // systems): system, first row and end row of each of ntiles tiles
+    int ntiles = 0;
+    const int *tile_sys = nullptr;
+    const long *tile_r0 = nullptr, *tile_r1 = nullptr;
 };
 
 struct DevMatrix {
@@ -157,6 +162,9 @@ DevMatrix *make_coo(int nrows, int ncols, long nnz, const int *indx1, const int 
                     ValKind kind);
 DevMatrix *make_cds(int nrows, int ncols, int half_band, const void *band, ValKind kind);
 DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const void *vals, ValKind kind);
+// times the CRS product for lanes-per-row choices around K.group and keeps
+// the fastest (returns it; times: (group, ms per product) if not null)
+int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair<int, float>> *times);
 // block-diagonal CRS matrix of nsys square CRS systems (copies), with its
 // Partition
 DevMatrix *make_blockdiag(int nsys, const DevMatrix *const *Ks);

@@ -68,6 +68,13 @@ class UserMatrix:
                 pass
             self._h = None
 
+    def autotune(self, reps=10):
+        """Pick the CRS kernel's lanes per row by timing (idrs_user_matrix_autotune);
+        returns the chosen group size."""
+        g = ctypes.c_int()
+        check(_lib.load().idrs_user_matrix_autotune(self._h, int(reps), ctypes.byref(g)))
+        return g.value
+
     def __mul__(self, v):
         """w = K*v (ruser_mv / cuser_mv, user_module.f90:197-241)."""
         v = np.asarray(v)

@@ -77,6 +77,12 @@ int idrs_dia_format_c128(int nrows, int ncols, int ndiag, const int *offsets, co
 /* Frees the device storage of a user matrix (the reference's allocatable
    components are released when the Fortran object goes out of scope). */
 int idrs_user_matrix_destroy(idrs_user_matrix_t A);
+/* Autotuner (no counterpart in the reference): times the CRS product with
+   the lanes-per-row choices around the default (reps products each) and
+   keeps the fastest for all later products of K; *group receives it.  A
+   different choice changes the rounding order inside a row, so the call is
+   explicit.  DIA storage: no-op. */
+int idrs_user_matrix_autotune(idrs_user_matrix_t K, int reps, int *group);
 /* nrows, ncols, stored entries, matrix_format (2 CRS, 3 COO, 4 CDS, 5 DIA) */
 int idrs_user_matrix_info(idrs_user_matrix_t A, int *nrows, int *ncols, int64_t *nnz, int *format);
 

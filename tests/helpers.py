@@ -35,3 +35,22 @@ def oracle_crs(crs, diag_precon=False, poly=None, complex_vectors=False):
         alpha = orc.chebyshev(degree, np.asarray(foci))
     return orc.OracleMatrix(orc.CRS, crs.n, crs=crs, which_precon=1 if diag_precon else 0, degree=degree,
                             alpha=alpha, complex_vectors=complex_vectors or np.iscomplexobj(crs.val))
+
+
+def rounding_spread(run, b, trials=4, seed=0):
+    """How far rounding alone moves a solver on this problem: run(b') for
+    b' = b (1 + 1e-16 g), g ~ N(0,1), against run(b).  run returns (x,
+    iterations); returns (max relative change of x, max |change of the
+    iteration count|).  Used where a recurrence amplifies rounding so much
+    that a fixed bound between two implementations would be meaningless."""
+    rng = np.random.default_rng(seed)
+    x0, it0 = run(b)
+    dx, dit = 0.0, 0
+    for _ in range(trials):
+        g = rng.standard_normal(b.shape)
+        if np.iscomplexobj(b):
+            g = g + 1j * rng.standard_normal(b.shape)
+        x1, it1 = run(b * (1 + 1e-16 * g))
+        dx = max(dx, rel(x1, x0))
+        dit = max(dit, abs(it1 - it0))
+    return dx, dit

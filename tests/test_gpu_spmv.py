@@ -170,3 +170,16 @@ def test_make_p_orthonormal(gpu):
     assert rel(PTg, PTo) < 1e-13
     r = raw[:, 0] + 1j * raw[:, 1]
     assert np.array_equal(idrs_amd.make_p(n, 1, 2, r=r, complex=True)[:, 0], np.conj(r))
+
+
+def test_autotune_keeps_results(gpu):
+    """The autotuned lanes-per-row choice changes only the in-row combination
+    order: products stay within the rounding bound."""
+    crs = P.random_dominant(20000, k=12, seed=3)
+    K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    g = K.autotune(reps=3)
+    assert g in (4, 8, 16, 32, 64)
+    x = np.random.default_rng(4).standard_normal(crs.n)
+    w = K * x
+    ref = oracle_crs(crs).user_mv(x)
+    assert np.all(np.abs(w - ref) <= _bound(crs, x))
