@@ -276,7 +276,7 @@ static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opt
 
 template <class T>
 static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double inner_tol,
-                       int max_outer, double *x, idrs_stats_t *stats, int *outer_steps) {
+                       int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats, int *outer_steps) {
     return guarded([&] {
         need(A && b && x, "null argument");
         need(s >= 1 && s <= SMAX, "s must be in 1..16");
@@ -297,7 +297,7 @@ static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idr
         if (o.device_pointers) {
             so.x0 = (const T *)o.x0;
             so.PT_raw = (const T *)o.PT_raw;
-            mixed_solve<T>(A->op, n, s, (const T *)b, (T *)x, so, inner_tol, max_outer, ms, st);
+            mixed_solve<T>(A->op, n, s, (const T *)b, (T *)x, so, inner_tol, max_outer, fp32_vectors != 0, ms, st);
         } else {
             DVec<T> db((const T *)b, n, st), dx(n), dx0, dP;
             if (o.x0) {
@@ -308,7 +308,7 @@ static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idr
                 dP = DVec<T>((const T *)o.PT_raw, (size_t)n * prow, st);
                 so.PT_raw = dP.p;
             }
-            mixed_solve<T>(A->op, n, s, db.p, dx.p, so, inner_tol, max_outer, ms, st);
+            mixed_solve<T>(A->op, n, s, db.p, dx.p, so, inner_tol, max_outer, fp32_vectors != 0, ms, st);
             dx.download((T *)x, st);
         }
         if (stats) {
@@ -336,6 +336,26 @@ static int matvec_entry(idrs_matrix_t A, const double *v, double *w, int what) {
         else if (what == 1) op_precon<T>(A->op, dv.p, dw.p, st);
         else op_scaleback<T>(A->op, dv.p, dw.p, w1.p, w2.p, w3.p, st);
         dw.download((T *)w, st);
+    });
+}
+
+template <class T>
+static int ms_scaleback_entry(idrs_matrix_t A, int ns, int degree, const double *alpha_s, const double *Y, double *X) {
+    return guarded([&] {
+        need(A && alpha_s && Y && X, "null argument");
+        need(ns >= 1 && degree >= 0, "bad arguments");
+        hipStream_t st = lib_stream();
+        const long n = A->op.K->nrows;
+        DVec<T> dy(n), dx(n), w1(n), w2(n);
+        for (int k = 0; k < ns; k++) {
+            std::vector<std::complex<double>> a(degree + 1);
+            for (int i = 0; i <= degree; i++)
+                a[i] = {alpha_s[2 * ((long)k * (degree + 1) + i)], alpha_s[2 * ((long)k * (degree + 1) + i) + 1]};
+            HIP_TRY(hipMemcpyAsync(dy.p, (const T *)Y + (long)k * n, sizeof(T) * n, hipMemcpyHostToDevice, st));
+            op_scaleback_with<T>(A->op, a, dy.p, dx.p, w1.p, w2.p, st);
+            HIP_TRY(hipMemcpyAsync((T *)X + (long)k * n, dx.p, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+        }
+        HIP_TRY(hipStreamSynchronize(st));
     });
 }
 
@@ -631,6 +651,15 @@ int idrs_precon_c128(idrs_matrix_t A, const double *v, double *w) { return matve
 int idrs_scaleback_f64(idrs_matrix_t A, const double *y, double *x) { return matvec_entry<double>(A, y, x, 2); }
 int idrs_scaleback_c128(idrs_matrix_t A, const double *y, double *x) { return matvec_entry<c128>(A, y, x, 2); }
 
+int idrs_ms_scaleback_f64(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
+                          double *X) {
+    return ms_scaleback_entry<double>(A, n_sigma, degree, alpha_s, Y, X);
+}
+int idrs_ms_scaleback_c128(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
+                           double *X) {
+    return ms_scaleback_entry<c128>(A, n_sigma, degree, alpha_s, Y, X);
+}
+
 int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                   idrs_stats_t *stats) {
     return idrs_entry<double>(A, n, b, s, opts, x, stats);
@@ -666,12 +695,14 @@ int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, con
 }
 
 int idrs_idrs_mixed_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
-                        double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps) {
-    return mixed_entry<double>(A, n, b, s, opts, inner_tolerance, max_outer, x, stats, outer_steps);
+                        double inner_tolerance, int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats,
+                        int *outer_steps) {
+    return mixed_entry<double>(A, n, b, s, opts, inner_tolerance, max_outer, fp32_vectors, x, stats, outer_steps);
 }
 int idrs_idrs_mixed_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
-                         double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps) {
-    return mixed_entry<c128>(A, n, b, s, opts, inner_tolerance, max_outer, x, stats, outer_steps);
+                         double inner_tolerance, int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats,
+                         int *outer_steps) {
+    return mixed_entry<c128>(A, n, b, s, opts, inner_tolerance, max_outer, fp32_vectors, x, stats, outer_steps);
 }
 
 int idrs_crs_transpose(idrs_user_matrix_t K, idrs_user_matrix_t *KT) {

@@ -118,6 +118,9 @@ template <class R> struct Tr<cplx<R>> {
 
 // real matrix value times vector entry (real x complex scales both parts)
 __host__ __device__ inline double mulv(double a, double x) { return a * x; }
+// single-precision matrix values with double-precision vectors
+__host__ __device__ inline double mulv(float a, double x) { return (double)a * x; }
+__host__ __device__ inline double fmav(float a, double x, double c) { return ::fma((double)a, x, c); }
 __host__ __device__ inline float mulv(float a, float x) { return a * x; }
 template <class R>
 __host__ __device__ inline cplx<R> mulv(R a, cplx<R> x) { return {a * x.re, a * x.im}; }

@@ -649,6 +649,21 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
         } else {
             throw std::invalid_argument("complex matrix values need complex128 vectors");
         }
+    } else if (op.values32 && !single) {
+        if constexpr (std::is_same<T, double>::value) {
+            const float *v32 = (const float *)(op.sval ? op.sval32 : K.val32);
+            if (!v32) throw std::invalid_argument("single-precision values not prepared");
+            if (has_seed) throw std::invalid_argument("seed shift with single-precision values is not supported");
+            auto A = base_args<float, T>(K, v32);
+            A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
+            A.has_seed = 0;
+            A.sdiag = nullptr;
+            A.seed = T(0);
+            if (tc) dispatch_epi_tiles<float, T>(K, A, E, epi, *tc, st);
+            else dispatch_epi<float, T>(K, A, E, epi, info, st);
+        } else {
+            throw std::invalid_argument("single-precision values with double vectors: real systems only");
+        }
     } else {
         const R *vals = single ? (const R *)(op.sval ? op.sval32 : K.val32) : (const R *)op.values();
         if (!vals) throw std::invalid_argument("single-precision values not prepared");
@@ -948,8 +963,9 @@ void op_precon(const Operator &op, const T *v, T *w, hipStream_t st) {
 }
 
 template <class T>
-void op_scaleback(const Operator &op, const T *y, T *x, T *w1, T *w2, T *w3, hipStream_t st) {
-    const int d = op.degree;
+void op_scaleback_with(const Operator &op, const std::vector<std::complex<double>> &alpha, const T *y, T *x, T *w1,
+                       T *w2, hipStream_t st) {
+    const int d = (int)alpha.size() - 1;
     long n = op.K->nrows;
     EpiArgs<T> none{};
     const T *p = y;
@@ -957,16 +973,24 @@ void op_scaleback(const Operator &op, const T *y, T *x, T *w1, T *w2, T *w3, hip
         // polpre: w = alpha_d y; for i = d-1..0: w = K'(w) + alpha_i y
         int g = vec_grid(n);
         T *cur = w1, *nxt = w2;
-        hipLaunchKernelGGL((k_scale_copy<T>), dim3(g), dim3(256), 0, st, n, coef<T>(op, d), y, cur);
+        hipLaunchKernelGGL((k_scale_copy<T>), dim3(g), dim3(256), 0, st, n, to_T<T>(alpha[d]), y, cur);
         LAUNCH_CHECK();
         for (int i = d - 1; i >= 0; i--) {
-            op_spmv<T>(op, cur, T(1), y, coef<T>(op, i), nxt, EPI_NONE, none, nullptr, st);
+            op_spmv<T>(op, cur, T(1), y, to_T<T>(alpha[i]), nxt, EPI_NONE, none, nullptr, st);
             std::swap(cur, nxt);
         }
         p = cur;
     }
-    (void)w3;
     op_precon<T>(op, p, x, st);
+}
+
+template <class T>
+void op_scaleback(const Operator &op, const T *y, T *x, T *w1, T *w2, T *w3, hipStream_t st) {
+    (void)w3;
+    if (op.degree > 0)
+        op_scaleback_with<T>(op, std::vector<std::complex<double>>(op.alpha.begin(), op.alpha.begin() + op.degree + 1),
+                             y, x, w1, w2, st);
+    else op_scaleback_with<T>(op, {std::complex<double>(1.0, 0.0)}, y, x, w1, w2, st);
 }
 
 template <class T>
@@ -1309,6 +1333,8 @@ void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> 
                               const int *, hipStream_t, const TileCtx *);                                    \
     template void op_precon<T>(const Operator &, const T *, T *, hipStream_t);                               \
     template void op_scaleback<T>(const Operator &, const T *, T *, T *, T *, T *, hipStream_t);             \
+    template void op_scaleback_with<T>(const Operator &, const std::vector<std::complex<double>> &, const T *, T *,  \
+                                       T *, T *, hipStream_t);                                                 \
     template void user_spmv<T>(const DevMatrix &, const T *, T *, hipStream_t);
 INST(double)
 INST(c128)

@@ -93,6 +93,9 @@ struct Operator {
     std::vector<std::complex<double>> alpha;   // alpha_p(0:degree)
     std::complex<double> seed{0.0, 0.0};       // A%rseed / A%cseed
     bool complex_poly = false;                 // coefficients set by complex_preconditioner
+    // products of a double-vector solve read the FP32 copy of the (scaled)
+    // values (set by the mixed-precision driver around its inner solves)
+    bool values32 = false;
     ~Operator();
     const void *values() const { return sval ? sval : K->val; }
     bool values_complex() const {
@@ -146,6 +149,12 @@ void op_precon(const Operator &op, const T *v, T *w, hipStream_t st);
 // x = scaleback(op, y) = polpre(y) / D
 template <class T>
 void op_scaleback(const Operator &op, const T *y, T *x, T *w1, T *w2, T *w3, hipStream_t st);
+
+// x = p(K') y / D with the given coefficients alpha(0:degree) in place of
+// the operator's own (the per-shift scale-back of the multishift solvers)
+template <class T>
+void op_scaleback_with(const Operator &op, const std::vector<std::complex<double>> &alpha, const T *y, T *x, T *w1,
+                       T *w2, hipStream_t st);
 
 // plain user-matrix product w = K v (no scaling, no polynomial)
 template <class T>

@@ -2,8 +2,8 @@
 //
 // Each outer step: r = b - A x in FP64 (the true residual, A applied with
 // FP64 values), stop if ||r|| <= tol ||b||; else solve A d = r / ||r|| by
-// IDR(s) in single precision -- every SpMV reads the FP32 copy of the
-// (scaled) values and FP32 vectors, every vector kernel moves FP32 -- to a
+// IDR(s) whose SpMVs read the FP32 copy of the (scaled) values -- with FP32
+// vectors and scalars as well (fp32_vectors), or in FP64 otherwise -- to a
 // relative tolerance inner_tol (no tighter than the remaining FP64 gap
 // needs), and update x += ||r|| d in FP64.  The inner solve is the same
 // device IDR(s) as the FP64 one (idrs.inc:1-311), instantiated for float /
@@ -81,9 +81,34 @@ inline double real_part(c128 v) { return v.re; }
 
 }  // namespace
 
+// d64 = c * r64 (the FP64-vector inner solve's right-hand side)
+template <class T>
+__global__ void k_scale(long n, const T *__restrict__ r, double c, T *__restrict__ d) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        if constexpr (Tr<T>::is_complex) d[q] = T(r[q].re * c, r[q].im * c);
+        else d[q] = r[q] * c;
+    }
+}
+
+// x = x + c * d (FP64 correction)
+template <class T>
+__global__ void k_add(long n, T *__restrict__ x, double c, const T *__restrict__ d) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        if constexpr (Tr<T>::is_complex) x[q] = T(fma(c, d[q].re, x[q].re), fma(c, d[q].im, x[q].im));
+        else x[q] = fma(c, d[q], x[q]);
+    }
+}
+
+// the operator reads its FP32 values while in scope
+struct Values32 {
+    Operator &op;
+    explicit Values32(Operator &o) : op(o) { op.values32 = true; }
+    ~Values32() { op.values32 = false; }
+};
+
 template <class T64>
 void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const SolveOpts<T64> &o, double inner_tol,
-                 int max_outer, MixedStats &st, hipStream_t stream) {
+                 int max_outer, bool fp32_vectors, MixedStats &st, hipStream_t stream) {
     using T32 = typename Lo<T64>::type;
     if (n < 1) throw std::invalid_argument("empty system");
     if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
@@ -98,7 +123,14 @@ void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const S
 
     const size_t v64 = sizeof(T64) * (size_t)n, v32 = sizeof(T32) * (size_t)n;
     const int prow = o.variant == 1 ? s : 1;
+    if (!fp32_vectors && !std::is_same<T64, double>::value)
+        throw std::invalid_argument("FP32 values with FP64 vectors: real systems only");
     Dev br(v64), bt(v64), bw1(v64), bw2(v64), br32(v32), bd32(v32), bpart(sizeof(T64) * MAX_PARTS), bsum(sizeof(T64));
+    std::unique_ptr<Dev> brh, bdh;
+    if (!fp32_vectors) {
+        brh.reset(new Dev(v64));
+        bdh.reset(new Dev(v64));
+    }
     std::unique_ptr<Dev> bP32;
     T64 *r = br.as<T64>(), *t = bt.as<T64>(), *w1 = bw1.as<T64>(), *w2 = bw2.as<T64>(), *part = bpart.as<T64>();
     T32 *r32 = br32.as<T32>(), *d32 = bd32.as<T32>();
@@ -143,22 +175,42 @@ void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const S
             info = 1;
             break;
         }
-        hipLaunchKernelGGL((k_to_lo<T64, T32>), dim3(gv), dim3(256), 0, stream, n, (const T64 *)r, 1.0 / normr, r32);
-        LAUNCH_CHECK();
-        SolveOpts<T32> io;
         // no tighter than the FP64 target needs (half of it, for the FP32 error)
-        io.tol = std::max(inner_tol, 0.5 * tolb / normr);
-        io.has_maxit = true;
-        io.maxit = maxit - its;
-        io.variant = o.variant;
-        io.PT_raw = bP32 ? bP32->as<T32>() : nullptr;
-        io.p_seed = o.p_seed;
+        const double itol = std::max(inner_tol, 0.5 * tolb / normr);
         SolveStats is;
-        idrs_solve<T32>(A, n, s, r32, d32, io, is, stream);
+        if (fp32_vectors) {
+            hipLaunchKernelGGL((k_to_lo<T64, T32>), dim3(gv), dim3(256), 0, stream, n, (const T64 *)r, 1.0 / normr, r32);
+            LAUNCH_CHECK();
+            SolveOpts<T32> io;
+            io.tol = itol;
+            io.has_maxit = true;
+            io.maxit = maxit - its;
+            io.variant = o.variant;
+            io.PT_raw = bP32 ? bP32->as<T32>() : nullptr;
+            io.p_seed = o.p_seed;
+            idrs_solve<T32>(A, n, s, r32, d32, io, is, stream);
+            hipLaunchKernelGGL((k_add_hi<T64, T32>), dim3(gv), dim3(256), 0, stream, n, x, normr, (const T32 *)d32);
+            LAUNCH_CHECK();
+        } else {
+            T64 *rh = brh->as<T64>(), *dh = bdh->as<T64>();
+            hipLaunchKernelGGL((k_scale<T64>), dim3(gv), dim3(256), 0, stream, n, (const T64 *)r, 1.0 / normr, rh);
+            LAUNCH_CHECK();
+            SolveOpts<T64> io;
+            io.tol = itol;
+            io.has_maxit = true;
+            io.maxit = maxit - its;
+            io.variant = o.variant;
+            io.PT_raw = o.PT_raw;
+            io.p_seed = o.p_seed;
+            {
+                Values32 v32(const_cast<Operator &>(A));
+                idrs_solve<T64>(A, n, s, rh, dh, io, is, stream);
+            }
+            hipLaunchKernelGGL((k_add<T64>), dim3(gv), dim3(256), 0, stream, n, x, normr, (const T64 *)dh);
+            LAUNCH_CHECK();
+        }
         its += is.iterations;
         outer++;
-        hipLaunchKernelGGL((k_add_hi<T64, T32>), dim3(gv), dim3(256), 0, stream, n, x, normr, (const T32 *)d32);
-        LAUNCH_CHECK();
         op_apply<T64>(A, x, t, w1, w2, EPI_NONE, none, nullptr, stream);
         const double prev = normr;
         normr = norm_of(b, t, r);
@@ -181,8 +233,8 @@ void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const S
 }
 
 template void mixed_solve<double>(const Operator &, long, int, const double *, double *, const SolveOpts<double> &,
-                                  double, int, MixedStats &, hipStream_t);
+                                  double, int, bool, MixedStats &, hipStream_t);
 template void mixed_solve<c128>(const Operator &, long, int, const c128 *, c128 *, const SolveOpts<c128> &, double,
-                                int, MixedStats &, hipStream_t);
+                                int, bool, MixedStats &, hipStream_t);
 
 }  // namespace idrs

@@ -15,11 +15,13 @@ struct MixedStats {
 };
 
 // x = A^-1 b with r = b - A x recomputed in FP64 after every inner solve
-// A32 d = r/||r|| to inner_tol (FP32 SpMV on FP32 copies of the values);
+// A32 d = r/||r|| to inner_tol (SpMV on FP32 copies of the values);
 // o.tol is the FP64 tolerance, o.maxit bounds the inner iterations in total.
-// T64 = double or c128 (then the matrix values must be real).
+// fp32_vectors: the inner IDR(s) keeps its vectors and scalars in FP32 too
+// (T64 = double or c128, real matrix values); otherwise only the matrix
+// values are FP32 and the inner solve runs in FP64 (T64 = double).
 template <class T64>
 void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const SolveOpts<T64> &o, double inner_tol,
-                 int max_outer, MixedStats &st, hipStream_t stream);
+                 int max_outer, bool fp32_vectors, MixedStats &st, hipStream_t stream);
 
 }  // namespace idrs

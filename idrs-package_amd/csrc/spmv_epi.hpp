@@ -11,6 +11,7 @@ namespace idrs {
 // elementwise helpers for the value/vector type combinations
 // ---------------------------------------------------------------------------
 __device__ inline double divv(double x, double d) { return x / d; }
+__device__ inline double divv(double x, float d) { return x / (double)d; }
 __device__ inline float divv(float x, float d) { return x / d; }
 template <class R>
 __device__ inline cplx<R> divv(cplx<R> x, R d) { return {x.re / d, x.im / d}; }

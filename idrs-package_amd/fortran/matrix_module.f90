@@ -12,6 +12,8 @@
 !   real_preconditioner / complex_preconditioner  :551-722
 !   CHEBYSHEV / NEUMANN / COMPUTE_ALPHA      matrix_module.f90:851-1057
 !   SCALEBACK     rscaleback / cscaleback    matrix_module.f90:808-849
+!                 ms_rscaleback / ms_cscaleback               :724-806
+!   MS_PARAMETERS real_ / complex_ms_parameters               :402-549
 !   Pmul          identity without deflation matrix_module.f90:1384-1431
 !
 ! Not offered (stop with a message): deflation, mass/damping matrices,
@@ -68,7 +70,7 @@ module matrix_module
    end interface
 
    interface SCALEBACK
-      module procedure CSCALEBACK, RSCALEBACK
+      module procedure CSCALEBACK, RSCALEBACK, MS_CSCALEBACK, MS_RSCALEBACK
    end interface
 
    interface COMPUTE_ALPHA
@@ -391,6 +393,54 @@ contains
       complex(kind=cp)                       :: x(size(y))
       call idrs_check( idrs_scaleback_c128(matrix_handle(A, .true.), y, x), 'scaleback' )
    end function cscaleback
+
+! Scale back the multishift solutions (matrix_module.f90:724-806, without
+! deflation and shift-and-invert): column k of x is p_k(K') y(:,k) / D with
+! the shifted polynomial alpha_s(:,k) from MS_PARAMETERS (alpha = 1 without
+! a polynomial preconditioner).
+   function ms_rscaleback( A, sigma, b, y ) result(x)
+      type(matrix), intent(in)               :: A
+      real(kind=rp), intent(in)              :: y(:,:)
+      real(kind=rp), intent(in)              :: b(:)
+      real(kind=rp), intent(in)              :: sigma(:)
+      real(kind=rp)                          :: x(size(y,1),size(y,2))
+      complex(kind=cp), allocatable          :: coef(:,:)
+      integer                                :: deg
+      if ( A%shift_invert ) error stop 'SCALEBACK: shift-and-invert is not offered by the GPU library'
+      deg = 0
+      if ( A%degree > 0 ) then
+         if ( .not. allocated(A%ralpha_s) ) error stop 'First call subroutine MS_PARAMETERS'
+         deg = A%degree
+         allocate( coef(0:deg, size(sigma)) )
+         coef = cmplx( A%ralpha_s(0:deg, 1:size(sigma)), 0._rp, kind=cp )
+      else
+         allocate( coef(0:0, size(sigma)) )
+         coef = (1._cp, 0._cp)
+      end if
+      call idrs_check( idrs_ms_scaleback_f64(matrix_handle(A, .false.), size(sigma), deg, coef, y, x), 'scaleback' )
+   end function ms_rscaleback
+
+   function ms_cscaleback( A, sigma, b, y ) result(x)
+      type(matrix), intent(in)               :: A
+      complex(kind=cp), intent(in)           :: y(:,:)
+      complex(kind=cp), intent(in)           :: b(:)
+      complex(kind=cp), intent(in)           :: sigma(:)
+      complex(kind=cp)                       :: x(size(y,1),size(y,2))
+      complex(kind=cp), allocatable          :: coef(:,:)
+      integer                                :: deg
+      if ( A%shift_invert ) error stop 'SCALEBACK: shift-and-invert is not offered by the GPU library'
+      deg = 0
+      if ( A%degree > 0 ) then
+         if ( .not. allocated(A%calpha_s) ) error stop 'First call subroutine MS_PARAMETERS'
+         deg = A%degree
+         allocate( coef(0:deg, size(sigma)) )
+         coef = A%calpha_s(0:deg, 1:size(sigma))
+      else
+         allocate( coef(0:0, size(sigma)) )
+         coef = (1._cp, 0._cp)
+      end if
+      call idrs_check( idrs_ms_scaleback_c128(matrix_handle(A, .true.), size(sigma), deg, coef, y, x), 'scaleback' )
+   end function ms_cscaleback
 
 ! Shifts of the preconditioned system and the scale-back polynomials
 ! (matrix_module.f90:402-549, without shift-and-invert): host arithmetic on

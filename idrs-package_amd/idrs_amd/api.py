@@ -594,10 +594,11 @@ def idrs_batched(A, b, s, tolerance=None, maximum_iterations=None, variant=None,
 
 
 def idrs_mixed(A, b, s, tolerance=None, maximum_iterations=None, variant=None, x0=None, PT_raw=None, p_seed=None,
-               inner_tolerance=1e-5, max_outer=10, out=None):
-    """x = A^-1 b to the FP64 tolerance with single-precision inner IDR(s)
-    solves and FP64 residual replacement (idrs_idrs_mixed_*).  b: numpy
-    array or CUDA tensor as for idrs()."""
+               inner_tolerance=1e-5, max_outer=10, fp32_vectors=None, out=None):
+    """x = A^-1 b to the FP64 tolerance with inner IDR(s) solves on FP32
+    matrix values and FP64 residual replacement (idrs_idrs_mixed_*).
+    fp32_vectors: inner vectors in FP32 too (default: only for complex b,
+    where it is required).  b: numpy array or CUDA tensor as for idrs()."""
     lib = _lib.load()
     device = _is_torch(b)
     n = int(b.shape[0])
@@ -639,7 +640,9 @@ def idrs_mixed(A, b, s, tolerance=None, maximum_iterations=None, variant=None, x
     o.p_seed = int(p_seed) if p_seed is not None else 0
     st = idrs_stats_t()
     outer = ctypes.c_int()
+    if fp32_vectors is None:
+        fp32_vectors = bool(cplx)
     f = lib.idrs_idrs_mixed_c128 if cplx else lib.idrs_idrs_mixed_f64
-    check(f(A.handle, n, bp, int(s), ctypes.byref(o), float(inner_tolerance), int(max_outer), xp, ctypes.byref(st),
-            ctypes.byref(outer)))
+    check(f(A.handle, n, bp, int(s), ctypes.byref(o), float(inner_tolerance), int(max_outer), int(bool(fp32_vectors)),
+            xp, ctypes.byref(st), ctypes.byref(outer)))
     return MixedResult(x, st.flag, st.relres, st.iterations, outer.value, st.solve_ms)

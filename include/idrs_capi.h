@@ -148,6 +148,15 @@ int idrs_precon_c128(idrs_matrix_t A, const double *v, double *w);
 int idrs_scaleback_f64(idrs_matrix_t A, const double *y, double *x);
 /* x = scaleback(A, b, y): cscaleback, matrix_module.f90:808-829 */
 int idrs_scaleback_c128(idrs_matrix_t A, const double *y, double *x);
+/* X(:,k) = ms_scaleback(A, sigma, b, Y)(:,k) for the shifted polynomials:
+   X(:,k) = p_k(K') Y(:,k) / D with alpha_s(0:degree, k) as (re, im) pairs,
+   column-major (degree+1) x n_sigma (MS_PARAMETERS output);
+   ms_rscaleback / ms_cscaleback, matrix_module.f90:724-806 (no deflation,
+   no shift-and-invert, as elsewhere in this library) */
+int idrs_ms_scaleback_f64(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
+                          double *X);
+int idrs_ms_scaleback_c128(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
+                           double *X);
 
 /* ---------------------------------------------------------------------
  * solvers
@@ -258,19 +267,24 @@ int idrs_msqmridr_c128(idrs_matrix_t A, int n, const double *b, int n_sigma, con
  * ------------------------------------------------------------------- */
 
 /* x = IDRS(A, b, s, ...) to the FP64 tolerance opts->tolerance with
-   single-precision inner solves: r = b - A x in FP64, then IDR(s) with FP32
-   values and vectors on A d = r/||r|| to inner_tolerance, x += ||r|| d,
-   repeated (at most max_outer times) until the true FP64 residual meets the
-   tolerance.  opts->maximum_iterations bounds the inner iterations in total;
-   x0, PT_raw, variant and device_pointers as for idrs_idrs_f64 (M1, U0,
-   omega, resvec rejected).  Matrix values must be real and the operator
-   without a seed shift.  stats->iterations: inner iterations; *outer_steps:
-   FP64 residual replacements. */
+   single-precision inner solves: r = b - A x in FP64, then IDR(s) on
+   A d = r/||r|| to inner_tolerance whose SpMVs read FP32 copies of the
+   (scaled) matrix values -- with FP32 vectors too if fp32_vectors != 0,
+   FP64 vectors otherwise -- then x += ||r|| d, repeated (at most max_outer
+   times) until the true FP64 residual meets the tolerance.
+   opts->maximum_iterations bounds the inner iterations in total; x0,
+   PT_raw, variant and device_pointers as for idrs_idrs_f64 (M1, U0, omega,
+   resvec rejected).  Matrix values must be real and the operator without a
+   seed shift.  stats->iterations: inner iterations; *outer_steps: FP64
+   residual replacements. */
 int idrs_idrs_mixed_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
-                        double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps);
-/* complex128 vectors (complex64 inner solves) on a real-valued operator */
+                        double inner_tolerance, int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats,
+                        int *outer_steps);
+/* complex128 vectors on a real-valued operator; fp32_vectors must be set
+   (complex64 inner solves) */
 int idrs_idrs_mixed_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts,
-                         double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats, int *outer_steps);
+                         double inner_tolerance, int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats,
+                         int *outer_steps);
 
 /* ---------------------------------------------------------------------
  * differentiable solves (no counterpart in the reference): the adjoint

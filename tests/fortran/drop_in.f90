@@ -16,6 +16,7 @@ program drop_in
    call cdr_preconditioned()
    call toeplitz_complex()
    call krylov_family()
+   call ms_pp_sequence()
 
 contains
 
@@ -187,5 +188,56 @@ contains
             ' relres', rr(i), ' err', norm( b - (A%K*X2(:,i) - sig(i)*X2(:,i)) )/norm( b )
       end do
    end subroutine krylov_family
+
+! The pp_idrs multishift sequence (PP_IDRS/run_ms_idrs.inc): Chebyshev
+! polynomial preconditioner, MS_PARAMETERS, MSIDRS on the preconditioned
+! shifted systems, SCALEBACK per shift; err is the true relative residual of
+! (K - sigma I) x = b for each shift (h^2-scaled 7-point Laplacian, m = 12).
+   subroutine ms_pp_sequence()
+      integer, parameter            :: m = 12, n = m*m*m, ns = 3
+      type(matrix)                  :: A
+      real(kind=rp)                 :: b(n), X(n,ns), sig(ns), sig_p(ns), rr(ns), foci(2), cw(7)
+      integer                       :: row_ptr(n+1), col_ind(7*n), off(7,3), flag, iter, i, j, l, q, nz
+      real(kind=rp)                 :: val(7*n)
+      character(len=24)             :: name
+
+      cw = [ -1._rp, -1._rp, -1._rp, 6._rp, -1._rp, -1._rp, -1._rp ]
+      off(:,1) = [0, 0, -1, 0, 1, 0, 0]
+      off(:,2) = [0, -1, 0, 0, 0, 1, 0]
+      off(:,3) = [-1, 0, 0, 0, 0, 0, 1]
+      nz = 0
+      row_ptr(1) = 1
+      do l = 0, m-1
+         do j = 0, m-1
+            do i = 0, m-1
+               do q = 1, 7
+                  if ( i+off(q,1) < 0 .or. i+off(q,1) >= m ) cycle
+                  if ( j+off(q,2) < 0 .or. j+off(q,2) >= m ) cycle
+                  if ( l+off(q,3) < 0 .or. l+off(q,3) >= m ) cycle
+                  nz = nz + 1
+                  col_ind(nz) = 1 + (i+off(q,1)) + m*(j+off(q,2)) + m*m*(l+off(q,3))
+                  val(nz) = cw(q)
+               end do
+               row_ptr(1 + i + m*j + m*m*l + 1) = nz + 1
+            end do
+         end do
+      end do
+      A%K = crs_format( n, n, row_ptr, col_ind(1:nz), val(1:nz) )
+      do i = 1, n
+         b(i) = real(mod(i, 7), rp) - 3._rp
+      end do
+      foci = [0.05_rp, 11.95_rp]
+      call real_preconditioner( A, 2, 4, foci )
+      sig = [0._rp, -1._rp, -3._rp]
+      call ms_parameters( A, sig, sig_p )
+      X = msidrs( A, Pmul(A,b), sig_p, 4, tolerance=1.e-9_rp, maximum_iterations=2000, flag=flag, relres=rr, &
+                  iterations=iter )
+      X = scaleback( A, sig, b, X )
+      do i = 1, ns
+         write(name,'(a,i0)') 'ms_pp_shift', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', rr(i), ' err', norm( b - A%K*X(:,i) + sig(i)*X(:,i) )/norm( b )
+      end do
+   end subroutine ms_pp_sequence
 
 end program drop_in

@@ -82,3 +82,12 @@ def test_multishift(cases, solver):
         c = cases[f"{solver}_shift{k + 1}"]
         assert c["flag"] == ref["flag"] == 0 and abs(c["iter"] - ref["iterations"]) <= 2
         assert c["err"] < 1e-7 and c["relres"] < 1e-7
+
+
+def test_multishift_pp_sequence(cases):
+    """ms_parameters -> msidrs on the preconditioned shifted systems ->
+    scaleback (PP_IDRS/run_ms_idrs.inc): each shift's true residual of
+    (K - sigma I) x = b, recomputed from the scaled-back solution."""
+    for k in range(3):
+        c = cases[f"ms_pp_shift{k + 1}"]
+        assert c["flag"] == 0 and c["err"] < 1e-7
