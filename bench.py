@@ -295,18 +295,33 @@ def bench_multishift(args):
         bh[(m // 2) * (1 + m + m * m)] = 1.0
         workload = (f"{m}^3 h^2-scaled 7-point Laplacian, sigma_k = (2 pi f_k h)^2 (1 - 0.05 i), ppw 4..30, "
                     "b = centre unit vector, tol 1e-8, complex128")
-    else:
+    elif args.ms_problem == "cdr":
         crs, _ = P.convection_diffusion_3d(m, 10.0)
         h = 1.0 / (m + 1)
         sig = -np.arange(args.nshift) * 0.01 * 6.0 / h ** 2
         bh = np.random.default_rng(0).uniform(-1.0, 1.0, crs.n)
         workload = (f"{m}^3 convection-diffusion (Pe 10), sigma_k = -k * 0.01 * diag, k = 0..{args.nshift - 1}, "
                     "b ~ U[-1,1], tol 1e-8, FP64")
+    else:
+        crs, _ = P.helmholtz_3d(m)
+        sig = -np.arange(args.nshift) * 0.01
+        bh = np.random.default_rng(0).uniform(-1.0, 1.0, crs.n)
+        workload = (f"{m}^3 h^2-scaled 7-point Laplacian, sigma_k = -0.01 k, k = 0..{args.nshift - 1}, "
+                    f"Chebyshev({args.ms_degree}) polynomial preconditioner on [0.05, 11.95] (MS_PARAMETERS "
+                    "shifts for the multishift solvers, a per-shift polynomial for the loop), b ~ U[-1,1], "
+                    "tol 1e-8, FP64")
     n = crs.n
     b = torch.from_numpy(bh).to("cuda:0")
     K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
     A = idrs_amd.Matrix(K)
     tol, maxit = 1e-8, args.ms_maxit
+    poly = args.ms_problem == "laplacian"
+    foci = (0.05, 11.95)
+    if poly:
+        idrs_amd.real_preconditioner(A, 2, args.ms_degree, foci=foci)
+        sig_solve, alpha_s = idrs_amd.ms_parameters(A, sig)
+    else:
+        sig_solve = sig
     torch.cuda.synchronize()
 
     def timed(fn):
@@ -316,23 +331,44 @@ def bench_multishift(args):
         torch.cuda.synchronize()
         return r, (time.perf_counter() - t) * 1e3
 
-    rm, ms_idrs = timed(lambda: idrs_amd.msidrs(A, b, sig, 4, tolerance=tol, maximum_iterations=maxit))
+    def ms(fn, s_):
+        r = fn(A, b, sig_solve, s_, tolerance=tol, maximum_iterations=maxit)
+        return (r, idrs_amd.ms_scaleback(A, alpha_s, r.x)) if poly else (r, r.x)
+
+    (rm, xm), ms_idrs = timed(lambda: ms(idrs_amd.msidrs, 4))
     print(f"# msidrs(4): {ms_idrs:.1f} ms, {rm.iterations} its, flag {rm.flag}, max relres {rm.relres.max():.2e}",
           file=sys.stderr, flush=True)
-    rq, ms_qmr = timed(lambda: idrs_amd.msqmridr(A, b, sig, 8, tolerance=tol, maximum_iterations=maxit))
+    (rq, xq), ms_qmr = timed(lambda: ms(idrs_amd.msqmridr, 8))
     print(f"# msqmridr(8): {ms_qmr:.1f} ms, {rq.iterations} its, flag {rq.flag}, max relres {rq.relres.max():.2e}",
           file=sys.stderr, flush=True)
-    # per-shift loop: IDR(4) on K - sigma_k I via the operator's seed
+    # per-shift loop: IDR(4) on K - sigma_k I via the operator's seed (with the
+    # polynomial rebuilt for the shifted operator, and its scale-back)
     loop_its, x = [], torch.empty_like(b)
+    xs_loop = []
     torch.cuda.synchronize()
     t = time.perf_counter()
     for sk in sig:
         A.set_seed(complex(sk) if np.iscomplexobj(sig) else float(sk))
+        if poly:
+            idrs_amd.real_preconditioner(A, 2, args.ms_degree, foci=foci)
         r = idrs_amd.idrs(A, b, 4, tolerance=tol, maximum_iterations=maxit, out=x)
+        if poly:
+            xs_loop.append(idrs_amd.ms_scaleback(A, A.alpha.reshape(-1, 1), x.reshape(-1, 1)))
         loop_its.append(r.iterations)
     torch.cuda.synchronize()
     loop_ms = (time.perf_counter() - t) * 1e3
     A.set_seed(0j if np.iscomplexobj(sig) else 0.0)
+    if poly:
+        # true residuals of (K - sigma_k I) x_k = b for the multishift solutions
+        Kt = crs.to_scipy()
+        worst = 0.0
+        for name, xx in (("msidrs", xm), ("msqmridr", xq)):
+            xh = xx.cpu().numpy()
+            for k, sk in enumerate(sig):
+                res = bh - (Kt @ xh[:, k] - sk * xh[:, k])
+                worst = max(worst, np.linalg.norm(res) / np.linalg.norm(bh))
+        print(f"# worst true residual of the scaled-back multishift solutions: {worst:.2e}", file=sys.stderr,
+              flush=True)
     print(f"# per-shift IDR(4) loop: {loop_ms:.1f} ms, iterations {loop_its}", file=sys.stderr, flush=True)
     ok = [(t_, name) for t_, name, f_ in ((ms_idrs, "msidrs4", rm.flag), (ms_qmr, "msqmridr8", rq.flag)) if f_ == 0]
     best = min(ok)[0] if ok else min(ms_idrs, ms_qmr)
@@ -370,7 +406,8 @@ def main():
                     help="inner vectors of the mixed form (matrix values are FP32 either way)")
     ap.add_argument("--ms-grid", type=int, default=128, help="grid side of the multishift form")
     ap.add_argument("--nshift", type=int, default=16)
-    ap.add_argument("--ms-problem", choices=["helmholtz", "cdr"], default="helmholtz")
+    ap.add_argument("--ms-problem", choices=["helmholtz", "cdr", "laplacian"], default="helmholtz")
+    ap.add_argument("--ms-degree", type=int, default=8, help="Chebyshev degree of the laplacian multishift problem")
     ap.add_argument("--ms-maxit", type=int, default=20000)
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")

@@ -340,20 +340,27 @@ static int matvec_entry(idrs_matrix_t A, const double *v, double *w, int what) {
 }
 
 template <class T>
-static int ms_scaleback_entry(idrs_matrix_t A, int ns, int degree, const double *alpha_s, const double *Y, double *X) {
+static int ms_scaleback_entry(idrs_matrix_t A, int ns, int degree, const double *alpha_s, const double *Y, double *X,
+                              int device) {
     return guarded([&] {
         need(A && alpha_s && Y && X, "null argument");
         need(ns >= 1 && degree >= 0, "bad arguments");
         hipStream_t st = lib_stream();
         const long n = A->op.K->nrows;
-        DVec<T> dy(n), dx(n), w1(n), w2(n);
+        DVec<T> dy(device ? 0 : n), dx(device ? 0 : n), w1(n), w2(n);
         for (int k = 0; k < ns; k++) {
             std::vector<std::complex<double>> a(degree + 1);
             for (int i = 0; i <= degree; i++)
                 a[i] = {alpha_s[2 * ((long)k * (degree + 1) + i)], alpha_s[2 * ((long)k * (degree + 1) + i) + 1]};
-            HIP_TRY(hipMemcpyAsync(dy.p, (const T *)Y + (long)k * n, sizeof(T) * n, hipMemcpyHostToDevice, st));
-            op_scaleback_with<T>(A->op, a, dy.p, dx.p, w1.p, w2.p, st);
-            HIP_TRY(hipMemcpyAsync((T *)X + (long)k * n, dx.p, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+            const T *yk = (const T *)Y + (long)k * n;
+            T *xk = (T *)X + (long)k * n;
+            if (device) {
+                op_scaleback_with<T>(A->op, a, yk, xk, w1.p, w2.p, st);
+            } else {
+                HIP_TRY(hipMemcpyAsync(dy.p, yk, sizeof(T) * n, hipMemcpyHostToDevice, st));
+                op_scaleback_with<T>(A->op, a, dy.p, dx.p, w1.p, w2.p, st);
+                HIP_TRY(hipMemcpyAsync(xk, dx.p, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+            }
         }
         HIP_TRY(hipStreamSynchronize(st));
     });
@@ -652,12 +659,12 @@ int idrs_scaleback_f64(idrs_matrix_t A, const double *y, double *x) { return mat
 int idrs_scaleback_c128(idrs_matrix_t A, const double *y, double *x) { return matvec_entry<c128>(A, y, x, 2); }
 
 int idrs_ms_scaleback_f64(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
-                          double *X) {
-    return ms_scaleback_entry<double>(A, n_sigma, degree, alpha_s, Y, X);
+                          double *X, int device_pointers) {
+    return ms_scaleback_entry<double>(A, n_sigma, degree, alpha_s, Y, X, device_pointers);
 }
 int idrs_ms_scaleback_c128(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
-                           double *X) {
-    return ms_scaleback_entry<c128>(A, n_sigma, degree, alpha_s, Y, X);
+                           double *X, int device_pointers) {
+    return ms_scaleback_entry<c128>(A, n_sigma, degree, alpha_s, Y, X, device_pointers);
 }
 
 int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,

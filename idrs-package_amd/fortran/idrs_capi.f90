@@ -274,21 +274,21 @@ module idrs_capi
          complex(c_double_complex), intent(out) :: w(*)
       end function
 
-      integer(c_int) function idrs_ms_scaleback_f64(A, n_sigma, degree, alpha_s, Y, X) &
+      integer(c_int) function idrs_ms_scaleback_f64(A, n_sigma, degree, alpha_s, Y, X, device_pointers) &
             bind(C, name="idrs_ms_scaleback_f64")
          import :: c_int, c_double, c_ptr, c_double_complex
          type(c_ptr), value :: A
-         integer(c_int), value :: n_sigma, degree
+         integer(c_int), value :: n_sigma, degree, device_pointers
          complex(c_double_complex), intent(in) :: alpha_s(*)
          real(c_double), intent(in) :: Y(*)
          real(c_double), intent(out) :: X(*)
       end function
 
-      integer(c_int) function idrs_ms_scaleback_c128(A, n_sigma, degree, alpha_s, Y, X) &
+      integer(c_int) function idrs_ms_scaleback_c128(A, n_sigma, degree, alpha_s, Y, X, device_pointers) &
             bind(C, name="idrs_ms_scaleback_c128")
          import :: c_int, c_ptr, c_double_complex
          type(c_ptr), value :: A
-         integer(c_int), value :: n_sigma, degree
+         integer(c_int), value :: n_sigma, degree, device_pointers
          complex(c_double_complex), intent(in) :: alpha_s(*)
          complex(c_double_complex), intent(in) :: Y(*)
          complex(c_double_complex), intent(out) :: X(*)

@@ -417,7 +417,7 @@ contains
          allocate( coef(0:0, size(sigma)) )
          coef = (1._cp, 0._cp)
       end if
-      call idrs_check( idrs_ms_scaleback_f64(matrix_handle(A, .false.), size(sigma), deg, coef, y, x), 'scaleback' )
+      call idrs_check( idrs_ms_scaleback_f64(matrix_handle(A, .false.), size(sigma), deg, coef, y, x, 0), 'scaleback' )
    end function ms_rscaleback
 
    function ms_cscaleback( A, sigma, b, y ) result(x)
@@ -439,7 +439,7 @@ contains
          allocate( coef(0:0, size(sigma)) )
          coef = (1._cp, 0._cp)
       end if
-      call idrs_check( idrs_ms_scaleback_c128(matrix_handle(A, .true.), size(sigma), deg, coef, y, x), 'scaleback' )
+      call idrs_check( idrs_ms_scaleback_c128(matrix_handle(A, .true.), size(sigma), deg, coef, y, x, 0), 'scaleback' )
    end function ms_cscaleback
 
 ! Shifts of the preconditioned system and the scale-back polynomials

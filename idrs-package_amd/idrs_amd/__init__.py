@@ -24,6 +24,8 @@ from .api import (  # noqa: F401
     idrs_batched,
     idrs_mixed,
     make_p,
+    ms_parameters,
+    ms_scaleback,
     msidrs,
     msqmridr,
     qmridr,

@@ -144,8 +144,8 @@ _PROTOS = {
                                     c_void_p, POINTER(idrs_stats_t), POINTER(c_int)]),
     "idrs_idrs_mixed_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_double, c_int, c_int,
                                      c_void_p, POINTER(idrs_stats_t), POINTER(c_int)]),
-    "idrs_ms_scaleback_f64": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
-    "idrs_ms_scaleback_c128": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "idrs_ms_scaleback_f64": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    "idrs_ms_scaleback_c128": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int]),
     "idrs_crs_transpose": (c_int, [c_void_p, POINTER(c_void_p)]),
     "idrs_crs_transpose_refresh": (c_int, [c_void_p, c_void_p]),
     "idrs_user_matrix_set_values_f64": (c_int, [c_void_p, c_void_p, c_int]),

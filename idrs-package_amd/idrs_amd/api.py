@@ -173,6 +173,7 @@ class Matrix:
         self._h = h
         self.is_complex = K.is_complex
         self.n = K.nrows
+        self.seed = 0.0
         if which_precon:
             self.set_precon(which_precon, shift)
 
@@ -200,6 +201,7 @@ class Matrix:
             check(lib.idrs_matrix_set_precon_f64(self._h, int(which_precon), _ptr(s)))
 
     def set_seed(self, seed):
+        self.seed = seed
         lib = _lib.load()
         if np.iscomplexobj(seed):
             s = np.array([np.real(seed), np.imag(seed)], dtype=np.float64)
@@ -646,3 +648,60 @@ def idrs_mixed(A, b, s, tolerance=None, maximum_iterations=None, variant=None, x
     check(f(A.handle, n, bp, int(s), ctypes.byref(o), float(inner_tolerance), int(max_outer), int(bool(fp32_vectors)),
             xp, ctypes.byref(st), ctypes.byref(outer)))
     return MixedResult(x, st.flag, st.relres, st.iterations, outer.value, st.solve_ms)
+
+
+def ms_parameters(A, sigma):
+    """sigma_p, alpha_s = MS_PARAMETERS(A, sigma) (matrix_module.f90:402-549,
+    without shift-and-invert).  With the operator's polynomial p (degree d)
+    and q(t) = t p(t): q(t) - q(s) = (t - s) p_s(t), so the shifted systems
+    (K - s I) x = b become K p(K) y - q(s) y = b with x = p_s(K) y; p_s is
+    the quotient of synthetic division, sigma_p = q(s).  Shifts are taken
+    relative to the operator's seed.  Returns sigma_p (n_sigma) and alpha_s
+    ((d+1) x n_sigma, column k the coefficients of p_{s_k})."""
+    sigma = np.atleast_1d(np.asarray(sigma))
+    alpha = A.alpha
+    d = len(alpha) - 1
+    cplx = np.iscomplexobj(sigma) or np.iscomplexobj(A.seed) or np.any(np.imag(alpha) != 0)
+    dt = np.complex128 if cplx else np.float64
+    sig_s = (sigma - A.seed).astype(dt)
+    alpha = alpha if cplx else alpha.real
+    alpha_s = np.zeros((d + 1, len(sigma)), dtype=dt)
+    sigma_p = np.zeros(len(sigma), dtype=dt)
+    for k, sk in enumerate(sig_s):
+        if d == 0:
+            alpha_s[0, k] = alpha[0]
+        else:
+            alpha_s[d, k] = alpha[d]
+            for i in range(d - 1, -1, -1):
+                alpha_s[i, k] = alpha[i] + sk * alpha_s[i + 1, k]
+        sigma_p[k] = sk * alpha_s[0, k]
+    return sigma_p, alpha_s
+
+
+def ms_scaleback(A, alpha_s, Y, out=None):
+    """X(:,k) = p_k(K') Y(:,k) / D (ms_rscaleback / ms_cscaleback,
+    matrix_module.f90:724-806) for the alpha_s of ms_parameters.  Y: (n,
+    n_sigma) numpy array (any layout) or the (n, n_sigma) view of a CUDA
+    tensor that msidrs/msqmridr returned (column k contiguous)."""
+    lib = _lib.load()
+    alpha_s = np.asarray(alpha_s)
+    d, ns = alpha_s.shape[0] - 1, alpha_s.shape[1]
+    a = np.ascontiguousarray(alpha_s.T.astype(np.complex128))   # (ns, d+1): column k contiguous
+    device = _is_torch(Y)
+    if device:
+        import torch
+
+        cplx = Y.is_complex()
+        Yc = Y.t().contiguous()                     # (ns, n): row k = column k of Y
+        Xc = torch.empty_like(Yc)
+        torch.cuda.current_stream().synchronize()
+        f = lib.idrs_ms_scaleback_c128 if cplx else lib.idrs_ms_scaleback_f64
+        check(f(A.handle, ns, d, a.ctypes.data, ctypes.c_void_p(Yc.data_ptr()), ctypes.c_void_p(Xc.data_ptr()), 1))
+        return Xc.t()
+    cplx = np.iscomplexobj(Y) or A.is_complex
+    dt = np.complex128 if cplx else np.float64
+    Yc = np.ascontiguousarray(np.asarray(Y, dtype=dt).T)
+    Xc = np.empty_like(Yc)
+    f = lib.idrs_ms_scaleback_c128 if cplx else lib.idrs_ms_scaleback_f64
+    check(f(A.handle, ns, d, a.ctypes.data, _ptr(Yc), _ptr(Xc), 0))
+    return Xc.T

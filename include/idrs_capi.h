@@ -152,11 +152,12 @@ int idrs_scaleback_c128(idrs_matrix_t A, const double *y, double *x);
    X(:,k) = p_k(K') Y(:,k) / D with alpha_s(0:degree, k) as (re, im) pairs,
    column-major (degree+1) x n_sigma (MS_PARAMETERS output);
    ms_rscaleback / ms_cscaleback, matrix_module.f90:724-806 (no deflation,
-   no shift-and-invert, as elsewhere in this library) */
+   no shift-and-invert, as elsewhere in this library).  Y and X are device
+   pointers if device_pointers != 0; alpha_s is always a host array. */
 int idrs_ms_scaleback_f64(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
-                          double *X);
+                          double *X, int device_pointers);
 int idrs_ms_scaleback_c128(idrs_matrix_t A, int n_sigma, int degree, const double *alpha_s, const double *Y,
-                           double *X);
+                           double *X, int device_pointers);
 
 /* ---------------------------------------------------------------------
  * solvers
