@@ -102,6 +102,8 @@ static int idrs_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs
         so.resvec = o.resvec;
         so.resvec_len = o.resvec_len;
         so.p_seed = o.p_seed ? o.p_seed : 1;
+        need(o.graph_mode >= 0 && o.graph_mode <= 2, "graph_mode must be 0, 1 or 2");
+        so.graph_mode = o.graph_mode;
         const int prow = (so.variant == 1) ? s : 1;
         SolveStats ss;
         SpmvTimer &tm = spmv_timer();

@@ -639,10 +639,36 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     HIP_TRY(hipEventCreateWithFlags(&pe[0], hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&pe[1], hipEventDisableTiming));
     long max_cycles = cap_it / (s + 1) + 2;
+    // Small systems are bound by the host's launch rate (~55 launches per
+    // cycle): capture one cycle in a HIP graph and replay it -- the same
+    // kernels with the same arguments in the same order.  Not while timing
+    // single SpMVs (events between launches).
+    const bool use_graph = !tm.enabled && (o.graph_mode == 1 || (o.graph_mode == 0 && n <= (1L << 21)));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    if (use_graph) {
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        try {
+            enqueue_cycle();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipHostFree(h_info);
+            (void)hipEventDestroy(pe[0]);
+            (void)hipEventDestroy(pe[1]);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        HIP_TRY(hipStreamEndCapture(stream, &graph));
+        HIP_TRY(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    }
     bool done = false;
     try {
         for (long c = 0; c < max_cycles && !done; c++) {
-            enqueue_cycle();
+            if (use_graph) HIP_TRY(hipGraphLaunch(gexec, stream));
+            else enqueue_cycle();
             HIP_TRY(hipMemcpyAsync(&h_info[c % 2], info, sizeof(int), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipEventRecord(pe[c % 2], stream));
             if (c >= 1) {
@@ -673,6 +699,8 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         out.spmv_ms = tm.enabled ? tm.total_ms() : 0.0;
         out.spmv_launches = tm.launches();
     } catch (...) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
         (void)hipHostFree(h_info);
         (void)hipEventDestroy(pe[0]);
         (void)hipEventDestroy(pe[1]);
@@ -680,6 +708,8 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         (void)hipEventDestroy(e1);
         throw;
     }
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
     (void)hipHostFree(h_info);
     (void)hipEventDestroy(pe[0]);
     (void)hipEventDestroy(pe[1]);

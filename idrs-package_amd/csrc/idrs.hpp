@@ -35,6 +35,7 @@ struct SolveOpts {
     int resvec_len = 0;
     const T *PT_raw = nullptr;   // device, raw shadow space before orthonormalisation
     unsigned long long p_seed = 1;   // default stream seed (reference: this_image() = 1)
+    int graph_mode = 0;              // 0: HIP graph per cycle when n <= 2^21, 1: always, 2: never
 };
 
 template <class T>

@@ -185,6 +185,11 @@ typedef struct {
     uint64_t p_seed;            /* seed of that stream (0 -> 1, the reference's this_image()) */
     int32_t device_pointers;    /* b, x, x0, U0, PT_raw are device pointers */
     int32_t time_spmv;          /* time each SpMV launch with HIP events  */
+    int32_t graph_mode;         /* 0: replay each IDR(s) cycle from a HIP graph
+                                   when n <= 2^21 (launch-bound sizes);
+                                   1: always; 2: never (no reference
+                                   counterpart; results are identical)     */
+    int32_t reserved;
 } idrs_opts_t;
 
 typedef struct {
