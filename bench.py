@@ -148,13 +148,14 @@ def bench_batched(args):
     dev = torch.device("cuda:0")
     Ks = [idrs_amd.crs_format(c.n, c.n, c.row_ptr, c.col_ind, rval=c.val) for c, _ in systems]
     KB = idrs_amd.batch_format(Ks)
+    group = KB.autotune() if not args.no_autotune else None
     AB = idrs_amd.Matrix(KB)
     b_all = torch.from_numpy(np.concatenate([bb for _, bb in systems])).to(dev)
     x_all = torch.empty_like(b_all)
     s, tol, maxit = 2, 1e-8, args.batch_maxit
 
-    def batch_step():
-        return idrs_amd.idrs_batched(AB, b_all, s, tolerance=tol, maximum_iterations=maxit, out=x_all)
+    def batch_step(graph=True):
+        return idrs_amd.idrs_batched(AB, b_all, s, tolerance=tol, maximum_iterations=maxit, graph=graph, out=x_all)
 
     for _ in range(args.warmup):
         batch_step()
@@ -165,7 +166,17 @@ def bench_batched(args):
     torch.cuda.synchronize()
     batch_ms = (time.perf_counter() - t_start) * 1e3 / args.steps
     print(f"# batched: {batch_ms:.1f} ms, flags {np.bincount(rb.flag)}, iterations min/median/max "
-          f"{rb.iterations.min()}/{int(np.median(rb.iterations))}/{rb.iterations.max()}", file=sys.stderr, flush=True)
+          f"{rb.iterations.min()}/{int(np.median(rb.iterations))}/{rb.iterations.max()}, lanes per row {group}",
+          file=sys.stderr, flush=True)
+    # the same batch with eager launches (no graph replay)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    re_ = batch_step(graph=False)
+    torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t_start) * 1e3
+    same = bool(np.array_equal(re_.iterations, rb.iterations))
+    x_graph = x_all.clone()
+    print(f"# batched, eager launches: {eager_ms:.1f} ms (same iteration counts: {same})", file=sys.stderr, flush=True)
     # per-system loop over the same device data
     As = [idrs_amd.Matrix(K) for K in Ks]
     offs = np.cumsum([0] + [c.n for c, _ in systems])
@@ -180,7 +191,7 @@ def bench_batched(args):
         loop_its[k] = r.iterations
     torch.cuda.synchronize()
     loop_ms = (time.perf_counter() - t_start) * 1e3
-    diff = float((x_all - xs).norm() / xs.norm())
+    diff = float((x_graph - xs).norm() / xs.norm())
     worst = np.argsort(-np.abs(rb.iterations - loop_its))[:3]
     for k in worst:
         c = systems[k][0]
@@ -211,6 +222,7 @@ def bench_batched(args):
         "config": {"workload": f"{args.nsys} 2-D 5-point convection-diffusion systems, side floor(32*16^u), "
                                f"Pe U[1,100], IDR(2), tol 1e-8, FP64 CRS", "rows": int(ntot), "nnz": int(nnz),
                    "per_system_loop_ms": round(loop_ms, 2), "speedup_vs_loop": round(loop_ms / batch_ms, 2),
+                   "eager_batch_ms": round(eager_ms, 2), "lanes_per_row": group,
                    "iterations_median": int(np.median(rb.iterations)), "iterations_max": int(rb.iterations.max()),
                    "flags": {str(k): int(v) for k, v in enumerate(np.bincount(rb.flag))},
                    "x_rel_diff_vs_loop": diff},
@@ -273,6 +285,71 @@ def bench_mixed(args):
                    "flag": rm.flag, "inner_iterations": rm.iterations, "outer_steps": rm.outer, "relres": rm.relres,
                    "inner_tolerance": args.inner_tol},
         "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def bench_small(args):
+    """The reference's own CPU-runnable case: plain IDR(4), no preconditioner,
+    32^3 convection-diffusion (Pe 10), b = A*ones, Gaussian shadow space
+    (seed 0), tol 1e-8, CRS.  At this size a cycle is launch-bound, so the
+    solve is timed with each cycle replayed from a HIP graph and with eager
+    launches; the oracle solves the same system on one host core."""
+    import numpy as np
+    import torch
+
+    import idrs_amd
+    from idrs_amd import problems as P
+
+    m = args.small_grid
+    crs, _ = P.convection_diffusion_3d(m, 10.0)
+    n = crs.n
+    bh = crs.matvec(np.ones(n))
+    PT = P.gaussian_shadow(n, args.s, seed=0)
+    A = idrs_amd.Matrix(idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val))
+    b = torch.from_numpy(bh).to("cuda:0")
+    PTd = torch.from_numpy(np.ascontiguousarray(PT)).to("cuda:0")
+    x = torch.empty_like(b)
+
+    def timed(graph):
+        for _ in range(args.warmup):
+            idrs_amd.idrs(A, b, args.s, tolerance=1e-8, PT_raw=PTd, graph=graph, out=x)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            r = idrs_amd.idrs(A, b, args.s, tolerance=1e-8, PT_raw=PTd, graph=graph, out=x)
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t) * 1e3 / args.steps
+
+    rg, g_ms = timed(True)
+    re_, e_ms = timed(False)
+    xg = x.cpu().numpy()
+    err = float(np.linalg.norm(xg - 1.0) / np.sqrt(n))
+    print(f"# graph: {g_ms:.2f} ms ({rg.iterations} its); eager: {e_ms:.2f} ms ({re_.iterations} its); "
+          f"rms error vs ones {err:.2e}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+
+        Ao = orc.OracleMatrix(orc.CRS, n, crs=crs)
+        t0 = time.perf_counter()
+        _, fo, ro, io, _ = orc.idrs(Ao, bh, args.s, tolerance=1e-8, PT_raw=PT)
+        dt = (time.perf_counter() - t0) * 1e3
+        print(f"# oracle: {dt:.1f} ms, {io} its, flag {fo}, relres {ro:.3e}", file=sys.stderr, flush=True)
+        cpu = {"value": round(dt, 2), "unit": "ms (time-to-solution)", "cores": 1, "kind": "port",
+               "sample": f"oracle (scalar C restatement) full IDR({args.s}) solve of the same system: "
+                         f"{io} iterations, flag {fo}"}
+    out = {
+        "metric": "time-to-solution (ms) at rel. residual 1e-8, the reference's CPU case",
+        "value": round(g_ms, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(g_ms, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{m}^3 convection-diffusion (Pe 10), plain IDR({args.s}), b = A*ones, "
+                               "Gaussian shadow space (seed 0), tol 1e-8, CRS",
+                   "iterations": rg.iterations, "flag": rg.flag, "relres": rg.relres,
+                   "eager_ms": round(e_ms, 3), "graph_speedup": round(e_ms / g_ms, 2)},
+        "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 
@@ -397,16 +474,18 @@ def main():
     ap.add_argument("--degree", type=int, default=8)
     ap.add_argument("--format", choices=["both", "crs", "dia"], default="both",
                     help="storage: both (headline DIA, then CRS), or one of them")
-    ap.add_argument("--form", choices=["solve", "batched", "mixed", "multishift"], default="solve",
+    ap.add_argument("--form", choices=["solve", "batched", "mixed", "multishift", "small"], default="solve",
                     help="solve: the 200^3 IDR(4)+Chebyshev solve (default); batched: many small systems")
     ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
+    ap.add_argument("--small-grid", type=int, default=32, help="grid side of the small form")
     ap.add_argument("--batch-maxit", type=int, default=4000)
-    ap.add_argument("--inner-tol", type=float, default=1e-4, help="inner tolerance of the mixed form")
+    ap.add_argument("--no-autotune", action="store_true", help="batched form: keep the default lanes per row")
+    ap.add_argument("--inner-tol", type=float, default=1e-9, help="inner tolerance of the mixed form")
     ap.add_argument("--mixed-vectors", choices=["fp64", "fp32"], default="fp64",
                     help="inner vectors of the mixed form (matrix values are FP32 either way)")
     ap.add_argument("--ms-grid", type=int, default=128, help="grid side of the multishift form")
     ap.add_argument("--nshift", type=int, default=16)
-    ap.add_argument("--ms-problem", choices=["helmholtz", "cdr", "laplacian"], default="helmholtz")
+    ap.add_argument("--ms-problem", choices=["helmholtz", "cdr", "laplacian"], default="laplacian")
     ap.add_argument("--ms-degree", type=int, default=8, help="Chebyshev degree of the laplacian multishift problem")
     ap.add_argument("--ms-maxit", type=int, default=20000)
     ap.add_argument("--cpu-sample-iters", type=int, default=6)
@@ -429,6 +508,9 @@ def main():
         return
     if args.form == "multishift":
         bench_multishift(args)
+        return
+    if args.form == "small":
+        bench_small(args)
         return
 
     import numpy as np
@@ -460,6 +542,9 @@ def main():
         A = idrs_amd.Matrix(K)
         A.set_precon(1)
         idrs_amd.real_preconditioner(A, 2, d, foci=foci)
+        # value entries one product reads (the scaled CDS operator does not
+        # store its all-ones main diagonal)
+        nvals, unit = A.stored_values()
         torch.cuda.synchronize()
 
         def step(timed):
@@ -492,13 +577,12 @@ def main():
         # roofline of the SpMV kernel over the timed launches
         tot_bytes, launches = solve_spmv_bytes(n, nnz, s, d, last.iterations, s)
         if fmt == "dia":
-            nd = len(dia.offsets)
-            tot_bytes = tot_bytes - launches * (12 * nnz + 4 * (n + 1)) + launches * 8 * nd * n
+            tot_bytes = tot_bytes - launches * (12 * nnz + 4 * (n + 1)) + launches * 8 * nvals
         spmv_ms = sum(r.spmv_ms for r in results) / len(results)
         achieved = tot_bytes / (spmv_ms * 1e-3) / 1e9
         it_bytes = iteration_bytes(n, nnz, s, d)
         if fmt == "dia":
-            it_bytes = it_bytes - (d + 1) * (12 * nnz + 4 * (n + 1)) + (d + 1) * 8 * len(dia.offsets) * n
+            it_bytes = it_bytes - (d + 1) * (12 * nnz + 4 * (n + 1)) + (d + 1) * 8 * nvals
         solve_gbs = last.iterations * it_bytes / (ms_per_step * 1e-3) / 1e9
         del A, K
         return {
@@ -512,6 +596,7 @@ def main():
                 "traffic": read_traffic(fmt),
                 "kernel": f"k_spmv_{fmt}",
                 "bytes_per_launch": round(tot_bytes / launches),
+                "stored_values": nvals, "implicit_unit_diagonal": unit,
                 "avg_launch_us": round(spmv_ms * 1e3 / launches, 2),
             },
             "solve_gbs_model": round(solve_gbs, 1),

@@ -554,6 +554,32 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
     // Restricts every later tile launch to the unfinished systems' tiles once
     // they are at most 3/4 of the tiles launched so far.  Finished systems
     // keep an empty tile range (their reduction kernels return at once).
+    // Each cycle replays a HIP graph of enqueue_cycle() (the late cycles run
+    // few live tiles and are launch-bound); a compaction changes the tile
+    // lists and grids, so the next cycle captures a new graph.  Replaced
+    // graphs may still be queued: all are destroyed after the final sync.
+    const bool use_graph = o.graph_mode != 2;
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
+    hipGraphExec_t cur = nullptr;
+    auto capture = [&]() {
+        HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        try {
+            enqueue_cycle();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamEndCapture(stream, &g));
+        graphs.push_back(g);
+        hipGraphExec_t e = nullptr;
+        HIP_TRY(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+        execs.push_back(e);
+        cur = e;
+    };
     auto compact = [&](const int *liveflag) {
         std::vector<int> tsys, t0(nsys + 1);
         std::vector<long> r0v, r1v;
@@ -585,11 +611,15 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
         live.tile_r1 = bv.tile_r1;
         gt = dim3(nt);
         cset ^= 1;
+        cur = nullptr;
     };
     hipEvent_t pe[2];
     HIP_TRY(hipEventCreateWithFlags(&pe[0], hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&pe[1], hipEventDisableTiming));
     auto cleanup = [&]() {
+        (void)hipStreamSynchronize(stream);
+        for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
+        for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
         (void)hipHostFree(h_flag);
         (void)hipEventDestroy(pe[0]);
         (void)hipEventDestroy(pe[1]);
@@ -600,7 +630,12 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
         const long max_cycles = cap_it / (s + 1) + 2;
         bool done = false;
         for (long c = 0; c < max_cycles && !done; c++) {
-            enqueue_cycle();
+            if (use_graph) {
+                if (!cur) capture();
+                HIP_TRY(hipGraphLaunch(cur, stream));
+            } else {
+                enqueue_cycle();
+            }
             int *slot = h_flag + (c % 2) * (nsys + 1);
             HIP_TRY(hipMemcpyAsync(slot, flag, sizeof(int) * (nsys + 1), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipEventRecord(pe[c % 2], stream));

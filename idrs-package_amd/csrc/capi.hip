@@ -241,6 +241,8 @@ static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opt
         so.omega = (const T *)o.omega;
         so.n_omega = o.omega ? o.n_omega : 0;
         so.p_seed = o.p_seed ? o.p_seed : 1;
+        need(o.graph_mode >= 0 && o.graph_mode <= 2, "graph_mode must be 0, 1 or 2");
+        so.graph_mode = o.graph_mode;
         const int nsys = part->nsys;
         const long ntot = part->h_row0[nsys];
         const int prow = (so.variant == 1) ? s : 1;
@@ -542,6 +544,13 @@ int idrs_matrix_set_precon_c128(idrs_matrix_t A, int which_precon, const double 
              "which_precon must be 0 (none) or 1 (diagonal); dense/band LU (2, 3) are not offered");
         std::complex<double> sh = shift ? std::complex<double>(shift[0], shift[1]) : 0.0;
         op_set_diag_precon(A->op, which_precon, shift ? &sh : nullptr);
+    });
+}
+int idrs_matrix_stored_values(idrs_matrix_t A, int64_t *count, int *unit_diagonal) {
+    return guarded([&] {
+        need(A && count, "null argument");
+        *count = A->op.stored_values();
+        if (unit_diagonal) *unit_diagonal = (A->op.sval && A->op.unit_diag >= 0) ? 1 : 0;
     });
 }
 int idrs_matrix_set_seed_f64(idrs_matrix_t A, double seed) {

@@ -21,6 +21,7 @@
 #include "spmv_epi.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstring>
 #include <numeric>
@@ -401,6 +402,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
     const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
     const T *__restrict__ x = A.x;
     const VT *__restrict__ val = A.val;
+    const int ud = A.ud;
     const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
     for (long c = c0; c < c1; c++) {
         const long row = c * SPMV_THREADS + threadIdx.x;
@@ -409,6 +411,9 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
         const T zq = zp[row];
         T pe[PC];
         if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
+        // diagonal d is stored at d - (d > ud); the implicit unit diagonal
+        // (d == ud) enters as fmav(1, x, acc), the same operation as a
+        // stored 1.0
         if constexpr (ND > 0) {
             VT vv[ND];
             T xx[ND];
@@ -416,7 +421,8 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
             for (int d = 0; d < ND; d++) {
                 const long cc = row + s_off[d];
                 const bool ok = cc >= 0 && cc < ncols;
-                vv[d] = ok ? ldnt(&val[(long)d * n + row]) : VT(0);
+                const long sd = d - (d > ud ? 1 : 0);
+                vv[d] = !ok ? VT(0) : (d == ud ? VT(1) : ldnt(&val[sd * n + row]));
                 xx[d] = ok ? x[cc] : T(0);
             }
 #pragma unroll
@@ -424,7 +430,8 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
         } else {
             for (int d = 0; d < ndiag; d++) {
                 const long cc = row + s_off[d];
-                if (cc >= 0 && cc < ncols) acc = fmav(ldnt(&val[(long)d * n + row]), x[cc], acc);
+                const long sd = d - (d > ud ? 1 : 0);
+                if (cc >= 0 && cc < ncols) acc = fmav(d == ud ? VT(1) : ldnt(&val[sd * n + row]), x[cc], acc);
             }
         }
         const T yv = finish_value(A, row, acc, zq);
@@ -623,6 +630,7 @@ static SpmvArgs<VT, T> base_args(const DevMatrix &K, const VT *val) {
     A.val = val;
     A.ndiag = K.ndiag;
     A.offsets = K.offsets;
+    A.ud = INT_MAX;
     return A;
 }
 
@@ -640,6 +648,7 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
     if (cv) {
         if constexpr (Tr<T>::is_complex && !single) {
             auto A = base_args<c128, T>(K, (const c128 *)op.values());
+            if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
             A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
             A.has_seed = has_seed;
             A.sdiag = op.which_precon == 1 ? (const c128 *)op.pdiag : nullptr;
@@ -655,6 +664,7 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
             if (!v32) throw std::invalid_argument("single-precision values not prepared");
             if (has_seed) throw std::invalid_argument("seed shift with single-precision values is not supported");
             auto A = base_args<float, T>(K, v32);
+            if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
             A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
             A.has_seed = 0;
             A.sdiag = nullptr;
@@ -668,6 +678,7 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
         const R *vals = single ? (const R *)(op.sval ? op.sval32 : K.val32) : (const R *)op.values();
         if (!vals) throw std::invalid_argument("single-precision values not prepared");
         auto A = base_args<R, T>(K, vals);
+        if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
         A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
         A.has_seed = has_seed;
         A.sdiag = op.which_precon == 1 ? (const R *)op.pdiag : nullptr;
@@ -1093,6 +1104,72 @@ __global__ void k_real_to_cplx(long n, const double *__restrict__ in, c128 *__re
         out[i] = c128(in[i], 0.0);
 }
 
+// per block: 1 if some v[i] is not exactly 1 (complex: (1, +0))
+__device__ inline bool exactly_one(double v) { return v == 1.0; }
+__device__ inline bool exactly_one(c128 v) { return v.re == 1.0 && __double_as_longlong(v.im) == 0; }
+template <class VT>
+__global__ void k_not_one(long n, const VT *__restrict__ v, int *__restrict__ bad) {
+    __shared__ int s_bad;
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    bool b = false;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b = b || !exactly_one(v[i]);
+    if (b) s_bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) bad[blockIdx.x] = s_bad;
+}
+
+// DIA with diagonal scaling: K D^-1 has the main diagonal a_ii / a_ii,
+// exactly 1 wherever a_ii is a nonzero without a shift.  When all of it is
+// 1, the scaled values drop that diagonal (one value stream less per
+// product) and the kernel multiplies by an implicit 1.
+static void drop_unit_diagonal(Operator &op, bool cvals, hipStream_t st) {
+    DevMatrix &K = *op.K;
+    const long n = K.nrows;
+    int z0 = -1;
+    for (int d = 0; d < K.ndiag; d++)
+        if (K.h_offsets[d] == 0) z0 = d;
+    if (z0 < 0 || n < 1 || K.ndiag < 2) return;
+    const size_t vsz = cvals ? sizeof(c128) : sizeof(double);
+    const int g = std::min(vec_grid(n), 1024);
+    int *bad = nullptr;
+    HIP_TRY(hipMalloc(&bad, sizeof(int) * g));
+    std::vector<int> hb(g);
+    try {
+        if (cvals)
+            hipLaunchKernelGGL((k_not_one<c128>), dim3(g), dim3(256), 0, st, n, (const c128 *)op.sval + (long)z0 * n, bad);
+        else
+            hipLaunchKernelGGL((k_not_one<double>), dim3(g), dim3(256), 0, st, n, (const double *)op.sval + (long)z0 * n,
+                               bad);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(hb.data(), bad, sizeof(int) * g, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipFree(bad);
+        throw;
+    }
+    (void)hipFree(bad);
+    for (int v : hb)
+        if (v) return;
+    void *packed = nullptr;
+    HIP_TRY(hipMalloc(&packed, vsz * n * (K.ndiag - 1)));
+    char *src = (char *)op.sval, *dst = (char *)packed;
+    if (z0 > 0) HIP_TRY(hipMemcpyAsync(dst, src, vsz * n * z0, hipMemcpyDeviceToDevice, st));
+    if (z0 < K.ndiag - 1)
+        HIP_TRY(hipMemcpyAsync(dst + vsz * n * z0, src + vsz * n * (z0 + 1), vsz * n * (K.ndiag - 1 - z0),
+                               hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    (void)hipFree(op.sval);
+    op.sval = packed;
+    op.unit_diag = z0;
+}
+
+long Operator::stored_values() const {
+    const long full = K->fmt == DevFmt::CRS ? K->nnz : (long)K->ndiag * K->nrows;
+    return (sval && unit_diag >= 0) ? full - K->nrows : full;
+}
+
 // REAL_PRECON/COMPLEX_PRECON(1, K[, shift]) (user_module.f90:1327-1381):
 // D = diag(K) [- shift] with zeros already replaced by 1 in diag(K).  The
 // values of K D^-1 are precomputed so that real_mv's t = v/D is folded into
@@ -1104,6 +1181,7 @@ void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shi
     (void)hipFree(op.sval);
     (void)hipFree(op.sval32);
     op.pdiag = op.sval = op.sval32 = nullptr;
+    op.unit_diag = -1;
     op.which_precon = which;
     if (which != 1) return;
     if (K.nrows != K.ncols) throw std::invalid_argument("diagonal scaling needs a square matrix");
@@ -1163,13 +1241,14 @@ void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shi
     LAUNCH_CHECK();
     HIP_TRY(hipStreamSynchronize(st));
     (void)hipFree(tmpc);
+    if (K.fmt != DevFmt::CRS) drop_unit_diagonal(op, cvals, st);
 }
 
 void op_make_lowprec(Operator &op) {
     hipStream_t st = lib_stream();
     DevMatrix &K = *op.K;
     if (op.values_complex()) throw std::invalid_argument("single-precision copy of complex values");
-    long cnt = K.fmt == DevFmt::CRS ? K.nnz : (long)K.ndiag * K.nrows;
+    long cnt = op.sval ? op.stored_values() : (K.fmt == DevFmt::CRS ? K.nnz : (long)K.ndiag * K.nrows);
     int gv = vec_grid(cnt);
     if (op.sval) {
         if (!op.sval32) {

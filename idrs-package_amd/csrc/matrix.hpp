@@ -88,6 +88,10 @@ struct Operator {
     // values of K D^-1 (column scaled) when which_precon == 1, same layout as K
     void *sval = nullptr;
     void *sval32 = nullptr;       // float copy of the scaled values
+    // DIA only: position (in K's diagonal list) of the main diagonal when its
+    // scaled values are all exactly 1; sval then omits that diagonal and the
+    // SpMV multiplies by an implicit 1 (-1: every diagonal stored)
+    int unit_diag = -1;
     int degree = 0;
     int poly_kind = POLY_NONE;
     std::vector<std::complex<double>> alpha;   // alpha_p(0:degree)
@@ -98,6 +102,8 @@ struct Operator {
     bool values32 = false;
     ~Operator();
     const void *values() const { return sval ? sval : K->val; }
+    // number of value entries the SpMV reads (sval, sval32 or K's)
+    long stored_values() const;
     bool values_complex() const {
         return (K->kind == ValKind::Complex) || (sval && pdiag_kind == ValKind::Complex);
     }

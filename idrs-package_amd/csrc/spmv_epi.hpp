@@ -35,6 +35,7 @@ struct SpmvArgs {
     const VT *val;
     int ndiag;
     const int *offsets;
+    int ud;            // DIA: index of an implicit unit diagonal (not in val), INT_MAX if none
     const T *x;
     T xs;
     const T *z;

@@ -31,6 +31,8 @@ module idrs_capi
       integer(c_int64_t) :: p_seed = 0
       integer(c_int32_t) :: device_pointers = 0
       integer(c_int32_t) :: time_spmv = 0
+      integer(c_int32_t) :: graph_mode = 0
+      integer(c_int32_t) :: reserved = 0
    end type idrs_opts_t
 
 ! Optional arguments of QMRIDR, MSIDRS and MSQMRIDR (idrs_krylov_opts_t)

@@ -44,6 +44,8 @@ class idrs_opts_t(ctypes.Structure):
         ("p_seed", c_uint64),
         ("device_pointers", c_int32),
         ("time_spmv", c_int32),
+        ("graph_mode", c_int32),
+        ("reserved", c_int32),
     ]
 
 
@@ -111,6 +113,7 @@ _PROTOS = {
     "idrs_matrix_destroy": (c_int, [c_void_p]),
     "idrs_matrix_set_precon_f64": (c_int, [c_void_p, c_int, c_void_p]),
     "idrs_matrix_set_precon_c128": (c_int, [c_void_p, c_int, c_void_p]),
+    "idrs_matrix_stored_values": (c_int, [c_void_p, POINTER(c_int64), POINTER(c_int)]),
     "idrs_matrix_set_seed_f64": (c_int, [c_void_p, c_double]),
     "idrs_matrix_set_seed_c128": (c_int, [c_void_p, c_void_p]),
     "idrs_real_preconditioner": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int]),

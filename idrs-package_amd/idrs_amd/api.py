@@ -200,6 +200,14 @@ class Matrix:
             s = None if shift is None else np.array([float(shift)])
             check(lib.idrs_matrix_set_precon_f64(self._h, int(which_precon), _ptr(s)))
 
+    def stored_values(self):
+        """(value entries one product reads, unit diagonal implicit) --
+        with diagonal scaling a CDS operator's all-ones main diagonal is not
+        stored (idrs_matrix_stored_values)."""
+        cnt, unit = ctypes.c_int64(), ctypes.c_int()
+        check(_lib.load().idrs_matrix_stored_values(self._h, ctypes.byref(cnt), ctypes.byref(unit)))
+        return cnt.value, bool(unit.value)
+
     def set_seed(self, seed):
         self.seed = seed
         lib = _lib.load()
@@ -288,7 +296,7 @@ def make_p(n, s, method=1, PT_raw=None, r=None, seed=1, complex=False):
 
 
 def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, U0=None,
-         omega=None, resvec=False, PT_raw=None, p_seed=None, time_spmv=False, out=None):
+         omega=None, resvec=False, PT_raw=None, p_seed=None, time_spmv=False, graph=None, out=None):
     """x = IDRS(A, b, s, M1, tolerance, maximum_iterations, variant, flag,
     relres, iterations, x0, U0, omega, resvec)  (idrs_module.f90:107-251).
 
@@ -297,6 +305,8 @@ def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None
     solve; x0, U0, PT_raw and out are then CUDA tensors too).
     PT_raw: raw shadow space, shape (n, s) (row q = PT(:,q)), orthonormalised
     by the library like MAKE_P does with its random numbers.
+    graph: None replays each cycle from a HIP graph when n <= 2^21, True
+    always, False never; the results are the same either way.
     """
     lib = _lib.load()
     device = _is_torch(b)
@@ -361,6 +371,7 @@ def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None
         o.resvec, o.resvec_len = rv.ctypes.data, len(rv)
     o.p_seed = int(p_seed) if p_seed is not None else 0
     o.time_spmv = 1 if time_spmv else 0
+    o.graph_mode = 0 if graph is None else (1 if graph else 2)
     st = idrs_stats_t()
     f = lib.idrs_idrs_c128 if cplx else lib.idrs_idrs_f64
     check(f(A.handle, n, bp, int(s), ctypes.byref(o), xp, ctypes.byref(st)))
@@ -530,11 +541,13 @@ def batch_info(K):
 
 
 def idrs_batched(A, b, s, tolerance=None, maximum_iterations=None, variant=None, x0=None, omega=None,
-                 PT_raw=None, p_seed=None, out=None):
+                 PT_raw=None, p_seed=None, graph=True, out=None):
     """x_k = IDRS(A_k, b_k, s, ...) for every system of a batch (A built on a
     batch_format user matrix).  b (and x0, PT_raw, out) are the systems'
     vectors concatenated: numpy arrays, or CUDA tensors for a
-    device-resident batch.  Returns BatchResult with per-system arrays."""
+    device-resident batch.  graph: replay each cycle from a HIP graph
+    (recaptured when finished systems' tiles are dropped); the results are
+    the same either way.  Returns BatchResult with per-system arrays."""
     lib = _lib.load()
     nsys, row0 = batch_info(A.K)
     if nsys == 0:
@@ -553,6 +566,7 @@ def idrs_batched(A, b, s, tolerance=None, maximum_iterations=None, variant=None,
         o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
     if variant is not None:
         o.has_variant, o.variant = 1, int(variant)
+    o.graph_mode = 0 if graph else 2
     if device:
         import torch
 

@@ -108,6 +108,12 @@ int idrs_matrix_set_precon_f64(idrs_matrix_t A, int which_precon, const double *
 /* A%P = COMPLEX_PRECON(which_precon, K[, shift]), user_module.f90:1465-1518;
    shift is one complex (re, im) or NULL. */
 int idrs_matrix_set_precon_c128(idrs_matrix_t A, int which_precon, const double *shift);
+/* Value entries one product of A reads (no reference counterpart: the
+   reference always multiplies by every stored entry).  With diagonal
+   scaling of a CDS matrix whose scaled main diagonal is exactly 1 that
+   diagonal is not stored (*unit_diagonal = 1; NULL allowed) and the
+   products are bit-identical to multiplying by the stored 1s. */
+int idrs_matrix_stored_values(idrs_matrix_t A, int64_t *count, int *unit_diagonal);
 /* A%rseed = seed (PP_IDRS/real_matrix.inc:41-45) */
 int idrs_matrix_set_seed_f64(idrs_matrix_t A, double seed);
 /* A%cseed = seed (complex_matrix.inc) */
@@ -186,9 +192,10 @@ typedef struct {
     int32_t device_pointers;    /* b, x, x0, U0, PT_raw are device pointers */
     int32_t time_spmv;          /* time each SpMV launch with HIP events  */
     int32_t graph_mode;         /* 0: replay each IDR(s) cycle from a HIP graph
-                                   when n <= 2^21 (launch-bound sizes);
-                                   1: always; 2: never (no reference
-                                   counterpart; results are identical)     */
+                                   when n <= 2^21 (launch-bound sizes; the
+                                   batched solve always); 1: always;
+                                   2: never.  No reference counterpart; the
+                                   results are identical either way        */
     int32_t reserved;
 } idrs_opts_t;
 

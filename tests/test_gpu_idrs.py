@@ -189,3 +189,37 @@ def test_complex_toeplitz_accuracy(gpu, s):
     print(f"s={s} gpu it={r.iterations} relres={r.relres:.2e} oracle it={io} relres={ro:.2e}")
     assert r.flag == fo == 0 and abs(r.iterations - io) <= max(2, 0.03 * io)
     assert r.relres <= 1e-12 and ro <= 1e-12
+
+
+@pytest.mark.parametrize("case", ["plain", "chebyshev", "m1_variant2", "complex", "omega_maxit"])
+def test_graph_replay_is_bit_identical(gpu, case):
+    """A cycle replayed from a HIP graph launches the same kernels with the
+    same arguments in the same order as the eager enqueue: x, the iteration
+    count, the flag and the residual history are bit-identical."""
+    crs, _ = P.convection_diffusion_3d(16, 10.0, reaction=1.0)
+    rng = np.random.default_rng(7)
+    b = rng.uniform(-1, 1, crs.n)
+    kw = dict(tolerance=1e-9, maximum_iterations=1500, resvec=True)
+    s = 4
+    if case == "chebyshev":
+        A = gpu_crs(crs, diag_precon=True, poly=(2, 8, [0.15, 1.85]))
+    elif case == "m1_variant2":
+        A = gpu_crs(crs)
+        kw.update(M1=gpu_crs(crs, diag_precon=True), variant=2)
+        s = 1
+    elif case == "complex":
+        A = idrs_amd.Matrix(idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, cval=crs.val * (1 + 0.1j)))
+        b = b + 1j * rng.uniform(-1, 1, crs.n)
+    else:
+        A = gpu_crs(crs)
+    if case == "omega_maxit":
+        kw.update(omega=[0.9, 1.1, 0.7], maximum_iterations=37)
+    e = idrs_amd.idrs(A, b, s, graph=False, **kw)
+    g = idrs_amd.idrs(A, b, s, graph=True, **kw)
+    print(f"{case}: it={e.iterations} flag={e.flag} relres={e.relres:.3e}")
+    assert e.iterations == g.iterations and e.flag == g.flag
+    assert np.array_equal(e.x, g.x)
+    assert np.array_equal(e.resvec, g.resvec)
+    assert e.relres == g.relres
+    if case != "omega_maxit":
+        assert e.flag == 0

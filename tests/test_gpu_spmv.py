@@ -183,3 +183,67 @@ def test_autotune_keeps_results(gpu):
     w = K * x
     ref = oracle_crs(crs).user_mv(x)
     assert np.all(np.abs(w - ref) <= _bound(crs, x))
+
+
+def test_scaled_dia_drops_unit_diagonal(gpu):
+    """Diagonal scaling of a CDS matrix: K D^-1 has an all-ones main diagonal,
+    which is then not stored (one value stream less per product); the
+    product still equals the reference operator's (real_mv: K (v / D))."""
+    rng = np.random.default_rng(12)
+    crs, dia = P.convection_diffusion_3d(12, 10.0, reaction=1.0)
+    n, nd = crs.n, len(dia.offsets)
+    x = rng.standard_normal(n)
+    A = idrs_amd.Matrix(idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals), which_precon=1)
+    assert A.stored_values() == ((nd - 1) * n, True)
+    Ao = orc.OracleMatrix(orc.DIA, n, dia=(dia.offsets, dia.vals), which_precon=1)
+    assert rel(A * x, Ao.matvec(x)) < 1e-14
+    # the same operator in CRS storage keeps every entry
+    Ac = idrs_amd.Matrix(idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val), which_precon=1)
+    assert Ac.stored_values() == (crs.nnz, False)
+    assert rel(A * x, Ac * x) < 1e-14
+    # complex values
+    cv = dia.vals * (1.0 + 0.3j)
+    Az = idrs_amd.Matrix(idrs_amd.dia_format(n, n, dia.offsets, cvals=cv), which_precon=1)
+    assert Az.stored_values() == ((nd - 1) * n, True)
+    Aoz = orc.OracleMatrix(orc.DIA, n, dia=(dia.offsets, cv), which_precon=1, complex_vectors=True)
+    xz = x + 1j * rng.standard_normal(n)
+    assert rel(Az * xz, Aoz.matvec(xz)) < 1e-14
+    # a zero on the diagonal is replaced by 1 in D, so the scaled entry is 0:
+    # the diagonal stays stored
+    z0 = list(dia.offsets).index(0)
+    v2 = dia.vals.copy()
+    v2[37, z0] = 0.0
+    A2 = idrs_amd.Matrix(idrs_amd.dia_format(n, n, dia.offsets, rvals=v2), which_precon=1)
+    assert A2.stored_values() == (nd * n, False)
+    Ao2 = orc.OracleMatrix(orc.DIA, n, dia=(dia.offsets, v2), which_precon=1)
+    assert rel(A2 * x, Ao2.matvec(x)) < 1e-14
+    # a shift moves D off the diagonal: stored
+    A3 = idrs_amd.Matrix(idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals), which_precon=1, shift=0.5)
+    assert A3.stored_values() == (nd * n, False)
+    # without scaling nothing is dropped
+    A4 = idrs_amd.Matrix(idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals))
+    assert A4.stored_values() == (nd * n, False)
+
+
+def test_scaled_dia_solve_and_mixed(gpu):
+    """IDR(4) + Chebyshev(8) on the unit-diagonal DIA operator against the
+    oracle, and the mixed-precision solve on it (FP32 copy of the packed
+    values)."""
+    crs, dia = P.convection_diffusion_3d(24, 6.2, reaction=1.0)
+    n = crs.n
+    b = np.random.default_rng(0).uniform(-1, 1, n)
+    poly = (2, 8, [0.15, 1.85])
+    A = idrs_amd.Matrix(idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals), which_precon=1)
+    idrs_amd.real_preconditioner(A, 2, 8, foci=poly[2])
+    assert A.stored_values()[1]
+    Ao = orc.OracleMatrix(orc.DIA, n, dia=(dia.offsets, dia.vals), which_precon=1, degree=8,
+                          alpha=orc.chebyshev(8, np.asarray(poly[2])))
+    PT = P.gaussian_shadow(n, 4, seed=0)
+    r = idrs_amd.idrs(A, b, 4, tolerance=1e-8, maximum_iterations=2000, PT_raw=PT)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, 4, tolerance=1e-8, maximum_iterations=2000, PT_raw=PT)
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= 2
+    g = idrs_amd.idrs(A, b, 4, tolerance=0.0, maximum_iterations=30, PT_raw=PT)
+    go = orc.idrs(Ao, b, 4, tolerance=0.0, maximum_iterations=30, PT_raw=PT)
+    assert rel(g.x, go[0]) < 1e-10
+    m = idrs_amd.idrs_mixed(A, b, 4, tolerance=1e-8, maximum_iterations=3000, inner_tolerance=1e-6)
+    assert m.flag == 0 and np.linalg.norm(b - A * m.x) <= 1.01e-8 * np.linalg.norm(b)
