@@ -177,6 +177,8 @@ def bench_batched(args):
     same = bool(np.array_equal(re_.iterations, rb.iterations))
     x_graph = x_all.clone()
     print(f"# batched, eager launches: {eager_ms:.1f} ms (same iteration counts: {same})", file=sys.stderr, flush=True)
+    if args.no_loop:
+        return
     # per-system loop over the same device data
     As = [idrs_amd.Matrix(K) for K in Ks]
     offs = np.cumsum([0] + [c.n for c, _ in systems])
@@ -479,7 +481,8 @@ def main():
     ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
     ap.add_argument("--small-grid", type=int, default=32, help="grid side of the small form")
     ap.add_argument("--batch-maxit", type=int, default=4000)
-    ap.add_argument("--no-autotune", action="store_true", help="batched form: keep the default lanes per row")
+    ap.add_argument("--no-autotune", action="store_true", help="CRS: keep the default lanes per row")
+    ap.add_argument("--no-loop", action="store_true", help="batched form: skip the per-system loop (profiling)")
     ap.add_argument("--inner-tol", type=float, default=1e-9, help="inner tolerance of the mixed form")
     ap.add_argument("--mixed-vectors", choices=["fp64", "fp32"], default="fp64",
                     help="inner vectors of the mixed form (matrix values are FP32 either way)")
@@ -535,8 +538,11 @@ def main():
     formats = ["dia", "crs"] if args.format == "both" else [args.format]
 
     def run(fmt):
+        group = None
         if fmt == "crs":
             K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+            # lanes per row (1: sliced-ELL copy) chosen by timing
+            group = K.autotune() if not args.no_autotune else None
         else:
             K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
         A = idrs_amd.Matrix(K)
@@ -596,7 +602,7 @@ def main():
                 "traffic": read_traffic(fmt),
                 "kernel": f"k_spmv_{fmt}",
                 "bytes_per_launch": round(tot_bytes / launches),
-                "stored_values": nvals, "implicit_unit_diagonal": unit,
+                "stored_values": nvals, "implicit_unit_diagonal": unit, "lanes_per_row": group,
                 "avg_launch_us": round(spmv_ms * 1e3 / launches, 2),
             },
             "solve_gbs_model": round(solve_gbs, 1),

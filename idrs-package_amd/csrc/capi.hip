@@ -86,6 +86,7 @@ static int idrs_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs
     return guarded([&] {
         need(A && b && x, "null argument");
         hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
         idrs_opts_t o{};
         if (opts) o = *opts;
         SolveOpts<T> so;
@@ -153,6 +154,7 @@ static int krylov_entry(Kry kind, idrs_matrix_t A, int n, const double *b, int n
         need(A && b && X, "null argument");
         need(kind == Kry::QMRIDR || (sigma && ns >= 1), "sigma must hold at least one shift");
         hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
         idrs_krylov_opts_t o{};
         if (opts) o = *opts;
         KrylovOpts<T> ko;
@@ -228,6 +230,7 @@ static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opt
         const Partition *part = A->op.K->part.get();
         need(part != nullptr, "not a batch matrix (idrs_batch_format)");
         hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
         idrs_opts_t o{};
         if (opts) o = *opts;
         need(!o.has_M1 && !o.U0 && !o.resvec, "M1, U0 and resvec are not offered for batches");
@@ -285,6 +288,7 @@ static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idr
         need(A && b && x, "null argument");
         need(s >= 1 && s <= SMAX, "s must be in 1..16");
         hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
         idrs_opts_t o{};
         if (opts) o = *opts;
         need(!o.has_M1 && !o.U0 && !o.resvec && !o.omega, "M1, U0, omega and resvec are not offered here");
@@ -333,6 +337,7 @@ static int matvec_entry(idrs_matrix_t A, const double *v, double *w, int what) {
     return guarded([&] {
         need(A && v && w, "null argument");
         hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
         long n = A->op.K->nrows;
         DVec<T> dv((const T *)v, n, st), dw(n), w1(n), w2(n), w3(n);
         EpiArgs<T> none{};
@@ -350,6 +355,7 @@ static int ms_scaleback_entry(idrs_matrix_t A, int ns, int degree, const double 
         need(A && alpha_s && Y && X, "null argument");
         need(ns >= 1 && degree >= 0, "bad arguments");
         hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
         const long n = A->op.K->nrows;
         DVec<T> dy(device ? 0 : n), dx(device ? 0 : n), w1(n), w2(n);
         for (int k = 0; k < ns; k++) {
@@ -476,6 +482,19 @@ int idrs_user_matrix_autotune(idrs_user_matrix_t K, int reps, int *group) {
         need(reps >= 1, "reps must be positive");
         const int g = autotune_group(*K->M, reps, lib_stream(), nullptr);
         if (group) *group = g;
+    });
+}
+
+int idrs_user_matrix_set_group(idrs_user_matrix_t K, int group) {
+    return guarded([&] {
+        need(K != nullptr, "null handle");
+        need(group == 1 || group == 2 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
+             "group must be 1, 2, 4, 8, 16, 32 or 64");
+        DevMatrix &M = *K->M;
+        if (M.fmt != DevFmt::CRS) return;
+        if (group == 1 && !M.sell) build_sell(M, lib_stream());
+        if (group != 1) M.sell.reset();
+        M.group = group;
     });
 }
 

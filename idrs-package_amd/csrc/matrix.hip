@@ -48,6 +48,13 @@ DevMatrix::~DevMatrix() {
     (void)hipFree(tperm);
 }
 
+Sell::~Sell() {
+    (void)hipFree(sl_ptr);
+    (void)hipFree(sl_col);
+    (void)hipFree(sl_perm);
+    (void)hipFree(sl_val);
+}
+
 Partition::~Partition() {
     (void)hipFree(row0);
     (void)hipFree(tile0);
@@ -59,6 +66,7 @@ Operator::~Operator() {
     (void)hipFree(pdiag);
     (void)hipFree(sval);
     (void)hipFree(sval32);
+    (void)hipFree(sl_sval);
 }
 
 SpmvTimer &spmv_timer() {
@@ -95,7 +103,7 @@ int vec_grid(long n) {
 }
 
 int spmv_grid(const DevMatrix &K) {
-    long rows_per_unit = (K.fmt == DevFmt::CRS) ? (long)(SPMV_THREADS / K.group) * 4 : SPMV_THREADS;
+    long rows_per_unit = (K.fmt == DevFmt::CRS && K.group > 1) ? (long)(SPMV_THREADS / K.group) * 4 : SPMV_THREADS;
     long units = (K.nrows + rows_per_unit - 1) / rows_per_unit;
     long b = units;
     if (b > 4096) b = 4096;
@@ -442,14 +450,86 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
     epi_store<T, EPI, NV>(e, E, s_red);
 }
 
+// Sliced copy of a CRS matrix (group 1): one thread per row.  Entry k of the
+// row is at p0 + SELL_C k, so each load instruction of a wavefront reads
+// SELL_C consecutive values (or column indices).  Rows of up to 8 entries
+// issue all their value and index loads, then all x gathers; the row sum
+// runs over k in CRS order (the reference's sequential row sum).
+template <class VT, class T>
+__device__ inline T sell_row(const SpmvArgs<VT, T> &A, long row, long p0, int w) {
+    const VT *__restrict__ val = A.val;
+    const int *__restrict__ scol = A.sl_col;
+    const T *__restrict__ x = A.x;
+    const int len = A.rp[row + 1] - A.rp[row];
+    T acc = T(0);
+    if (w <= 8) {
+        VT vv[8];
+        int cc[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const bool ok = k < len;
+            vv[k] = ok ? ldnt(&val[p0 + (long)SELL_C * k]) : VT(0);
+            cc[k] = ok ? ldnt(&scol[p0 + (long)SELL_C * k]) : 0;
+        }
+        T xx[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) xx[k] = k < len ? x[cc[k]] : T(0);
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (k < len) acc = fmav(vv[k], xx[k], acc);
+    } else {
+        for (int k = 0; k < len; k++)
+            acc = fmav(ldnt(&val[p0 + (long)SELL_C * k]), x[ldnt(&scol[p0 + (long)SELL_C * k])], acc);
+    }
+    return acc;
+}
+
+template <class VT, class T, int EPI, int NV>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv_sell(SpmvArgs<VT, T> A, EpiArgs<T> E, const int *info) {
+    static_assert(SPMV_THREADS % SELL_C == 0 && SELL_C == WAVE, "one slice per wavefront");
+    if (info && *info >= 0) return;
+    constexpr int PC = EPI == EPI_PDOT ? NV : 1;
+    __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
+    T e[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) e[i] = T(0);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const long n = A.nrows;
+    const long nchunk = (n + SPMV_THREADS - 1) / SPMV_THREADS;
+    const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
+    const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
+    const int wv = threadIdx.x / WAVE;
+    for (long c = c0; c < c1; c++) {
+        // the wavefront's slice (uniform: chunks start at multiples of SELL_C)
+        const int slice = __builtin_amdgcn_readfirstlane((int)(c * (SPMV_THREADS / SELL_C) + wv));
+        const long row = c * SPMV_THREADS + threadIdx.x;
+        if (row >= n) continue;
+        const long sp = A.sl_ptr[slice];
+        const int w = (int)((A.sl_ptr[slice + 1] - sp) / SELL_C);
+        const T zq = zp[row];
+        T pe[PC];
+        if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
+        const T acc = sell_row(A, row, sp + (row % SELL_C), w);
+        const T yv = finish_value(A, row, acc, zq);
+        A.y[row] = yv;
+        if constexpr (EPI != EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
+        epi_accum<T, EPI, NV, PC>(e, pe, E, yv, 0, 1);
+    }
+    epi_store<T, EPI, NV>(e, E, s_red);
+}
+
 template <class VT, class T, int EPI, int NV>
 static void launch_spmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T> E, const int *info,
                         hipStream_t st) {
     int grid = spmv_grid(K);
     E.nparts = grid;
     dim3 gd(grid), bd(SPMV_THREADS);
-    if (K.fmt == DevFmt::CRS) {
+    if (K.fmt == DevFmt::CRS && A.sl_ptr) {
+        hipLaunchKernelGGL((k_spmv_sell<VT, T, EPI, NV>), gd, bd, 0, st, A, E, info);
+    } else if (K.fmt == DevFmt::CRS) {
+        // group 1 without sliced values (e.g. the FP32 copy): 2 lanes per row
         switch (K.group) {
+        case 1:
         case 2: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 2>), gd, bd, 0, st, A, E, info); break;
         case 4: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 4>), gd, bd, 0, st, A, E, info); break;
         case 8: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 8>), gd, bd, 0, st, A, E, info); break;
@@ -572,6 +652,35 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_crs(SpmvArgs<VT, T> A, E
     else epi_store<T, EPI, NV>(e, E, s_red);
 }
 
+// tile form of k_spmv_sell: a tile starts at any row, so a wavefront may
+// span two slices and each lane reads its own slice's start and width
+template <class VT, class T, int EPI, int NV>
+__global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_sell(SpmvArgs<VT, T> A, EpiArgs<T> E, TileArgs ta) {
+    const int tile = blockIdx.x;
+    if (ta.info && ta.info[(long)ta.tile_sys[tile] * ta.stride] >= 0) return;
+    constexpr int PC = EPI == EPI_PDOT ? NV : 1;
+    __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
+    T e[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) e[i] = T(0);
+    const long c0 = ta.tile_r0[tile], c1 = ta.tile_r1[tile];
+    const T *zp = A.z ? A.z : A.y;
+    for (long row = c0 + threadIdx.x; row < c1; row += SPMV_THREADS) {
+        const long slice = row / SELL_C;
+        const long sp = A.sl_ptr[slice];
+        const int w = (int)((A.sl_ptr[slice + 1] - sp) / SELL_C);
+        const T zq = zp[row];
+        T pe[PC];
+        if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
+        const T acc = sell_row(A, row, sp + (row % SELL_C), w);
+        const T yv = finish_value(A, row, acc, zq);
+        A.y[row] = yv;
+        if constexpr (EPI != EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
+        epi_accum<T, EPI, NV, PC>(e, pe, E, yv, 0, 1);
+    }
+    epi_store<T, EPI, NV>(e, E, s_red);
+}
+
 template <class VT, class T, int EPI, int NV>
 static void launch_bspmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T> E, const TileCtx &tc,
                          hipStream_t st) {
@@ -583,7 +692,13 @@ static void launch_bspmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T
     if (nt < 1) return;
     E.nparts = nt;
     dim3 gd(nt), bd(SPMV_THREADS);
+    if (A.sl_ptr) {
+        hipLaunchKernelGGL((k_bspmv_sell<VT, T, EPI, NV>), gd, bd, 0, st, A, E, ta);
+        LAUNCH_CHECK();
+        return;
+    }
     switch (K.group) {
+    case 1:
     case 2: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 2>), gd, bd, 0, st, A, E, ta); break;
     case 4: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 4>), gd, bd, 0, st, A, E, ta); break;
     case 8: hipLaunchKernelGGL((k_bspmv_crs<VT, T, EPI, NV, 8>), gd, bd, 0, st, A, E, ta); break;
@@ -631,7 +746,22 @@ static SpmvArgs<VT, T> base_args(const DevMatrix &K, const VT *val) {
     A.ndiag = K.ndiag;
     A.offsets = K.offsets;
     A.ud = INT_MAX;
+    A.sl_ptr = nullptr;
+    A.sl_col = nullptr;
     return A;
+}
+
+// group 1: point the product at the sliced copy of the values it reads
+// (K's own or the operator's scaled ones); without one it stays on CRS
+template <class VT, class T>
+static void use_sliced(const Operator &op, SpmvArgs<VT, T> &A) {
+    const DevMatrix &K = *op.K;
+    if (K.fmt != DevFmt::CRS || K.group != 1 || !K.sell) return;
+    const void *v = op.sval ? op.sl_sval : K.sell->sl_val;
+    if (!v) return;
+    A.val = (const VT *)v;
+    A.sl_ptr = K.sell->sl_ptr;
+    A.sl_col = K.sell->sl_col;
 }
 
 template <class T>
@@ -649,6 +779,7 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
         if constexpr (Tr<T>::is_complex && !single) {
             auto A = base_args<c128, T>(K, (const c128 *)op.values());
             if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
+            use_sliced(op, A);
             A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
             A.has_seed = has_seed;
             A.sdiag = op.which_precon == 1 ? (const c128 *)op.pdiag : nullptr;
@@ -679,6 +810,7 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
         if (!vals) throw std::invalid_argument("single-precision values not prepared");
         auto A = base_args<R, T>(K, vals);
         if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
+        if constexpr (!single) use_sliced(op, A);
         A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
         A.has_seed = has_seed;
         A.sdiag = op.which_precon == 1 ? (const R *)op.pdiag : nullptr;
@@ -878,6 +1010,16 @@ static void refresh_diag(DevMatrix &K, hipStream_t st) {
     LAUNCH_CHECK();
 }
 
+static void *sliced_copy(const Sell &S, const void *src, bool cplx, hipStream_t st);
+
+// the sliced copy's values after K.val changed
+static void refresh_sliced(DevMatrix &K, hipStream_t st) {
+    if (!K.sell) return;
+    (void)hipFree(K.sell->sl_val);
+    K.sell->sl_val = nullptr;
+    K.sell->sl_val = sliced_copy(*K.sell, K.val, K.kind == ValKind::Complex, st);
+}
+
 void transpose_refresh(const DevMatrix &K, DevMatrix &KT, hipStream_t st) {
     if (!KT.tperm || KT.nnz != K.nnz || KT.nrows != K.ncols || KT.kind != K.kind)
         throw std::invalid_argument("not a transpose of this matrix");
@@ -891,6 +1033,7 @@ void transpose_refresh(const DevMatrix &K, DevMatrix &KT, hipStream_t st) {
                                (double *)KT.val);
         LAUNCH_CHECK();
     }
+    refresh_sliced(KT, st);
     if (KT.nrows) refresh_diag(KT, st);
 }
 
@@ -903,6 +1046,7 @@ void set_values(DevMatrix &K, const void *vals, bool device, hipStream_t st) {
         HIP_TRY(hipMemcpyAsync(K.val, vals, vsz * K.nnz, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
     (void)hipFree(K.val32);
     K.val32 = nullptr;
+    refresh_sliced(K, st);
     if (K.nrows) refresh_diag(K, st);
 }
 
@@ -1013,6 +1157,70 @@ void user_spmv(const DevMatrix &K, const T *v, T *w, hipStream_t st) {
     tmp.K = nullptr;
 }
 
+// sliced copies: dst[p] = src[perm[p]] (padding: zero)
+template <class V>
+__global__ void k_sell_gather(long nent, const int *__restrict__ perm, const V *__restrict__ src, V *__restrict__ dst) {
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < nent; p += (long)gridDim.x * blockDim.x) {
+        const int q = perm[p];
+        dst[p] = q >= 0 ? src[q] : V(0);
+    }
+}
+
+static void *sliced_copy(const Sell &S, const void *src, bool cplx, hipStream_t st) {
+    const size_t vsz = cplx ? sizeof(c128) : sizeof(double);
+    void *dst = nullptr;
+    HIP_TRY(hipMalloc(&dst, vsz * (S.nent ? S.nent : 1)));
+    const int g = vec_grid(S.nent);
+    if (cplx)
+        hipLaunchKernelGGL((k_sell_gather<c128>), dim3(g), dim3(256), 0, st, S.nent, S.sl_perm, (const c128 *)src,
+                           (c128 *)dst);
+    else
+        hipLaunchKernelGGL((k_sell_gather<double>), dim3(g), dim3(256), 0, st, S.nent, S.sl_perm,
+                           (const double *)src, (double *)dst);
+    LAUNCH_CHECK();
+    return dst;
+}
+
+// slice widths and the entry permutation on the host (row pointers read
+// back once), column indices and values gathered on the device
+void build_sell(DevMatrix &K, hipStream_t st) {
+    if (K.fmt != DevFmt::CRS) throw std::invalid_argument("sliced copy of a non-CRS matrix");
+    const long n = K.nrows;
+    std::vector<int> rp(n + 1);
+    HIP_TRY(hipMemcpy(rp.data(), K.row_ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    std::unique_ptr<Sell> S(new Sell);
+    S->nslices = (int)((n + SELL_C - 1) / SELL_C);
+    std::vector<long> sp(S->nslices + 1, 0);
+    for (int q = 0; q < S->nslices; q++) {
+        int w = 0;
+        for (long r = (long)q * SELL_C; r < std::min(n, (long)(q + 1) * SELL_C); r++) w = std::max(w, rp[r + 1] - rp[r]);
+        sp[q + 1] = sp[q] + (long)w * SELL_C;
+    }
+    S->nent = sp[S->nslices];
+    std::vector<int> perm(S->nent, -1);
+    for (long r = 0; r < n; r++) {
+        const long p0 = sp[r / SELL_C] + r % SELL_C;
+        for (int k = 0; k < rp[r + 1] - rp[r]; k++) perm[p0 + (long)SELL_C * k] = rp[r] + k;
+    }
+    S->sl_ptr = dev_upload(sp.data(), sp.size());
+    S->sl_perm = dev_upload(perm.data(), perm.size());
+    HIP_TRY(hipMalloc(&S->sl_col, sizeof(int) * (S->nent ? S->nent : 1)));
+    const int g = vec_grid(S->nent);
+    hipLaunchKernelGGL((k_sell_gather<int>), dim3(g), dim3(256), 0, st, S->nent, S->sl_perm, (const int *)K.col,
+                       S->sl_col);
+    LAUNCH_CHECK();
+    S->sl_val = sliced_copy(*S, K.val, K.kind == ValKind::Complex, st);
+    HIP_TRY(hipStreamSynchronize(st));
+    K.sell = std::move(S);
+}
+
+void op_prepare(const Operator &op, hipStream_t st) {
+    const DevMatrix &K = *op.K;
+    if (K.fmt != DevFmt::CRS || !K.sell || !op.sval || op.sl_sval) return;
+    op.sl_sval = sliced_copy(*K.sell, op.sval, op.values_complex(), st);
+    HIP_TRY(hipStreamSynchronize(st));
+}
+
 // Autotuner of the CRS kernel's lanes per row: times the product with the
 // candidate group sizes around the average-row-length choice (reps launches
 // each, HIP events on the library stream) and keeps the fastest.  Explicit,
@@ -1029,12 +1237,20 @@ int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    const int g0 = K.group;
-    int best = g0;
+    const int g0 = K.group == 1 ? 2 : K.group;
+    const int gin = K.group;
+    int best = gin;
     float best_ms = 0.f;
+    // the sliced copy is a candidate when its padding stays within 1/4 of
+    // the stored entries (it is built for the timing and kept if chosen)
+    bool sell_ok = false;
+    if (!K.sell && K.nnz > 0) build_sell(K, st);
+    if (K.sell) {
+        sell_ok = K.sell->nent <= K.nnz + K.nnz / 4;
+    }
     try {
-        for (int g : {g0 / 2, g0, g0 * 2, g0 * 4}) {
-            if (g < 2 || g > 64) continue;
+        for (int g : {1, g0 / 2, g0, g0 * 2, g0 * 4}) {
+            if (g == 1 ? !sell_ok : (g < 2 || g > 64)) continue;
             K.group = g;
             if (cplx) user_spmv<c128>(K, (const c128 *)x, (c128 *)y, st);   // warm-up
             else user_spmv<double>(K, (const double *)x, (double *)y, st);
@@ -1054,7 +1270,7 @@ int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair
             }
         }
     } catch (...) {
-        K.group = g0;
+        K.group = gin;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         (void)hipFree(x);
@@ -1062,6 +1278,7 @@ int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair
         throw;
     }
     K.group = best;
+    if (best != 1) K.sell.reset();
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(x);
@@ -1180,7 +1397,9 @@ void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shi
     (void)hipFree(op.pdiag);
     (void)hipFree(op.sval);
     (void)hipFree(op.sval32);
+    (void)hipFree(op.sl_sval);
     op.pdiag = op.sval = op.sval32 = nullptr;
+    op.sl_sval = nullptr;
     op.unit_diag = -1;
     op.which_precon = which;
     if (which != 1) return;
@@ -1242,6 +1461,7 @@ void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shi
     HIP_TRY(hipStreamSynchronize(st));
     (void)hipFree(tmpc);
     if (K.fmt != DevFmt::CRS) drop_unit_diagonal(op, cvals, st);
+    else op_prepare(op, st);
 }
 
 void op_make_lowprec(Operator &op) {

@@ -49,6 +49,22 @@ struct TileCtx {
     const long *tile_r0 = nullptr, *tile_r1 = nullptr;
 };
 
+// Sliced ELL copy of a CRS matrix, used when group == 1 (one lane per row):
+// slices of SELL_C consecutive rows, each stored entry-major at the slice's
+// width (its longest row), so entry k of row r sits at
+// sl_ptr[r / SELL_C] + SELL_C k + r % SELL_C and every load of a wavefront
+// reads SELL_C consecutive entries.  Row lengths still come from row_ptr.
+constexpr int SELL_C = 64;
+struct Sell {
+    int nslices = 0;
+    long nent = 0;                 // sliced entries, padding included
+    long *sl_ptr = nullptr;        // nslices+1
+    int *sl_col = nullptr;         // nent (padding: 0, never read)
+    int *sl_perm = nullptr;        // nent: CRS entry of each sliced entry, -1 for padding
+    void *sl_val = nullptr;        // K's values (double or c128) in sliced order
+    ~Sell();
+};
+
 struct DevMatrix {
     int orig_format = 0;     // user_module matrix_format: 2 CRS, 3 COO, 4 CDS, 5 DIA
     DevFmt fmt = DevFmt::CRS;
@@ -57,7 +73,8 @@ struct DevMatrix {
     long nnz = 0;            // stored entries (CRS) or ndiag*nrows (DIA)
     // CRS (0-based on device)
     int *row_ptr = nullptr, *col = nullptr;
-    int group = 8;           // lanes that share one row in the CRS kernel
+    int group = 8;           // lanes that share one row in the CRS kernel; 1: the sliced copy
+    std::unique_ptr<Sell> sell;   // built for group 1 (autotune_group)
     // DIA
     int ndiag = 0;
     int *offsets = nullptr;
@@ -92,6 +109,8 @@ struct Operator {
     // scaled values are all exactly 1; sval then omits that diagonal and the
     // SpMV multiplies by an implicit 1 (-1: every diagonal stored)
     int unit_diag = -1;
+    // sval in K's sliced order (K.sell), made by op_prepare
+    mutable void *sl_sval = nullptr;
     int degree = 0;
     int poly_kind = POLY_NONE;
     std::vector<std::complex<double>> alpha;   // alpha_p(0:degree)
@@ -177,9 +196,15 @@ DevMatrix *make_coo(int nrows, int ncols, long nnz, const int *indx1, const int 
                     ValKind kind);
 DevMatrix *make_cds(int nrows, int ncols, int half_band, const void *band, ValKind kind);
 DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const void *vals, ValKind kind);
-// times the CRS product for lanes-per-row choices around K.group and keeps
-// the fastest (returns it; times: (group, ms per product) if not null)
+// times the CRS product for lanes-per-row choices around K.group, and the
+// sliced copy (group 1) when its padding is small, and keeps the fastest
+// (returns it; times: (group, ms per product) if not null)
 int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair<int, float>> *times);
+// builds / drops K's sliced copy (group 1)
+void build_sell(DevMatrix &K, hipStream_t st);
+// makes the operator's sliced scaled values when K has a sliced copy; call
+// before products are captured into a graph
+void op_prepare(const Operator &op, hipStream_t st);
 // block-diagonal CRS matrix of nsys square CRS systems (copies), with its
 // Partition
 DevMatrix *make_blockdiag(int nsys, const DevMatrix *const *Ks);

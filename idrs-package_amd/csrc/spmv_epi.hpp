@@ -36,6 +36,8 @@ struct SpmvArgs {
     int ndiag;
     const int *offsets;
     int ud;            // DIA: index of an implicit unit diagonal (not in val), INT_MAX if none
+    const long *sl_ptr;   // CRS: sliced copy (val in sliced order), nullptr: CRS order
+    const int *sl_col;
     const T *x;
     T xs;
     const T *z;

@@ -70,10 +70,15 @@ class UserMatrix:
 
     def autotune(self, reps=10):
         """Pick the CRS kernel's lanes per row by timing (idrs_user_matrix_autotune);
-        returns the chosen group size."""
+        returns the chosen group size (1: the sliced copy, one lane per row)."""
         g = ctypes.c_int()
         check(_lib.load().idrs_user_matrix_autotune(self._h, int(reps), ctypes.byref(g)))
         return g.value
+
+    def set_group(self, group):
+        """Lanes per row of the CRS products (idrs_user_matrix_set_group);
+        1 builds the sliced copy."""
+        check(_lib.load().idrs_user_matrix_set_group(self._h, int(group)))
 
     def __mul__(self, v):
         """w = K*v (ruser_mv / cuser_mv, user_module.f90:197-241)."""

@@ -78,11 +78,19 @@ int idrs_dia_format_c128(int nrows, int ncols, int ndiag, const int *offsets, co
    components are released when the Fortran object goes out of scope). */
 int idrs_user_matrix_destroy(idrs_user_matrix_t A);
 /* Autotuner (no counterpart in the reference): times the CRS product with
-   the lanes-per-row choices around the default (reps products each) and
-   keeps the fastest for all later products of K; *group receives it.  A
-   different choice changes the rounding order inside a row, so the call is
-   explicit.  DIA storage: no-op. */
+   the lanes-per-row choices around the default and, when its padding is
+   at most 1/4 of the entries, with a sliced-ELL copy of the matrix (one
+   lane per row, *group = 1), reps products each, and keeps the fastest
+   for all later products of K; *group receives it.  A different choice
+   changes the rounding order inside a row, so the call is explicit; the
+   sliced copy sums each row in CRS order like the reference's CRS
+   product.  DIA storage: no-op. */
 int idrs_user_matrix_autotune(idrs_user_matrix_t K, int reps, int *group);
+/* Sets the lanes per row of K's CRS products (no counterpart in the
+   reference): 1 (sliced copy, built here), 2, 4, 8, 16, 32 or 64 -- e.g. a
+   choice an earlier autotune made.  Operators made on K pick the change up
+   at their next solve or product call. */
+int idrs_user_matrix_set_group(idrs_user_matrix_t K, int group);
 /* nrows, ncols, stored entries, matrix_format (2 CRS, 3 COO, 4 CDS, 5 DIA) */
 int idrs_user_matrix_info(idrs_user_matrix_t A, int *nrows, int *ncols, int64_t *nnz, int *format);
 

@@ -177,6 +177,24 @@ def test_batch_graph_replay_is_bit_identical(gpu, poly):
     assert np.array_equal(e.x, g.x, equal_nan=True)
 
 
+def test_batch_sliced_copy(gpu):
+    """The batch's tile product on the sliced-ELL copy (group 1; tiles start
+    mid-slice at system boundaries) against each system's oracle solve."""
+    systems = _systems(SPECS)
+    A, b = _batch(systems)
+    A.K.set_group(1)
+    _, row0 = idrs_amd.batch_info(A.K)
+    r = idrs_amd.idrs_batched(A, b, 2, tolerance=1e-8, maximum_iterations=3000)
+    for k, (c, bb) in enumerate(systems):
+        xo, fo, ro, io, _ = orc.idrs(oracle_crs(c), bb, 2, tolerance=1e-8, maximum_iterations=3000)
+        assert r.flag[k] == fo == 0 and abs(r.iterations[k] - io) <= 2, (k, r.iterations[k], io)
+        assert np.linalg.norm(bb - c.matvec(r.x[row0[k]:row0[k + 1]])) <= 1.01e-8 * np.linalg.norm(bb)
+    g = idrs_amd.idrs_batched(A, b, 2, tolerance=0.0, maximum_iterations=14)
+    for k, (c, bb) in enumerate(systems):
+        xo = orc.idrs(oracle_crs(c), bb, 2, tolerance=0.0, maximum_iterations=14)[0]
+        assert rel(g.x[row0[k]:row0[k + 1]], xo) < 1e-10, k
+
+
 def test_batch_rejects(gpu):
     crs, _ = P.convection_diffusion_2d(6, 1.0)
     K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, rval=crs.val)

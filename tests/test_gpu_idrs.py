@@ -223,3 +223,29 @@ def test_graph_replay_is_bit_identical(gpu, case):
     assert e.relres == g.relres
     if case != "omega_maxit":
         assert e.flag == 0
+
+
+@pytest.mark.parametrize("precon", [False, True])
+def test_sliced_copy_solve(gpu, precon):
+    """IDR(4) on the sliced-ELL copy (group 1) against the oracle: the
+    plain 32^3 case and the diagonally scaled Chebyshev(8) operator."""
+    if precon:
+        crs, _ = P.convection_diffusion_3d(24, 6.2, reaction=1.0)
+        b = np.random.default_rng(0).uniform(-1, 1, crs.n)
+        kw = dict(diag_precon=True, poly=(2, 8, [0.15, 1.85]))
+    else:
+        crs, _ = P.convection_diffusion_3d(32, 10.0)
+        b = crs.matvec(np.ones(crs.n))
+        kw = {}
+    A = gpu_crs(crs, **kw)
+    A.K.set_group(1)
+    Ao = oracle_crs(crs, **kw)
+    PT = P.gaussian_shadow(crs.n, 4, seed=0)
+    r = idrs_amd.idrs(A, b, 4, tolerance=1e-8, maximum_iterations=2000, PT_raw=PT)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, 4, tolerance=1e-8, maximum_iterations=2000, PT_raw=PT)
+    print(f"gpu it={r.iterations} oracle it={io}")
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= 2
+    k = min(r.iterations, io) // 2
+    g = idrs_amd.idrs(A, b, 4, tolerance=0.0, maximum_iterations=k, PT_raw=PT)
+    go = orc.idrs(Ao, b, 4, tolerance=0.0, maximum_iterations=k, PT_raw=PT)
+    assert rel(g.x, go[0]) < 1e-10
