@@ -10,8 +10,10 @@ solve with b, x and the matrix already resident in HBM.
 
 Prints one JSON line.  roofline: the SpMV kernel (the dominant kernel),
 achieved = algorithmic bytes of the SpMV launches of one solve / their summed
-HIP event time (an extra solve after the timed steps, so the events do not
-slow the timed ones), against the 8 TB/s HBM peak.  cpu_baseline: the oracle (the
+HIP event time (events on the solver's stream around each operator
+application, i.e. its d+1 back-to-back SpMV launches; an extra solve after
+the timed steps, so the events do not slow the timed ones), against the
+8 TB/s HBM peak.  cpu_baseline: the oracle (the
 project's scalar C restatement of the reference) on a bounded sample of the
 same workload, extrapolated to the GPU's iteration count.
 """
@@ -254,6 +256,8 @@ def bench_mixed(args):
     fmt = "crs" if args.format == "crs" else "dia"
     K = (idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val) if fmt == "crs"
          else idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals))
+    if fmt == "crs" and not args.no_autotune:
+        K.autotune()
     A = idrs_amd.Matrix(K)
     A.set_precon(1)
     idrs_amd.real_preconditioner(A, 2, d, foci=(0.15, 1.85))
@@ -392,6 +396,7 @@ def bench_multishift(args):
     n = crs.n
     b = torch.from_numpy(bh).to("cuda:0")
     K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+    group = K.autotune() if not args.no_autotune else None
     A = idrs_amd.Matrix(K)
     tol, maxit = 1e-8, args.ms_maxit
     poly = args.ms_problem == "laplacian"
@@ -460,7 +465,7 @@ def bench_multishift(args):
                    "msidrs4_ms": round(ms_idrs, 2), "msidrs4_iterations": rm.iterations, "msidrs4_flag": rm.flag,
                    "msqmridr8_ms": round(ms_qmr, 2), "msqmridr8_iterations": rq.iterations, "msqmridr8_flag": rq.flag,
                    "per_shift_loop_ms": round(loop_ms, 2), "per_shift_iterations": loop_its,
-                   "speedup_vs_loop": round(loop_ms / best, 2)},
+                   "speedup_vs_loop": round(loop_ms / best, 2), "lanes_per_row": group},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -576,7 +581,7 @@ def main():
         last = results[-1]
         assert last.flag == 0, f"solve did not converge: flag {last.flag}"
         # the SpMV roofline from one more solve, outside the timed steps, with
-        # HIP events around every SpMV launch on the solver's stream
+        # HIP events on the solver's stream around each operator application
         results = [step(True)]
         print(f"# {fmt} event-timed solve: spmv={results[0].spmv_ms:.1f} ms / {results[0].spmv_launches} launches",
               file=sys.stderr, flush=True)

@@ -53,6 +53,7 @@ Sell::~Sell() {
     (void)hipFree(sl_col);
     (void)hipFree(sl_perm);
     (void)hipFree(sl_val);
+    (void)hipFree(sl_val32);
 }
 
 Partition::~Partition() {
@@ -67,6 +68,7 @@ Operator::~Operator() {
     (void)hipFree(sval);
     (void)hipFree(sval32);
     (void)hipFree(sl_sval);
+    (void)hipFree(sl_sval32);
 }
 
 SpmvTimer &spmv_timer() {
@@ -83,9 +85,12 @@ double SpmvTimer::total_ms() const {
     return tot;
 }
 
-static void timer_mark(hipStream_t st) {
+// an event on the stream; the closing mark of an interval adds the launches
+// it holds
+static void timer_mark(hipStream_t st, int launches = 0) {
     SpmvTimer &t = spmv_timer();
     if (!t.enabled) return;
+    t.counted += launches;
     if (t.used >= (int)t.ev.size()) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
@@ -757,7 +762,8 @@ template <class VT, class T>
 static void use_sliced(const Operator &op, SpmvArgs<VT, T> &A) {
     const DevMatrix &K = *op.K;
     if (K.fmt != DevFmt::CRS || K.group != 1 || !K.sell) return;
-    const void *v = op.sval ? op.sl_sval : K.sell->sl_val;
+    constexpr bool lo = std::is_same<VT, float>::value;
+    const void *v = op.sval ? (lo ? op.sl_sval32 : op.sl_sval) : (lo ? K.sell->sl_val32 : K.sell->sl_val);
     if (!v) return;
     A.val = (const VT *)v;
     A.sl_ptr = K.sell->sl_ptr;
@@ -796,6 +802,7 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
             if (has_seed) throw std::invalid_argument("seed shift with single-precision values is not supported");
             auto A = base_args<float, T>(K, v32);
             if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
+            use_sliced(op, A);
             A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
             A.has_seed = 0;
             A.sdiag = nullptr;
@@ -810,7 +817,7 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
         if (!vals) throw std::invalid_argument("single-precision values not prepared");
         auto A = base_args<R, T>(K, vals);
         if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
-        if constexpr (!single) use_sliced(op, A);
+        use_sliced(op, A);
         A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
         A.has_seed = has_seed;
         A.sdiag = op.which_precon == 1 ? (const R *)op.pdiag : nullptr;
@@ -831,26 +838,21 @@ void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const
               const int *info, hipStream_t st, const TileCtx *tc) {
     const int d = op.degree;
     EpiArgs<T> none{};
+    timer_mark(st);
     if (d == 0) {
-        timer_mark(st);
         op_spmv<T>(op, v, T(1), nullptr, T(0), w, epi, E, info, st, tc);
-        timer_mark(st);
+        timer_mark(st, 1);
         return;
     }
     // Horner: w = alpha_d v; for i = d-1..0: w = K'(w) + alpha_i v; then K'(w)
     T *cur = w1, *nxt = w2;
-    timer_mark(st);
     op_spmv<T>(op, v, coef<T>(op, d), v, coef<T>(op, d - 1), cur, EPI_NONE, none, info, st, tc);
-    timer_mark(st);
     for (int i = d - 2; i >= 0; i--) {
-        timer_mark(st);
         op_spmv<T>(op, cur, T(1), v, coef<T>(op, i), nxt, EPI_NONE, none, info, st, tc);
-        timer_mark(st);
         std::swap(cur, nxt);
     }
-    timer_mark(st);
     op_spmv<T>(op, cur, T(1), nullptr, T(0), w, epi, E, info, st, tc);
-    timer_mark(st);
+    timer_mark(st, d + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,7 +1018,8 @@ static void *sliced_copy(const Sell &S, const void *src, bool cplx, hipStream_t 
 static void refresh_sliced(DevMatrix &K, hipStream_t st) {
     if (!K.sell) return;
     (void)hipFree(K.sell->sl_val);
-    K.sell->sl_val = nullptr;
+    (void)hipFree(K.sell->sl_val32);
+    K.sell->sl_val = K.sell->sl_val32 = nullptr;
     K.sell->sl_val = sliced_copy(*K.sell, K.val, K.kind == ValKind::Complex, st);
 }
 
@@ -1398,8 +1401,9 @@ void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shi
     (void)hipFree(op.sval);
     (void)hipFree(op.sval32);
     (void)hipFree(op.sl_sval);
+    (void)hipFree(op.sl_sval32);
     op.pdiag = op.sval = op.sval32 = nullptr;
-    op.sl_sval = nullptr;
+    op.sl_sval = op.sl_sval32 = nullptr;
     op.unit_diag = -1;
     op.which_precon = which;
     if (which != 1) return;
@@ -1482,6 +1486,20 @@ void op_make_lowprec(Operator &op) {
         hipLaunchKernelGGL((k_convert<double, float>), dim3(gv), dim3(256), 0, st, cnt, (const double *)K.val,
                            (float *)K.val32);
         LAUNCH_CHECK();
+    }
+    // float copies of the sliced values (group 1)
+    if (K.fmt == DevFmt::CRS && K.sell) {
+        op_prepare(op, st);
+        const long ne = K.sell->nent;
+        const int gs = vec_grid(ne);
+        void *&dst = op.sval ? op.sl_sval32 : K.sell->sl_val32;
+        const void *src = op.sval ? op.sl_sval : K.sell->sl_val;
+        if (!dst && src) {
+            HIP_TRY(hipMalloc(&dst, sizeof(float) * (ne ? ne : 1)));
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(gs), dim3(256), 0, st, ne, (const double *)src,
+                               (float *)dst);
+            LAUNCH_CHECK();
+        }
     }
     HIP_TRY(hipStreamSynchronize(st));
 }

@@ -62,6 +62,7 @@ struct Sell {
     int *sl_col = nullptr;         // nent (padding: 0, never read)
     int *sl_perm = nullptr;        // nent: CRS entry of each sliced entry, -1 for padding
     void *sl_val = nullptr;        // K's values (double or c128) in sliced order
+    void *sl_val32 = nullptr;      // float copy of sl_val (op_make_lowprec)
     ~Sell();
 };
 
@@ -111,6 +112,7 @@ struct Operator {
     int unit_diag = -1;
     // sval in K's sliced order (K.sell), made by op_prepare
     mutable void *sl_sval = nullptr;
+    mutable void *sl_sval32 = nullptr;   // float copy of sl_sval (op_make_lowprec)
     int degree = 0;
     int poly_kind = POLY_NONE;
     std::vector<std::complex<double>> alpha;   // alpha_p(0:degree)
@@ -141,13 +143,16 @@ struct EpiArgs {
 };
 
 // Kernel timing (events around the dominant SpMV launches) for bench.
+// HIP events around each operator application (its d+1 back-to-back SpMV
+// launches, so one event pair's overhead is shared by all of them)
 struct SpmvTimer {
     bool enabled = false;
     std::vector<hipEvent_t> ev;
     int used = 0;
-    void reset() { used = 0; }
+    int counted = 0;   // SpMV launches inside the timed intervals
+    void reset() { used = counted = 0; }
     double total_ms() const;
-    int launches() const { return used / 2; }
+    int launches() const { return counted; }
     ~SpmvTimer();
 };
 SpmvTimer &spmv_timer();

@@ -35,12 +35,16 @@ def test_mixed_reaches_fp64_tolerance(gpu, s, variant, fp32_vectors):
     assert rel(r.x, f.x) < 1e-8
 
 
+@pytest.mark.parametrize("group", [None, 1])
 @pytest.mark.parametrize("fp32_vectors", [False, True])
-def test_mixed_chebyshev_operator(gpu, fp32_vectors):
-    """The pp_idrs operator (diagonal scaling + Chebyshev(8)) on FP32 values inside."""
+def test_mixed_chebyshev_operator(gpu, fp32_vectors, group):
+    """The pp_idrs operator (diagonal scaling + Chebyshev(8)) on FP32 values
+    inside; group 1: the FP32 copies of the sliced values."""
     crs, _ = P.convection_diffusion_3d(24, 6.2, reaction=1.0)
     b = np.random.default_rng(0).uniform(-1, 1, crs.n)
     A = gpu_crs(crs, diag_precon=True, poly=(2, 8, (0.15, 1.85)))
+    if group:
+        A.K.set_group(group)
     r = idrs_amd.idrs_mixed(A, b, 4, tolerance=1e-8, maximum_iterations=3000, inner_tolerance=1e-4,
                             fp32_vectors=fp32_vectors)
     f = idrs_amd.idrs(A, b, 4, tolerance=1e-8, maximum_iterations=3000)
