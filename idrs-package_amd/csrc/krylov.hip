@@ -17,6 +17,7 @@
 #include "krylov.hpp"
 
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -251,35 +252,67 @@ __global__ __launch_bounds__(256) void k_qmr_update(long n, int ns, int s, int h
     }
 }
 
+// Reduces the partials and writes the nv values straight into pinned host
+// memory, then the call's sequence number: the host waits for that number
+// instead of a copy and a stream synchronisation.  The values are made
+// visible system-wide before the number (fence, then a release store).
 template <class T>
-__global__ __launch_bounds__(FIN_THREADS) void k_red(const T *partials, int nparts, int nv, T *out) {
+__global__ __launch_bounds__(FIN_THREADS) void k_red_host(const T *partials, int nparts, int nv, T *out,
+                                                          unsigned *seq_out, unsigned seq) {
     __shared__ T red[MAX_RED];
     reduce_partials(partials, nparts, nv, red);
     if (threadIdx.x < nv) out[threadIdx.x] = red[threadIdx.x];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---------------------------------------------------------------------------
-// host-side workspace: reductions with a pinned result buffer
+// host-side workspace: reductions whose results land in pinned host memory
 // ---------------------------------------------------------------------------
 template <class T>
 struct Work {
     using H = typename HostT<T>::type;
+    static constexpr size_t NCOEF = (size_t)MAXS * (SMAX + 4);
     hipStream_t st;
     long n;
     int gv;
-    T *part = nullptr, *dres = nullptr, *hres = nullptr, *dcoef = nullptr, *hcoef = nullptr;
+    T *part = nullptr, *dcoef = nullptr, *hcoef = nullptr;
+    T *hres = nullptr, *hres_d = nullptr;      // coherent pinned result slots (host / device view)
+    unsigned *hseq = nullptr, *hseq_d = nullptr;
+    unsigned seq = 0;
+    int cslot = 0;                             // hcoef half the next put_coef fills
     std::vector<void *> bufs;
     explicit Work(long n_, hipStream_t s) : st(s), n(n_), gv(vec_grid(n_)) {
         part = alloc(MAX_NV * (size_t)MAX_PARTS);
-        dres = alloc(MAX_RED);
-        dcoef = alloc((size_t)MAXS * (SMAX + 4));
-        HIP_TRY(hipHostMalloc((void **)&hres, sizeof(T) * MAX_RED, hipHostMallocDefault));
-        HIP_TRY(hipHostMalloc((void **)&hcoef, sizeof(T) * MAXS * (SMAX + 4), hipHostMallocDefault));
+        dcoef = alloc(NCOEF);
+        HIP_TRY(hipHostMalloc((void **)&hres, sizeof(T) * MAX_RED, hipHostMallocCoherent | hipHostMallocMapped));
+        HIP_TRY(hipHostMalloc((void **)&hseq, 64, hipHostMallocCoherent | hipHostMallocMapped));
+        *hseq = 0;
+        HIP_TRY(hipHostGetDevicePointer((void **)&hres_d, hres, 0));
+        HIP_TRY(hipHostGetDevicePointer((void **)&hseq_d, hseq, 0));
+        HIP_TRY(hipHostMalloc((void **)&hcoef, sizeof(T) * 2 * NCOEF, hipHostMallocDefault));
     }
     ~Work() {
+        (void)hipStreamSynchronize(st);
         for (void *p : bufs) (void)hipFree(p);
         (void)hipHostFree(hres);
+        (void)hipHostFree(hseq);
         (void)hipHostFree(hcoef);
+    }
+    // waits for k_red_host's sequence number; after a second of spinning
+    // (long queued work) it waits on the stream instead
+    void wait_seq(unsigned q) {
+        const auto t0 = std::chrono::steady_clock::now();
+        long spins = 0;
+        while (__atomic_load_n(hseq, __ATOMIC_ACQUIRE) != q) {
+            if ((++spins & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+                HIP_TRY(hipStreamSynchronize(st));
+                if (__atomic_load_n(hseq, __ATOMIC_ACQUIRE) != q)
+                    throw std::runtime_error("reduction result did not reach the host");
+                break;
+            }
+        }
     }
     T *alloc(size_t count) {
         void *p = nullptr;
@@ -294,17 +327,23 @@ struct Work {
     }
     // reduce nv values of partials laid out [v][nparts] and bring them to the host
     std::vector<H> get(int nv, int nparts) {
-        hipLaunchKernelGGL((k_red<T>), dim3(1), dim3(FIN_THREADS), 0, st, part, nparts, nv, dres);
+        const unsigned q = ++seq;
+        hipLaunchKernelGGL((k_red_host<T>), dim3(1), dim3(FIN_THREADS), 0, st, part, nparts, nv, hres_d, hseq_d, q);
         LAUNCH_CHECK();
-        HIP_TRY(hipMemcpyAsync(hres, dres, sizeof(T) * nv, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
+        wait_seq(q);
         std::vector<H> out(nv);
         for (int i = 0; i < nv; i++) out[i] = to_host(hres[i]);
         return out;
     }
+    // coefficients to the device; the two halves of hcoef alternate so a
+    // half is rewritten only after the copy that read it has run (a get()
+    // separates consecutive calls)
     void put_coef(const std::vector<H> &c) {
-        for (size_t i = 0; i < c.size(); i++) hcoef[i] = to_dev(c[i]);
-        HIP_TRY(hipMemcpyAsync(dcoef, hcoef, sizeof(T) * c.size(), hipMemcpyHostToDevice, st));
+        if (c.size() > NCOEF) throw std::invalid_argument("too many coefficients");
+        T *h = hcoef + (size_t)cslot * NCOEF;
+        cslot ^= 1;
+        for (size_t i = 0; i < c.size(); i++) h[i] = to_dev(c[i]);
+        HIP_TRY(hipMemcpyAsync(dcoef, h, sizeof(T) * c.size(), hipMemcpyHostToDevice, st));
     }
     // y = sum c_i src_i; returns ||y|| when want_norm
     double comb(T *y, const std::vector<const T *> &src, const std::vector<H> &cof, bool want_norm) {
