@@ -107,7 +107,19 @@ int vec_grid(long n) {
     return (int)b;
 }
 
+// blocks of a one-thread-per-row product (DIA, sliced CRS): one per 256 rows
+// up to 16384, a multiple of the XCD count -- many short row slabs keep HBM
+// busy to the end of the launch (epilogue partials: one per block)
+static int wide_grid(long nrows) {
+    long b = (nrows + SPMV_THREADS - 1) / SPMV_THREADS;
+    if (b > 16384) b = 16384;
+    if (b < 1) b = 1;
+    if (b >= 64) b = (b / NUM_XCD) * NUM_XCD;
+    return (int)b;
+}
+
 int spmv_grid(const DevMatrix &K) {
+    if (K.fmt != DevFmt::CRS || K.group == 1) return wide_grid(K.nrows);
     long rows_per_unit = (K.fmt == DevFmt::CRS && K.group > 1) ? (long)(SPMV_THREADS / K.group) * 4 : SPMV_THREADS;
     long units = (K.nrows + rows_per_unit - 1) / rows_per_unit;
     long b = units;
@@ -401,11 +413,14 @@ template <class VT, class T, int EPI, int NV, int ND>
 __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, EpiArgs<T> E, const int *info) {
     if (info && *info >= 0) return;
     constexpr int PC = EPI == EPI_PDOT ? NV : 1;
-    __shared__ int s_off[MAX_DIAG];
+    static_assert(ND <= 8, "known diagonal counts take their offsets by value");
+    __shared__ int s_off[ND > 0 ? 1 : MAX_DIAG];
     __shared__ T s_red[NV * (SPMV_THREADS / WAVE)];
     const int ndiag = ND > 0 ? ND : A.ndiag;
-    for (int d = threadIdx.x; d < ndiag; d += SPMV_THREADS) s_off[d] = A.offsets[d];
-    __syncthreads();
+    if constexpr (ND == 0) {
+        for (int d = threadIdx.x; d < ndiag; d += SPMV_THREADS) s_off[d] = A.offsets[d];
+        __syncthreads();
+    }
     T e[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) e[i] = T(0);
@@ -417,6 +432,21 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
     const VT *__restrict__ val = A.val;
     const int ud = A.ud;
     const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
+    // diagonal d is stored at d - (d > ud); the implicit unit diagonal
+    // (d == ud) enters as fmav(1, x, acc), the same operation as a stored
+    // 1.0.  With a known diagonal count the offsets, value streams and x
+    // bases are wave-uniform (scalar registers), rows index them in 32 bits.
+    int offr[ND > 0 ? ND : 1];
+    const VT *vb[ND > 0 ? ND : 1];
+    const T *xb[ND > 0 ? ND : 1];
+    if constexpr (ND > 0) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            offr[d] = A.off8[d];
+            vb[d] = val + (size_t)(d - (d > ud ? 1 : 0)) * (size_t)n;
+            xb[d] = x + offr[d];
+        }
+    }
     for (long c = c0; c < c1; c++) {
         const long row = c * SPMV_THREADS + threadIdx.x;
         if (row >= n) continue;
@@ -424,19 +454,16 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
         const T zq = zp[row];
         T pe[PC];
         if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
-        // diagonal d is stored at d - (d > ud); the implicit unit diagonal
-        // (d == ud) enters as fmav(1, x, acc), the same operation as a
-        // stored 1.0
         if constexpr (ND > 0) {
+            const int r32 = (int)row;
             VT vv[ND];
             T xx[ND];
 #pragma unroll
             for (int d = 0; d < ND; d++) {
-                const long cc = row + s_off[d];
+                const long cc = row + offr[d];
                 const bool ok = cc >= 0 && cc < ncols;
-                const long sd = d - (d > ud ? 1 : 0);
-                vv[d] = !ok ? VT(0) : (d == ud ? VT(1) : ldnt(&val[sd * n + row]));
-                xx[d] = ok ? x[cc] : T(0);
+                vv[d] = !ok ? VT(0) : (d == ud ? VT(1) : ldnt(vb[d] + r32));
+                xx[d] = ok ? xb[d][r32] : T(0);
             }
 #pragma unroll
             for (int d = 0; d < ND; d++) acc = fmav(vv[d], xx[d], acc);
@@ -526,7 +553,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_sell(SpmvArgs<VT, T> A, E
 template <class VT, class T, int EPI, int NV>
 static void launch_spmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T> E, const int *info,
                         hipStream_t st) {
-    int grid = spmv_grid(K);
+    const int grid = spmv_grid(K);
     E.nparts = grid;
     dim3 gd(grid), bd(SPMV_THREADS);
     if (K.fmt == DevFmt::CRS && A.sl_ptr) {
@@ -750,6 +777,7 @@ static SpmvArgs<VT, T> base_args(const DevMatrix &K, const VT *val) {
     A.val = val;
     A.ndiag = K.ndiag;
     A.offsets = K.offsets;
+    for (int d = 0; d < 8; d++) A.off8[d] = d < (int)K.h_offsets.size() ? K.h_offsets[d] : 0;
     A.ud = INT_MAX;
     A.sl_ptr = nullptr;
     A.sl_col = nullptr;

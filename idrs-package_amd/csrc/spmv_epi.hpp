@@ -35,6 +35,7 @@ struct SpmvArgs {
     const VT *val;
     int ndiag;
     const int *offsets;
+    int off8[8];       // DIA with ndiag <= 8: the offsets by value (kernel argument registers)
     int ud;            // DIA: index of an implicit unit diagonal (not in val), INT_MAX if none
     const long *sl_ptr;   // CRS: sliced copy (val in sliced order), nullptr: CRS order
     const int *sl_col;
