@@ -684,8 +684,8 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_crs(SpmvArgs<VT, T> A, E
     else epi_store<T, EPI, NV>(e, E, s_red);
 }
 
-// tile form of k_spmv_sell: a tile starts at any row, so a wavefront may
-// span two slices and each lane reads its own slice's start and width
+// tile form of k_spmv_sell: a tile starts at any row; each wavefront walks
+// whole slices and masks the rows outside the tile
 template <class VT, class T, int EPI, int NV>
 __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_sell(SpmvArgs<VT, T> A, EpiArgs<T> E, TileArgs ta) {
     const int tile = blockIdx.x;
@@ -697,8 +697,15 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_sell(SpmvArgs<VT, T> A, 
     for (int i = 0; i < NV; i++) e[i] = T(0);
     const long c0 = ta.tile_r0[tile], c1 = ta.tile_r1[tile];
     const T *zp = A.z ? A.z : A.y;
-    for (long row = c0 + threadIdx.x; row < c1; row += SPMV_THREADS) {
-        const long slice = row / SELL_C;
+    // wavefront w takes the slices q0 + w, q0 + w + 4, ... that overlap the
+    // tile (one slice per wavefront: uniform start and width); lanes whose
+    // row lies outside the tile idle
+    const int lane = threadIdx.x % WAVE;
+    const long q0 = c0 / SELL_C, q1 = (c1 + SELL_C - 1) / SELL_C;
+    for (long q = q0 + threadIdx.x / WAVE; q < q1; q += SPMV_THREADS / WAVE) {
+        const int slice = __builtin_amdgcn_readfirstlane((int)q);
+        const long row = (long)slice * SELL_C + lane;
+        if (row < c0 || row >= c1) continue;
         const long sp = A.sl_ptr[slice];
         const int w = (int)((A.sl_ptr[slice + 1] - sp) / SELL_C);
         const T zq = zp[row];

@@ -65,6 +65,19 @@ template <class T, int EPI, int PC>
 __device__ inline void epi_load(T (&pe)[PC], const EpiArgs<T> &E, long row, int c, int step) {
     if constexpr (EPI == EPI_PDOT) {
         const T *p = E.P + row * E.prow;
+        if constexpr (std::is_same<T, double>::value && PC % 2 == 0) {
+            // a whole, even-length P row: 16-byte loads (the row starts
+            // 16-byte aligned)
+            if (step == 1 && c == 0 && E.prow == PC) {
+#pragma unroll
+                for (int k = 0; k < PC; k += 2) {
+                    const double2 v = *(const double2 *)(p + k);
+                    pe[k] = v.x;
+                    pe[k + 1] = v.y;
+                }
+                return;
+            }
+        }
         const int last = E.prow - 1;
 #pragma unroll
         for (int k = 0; k < PC; k++) pe[k] = p[min(c + k * step, last)];
