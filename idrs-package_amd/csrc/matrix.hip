@@ -431,7 +431,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
     const T *__restrict__ x = A.x;
     const VT *__restrict__ val = A.val;
     const int ud = A.ud;
-    const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
+    const T *zp = A.z;   // Horner vector; products without one read nothing here
     // diagonal d is stored at d - (d > ud); the implicit unit diagonal
     // (d == ud) enters as fmav(1, x, acc), the same operation as a stored
     // 1.0.  With a known diagonal count the offsets, value streams and x
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
         const long row = c * SPMV_THREADS + threadIdx.x;
         if (row >= n) continue;
         T acc = T(0);
-        const T zq = zp[row];
+        const T zq = zp ? zp[row] : T(0);
         T pe[PC];
         if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
         if constexpr (ND > 0) {
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_sell(SpmvArgs<VT, T> A, E
     const long n = A.nrows;
     const long nchunk = (n + SPMV_THREADS - 1) / SPMV_THREADS;
     const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
-    const T *zp = A.z ? A.z : A.y;   // valid address of nrows entries; ignored without z
+    const T *zp = A.z;   // Horner vector; products without one read nothing here
     const int wv = threadIdx.x / WAVE;
     for (long c = c0; c < c1; c++) {
         // the wavefront's slice (uniform: chunks start at multiples of SELL_C)
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_sell(SpmvArgs<VT, T> A, E
         if (row >= n) continue;
         const long sp = A.sl_ptr[slice];
         const int w = (int)((A.sl_ptr[slice + 1] - sp) / SELL_C);
-        const T zq = zp[row];
+        const T zq = zp ? zp[row] : T(0);
         T pe[PC];
         if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
         const T acc = sell_row(A, row, sp + (row % SELL_C), w);
@@ -696,7 +696,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_sell(SpmvArgs<VT, T> A, 
 #pragma unroll
     for (int i = 0; i < NV; i++) e[i] = T(0);
     const long c0 = ta.tile_r0[tile], c1 = ta.tile_r1[tile];
-    const T *zp = A.z ? A.z : A.y;
+    const T *zp = A.z;
     // wavefront w takes the slices q0 + w, q0 + w + 4, ... that overlap the
     // tile (one slice per wavefront: uniform start and width); lanes whose
     // row lies outside the tile idle
@@ -708,7 +708,7 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_bspmv_sell(SpmvArgs<VT, T> A, 
         if (row < c0 || row >= c1) continue;
         const long sp = A.sl_ptr[slice];
         const int w = (int)((A.sl_ptr[slice + 1] - sp) / SELL_C);
-        const T zq = zp[row];
+        const T zq = zp ? zp[row] : T(0);
         T pe[PC];
         if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
         const T acc = sell_row(A, row, sp + (row % SELL_C), w);
