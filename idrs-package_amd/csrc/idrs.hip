@@ -403,6 +403,17 @@ __global__ __launch_bounds__(256) void k_omega_update(long n, const State<T> *st
         x[q] = fmaT(om, vq, x[q]);
         r[q] = rq;
         e[0] = e[0] + T(Tr<T>::abs2(rq));
+        if constexpr (std::is_same<T, double>::value && NV % 2 == 0) {
+            if (prow == NV) {   // whole, even-length P row: 16-byte loads
+#pragma unroll
+                for (int i = 0; i < NV; i += 2) {
+                    const double2 p2 = *(const double2 *)(P + q * NV + i);
+                    e[1 + i] = fmaT(p2.x, rq, e[1 + i]);
+                    e[2 + i] = fmaT(p2.y, rq, e[2 + i]);
+                }
+                continue;
+            }
+        }
 #pragma unroll
         for (int i = 0; i < NV; i++)
             if (i < prow) {
