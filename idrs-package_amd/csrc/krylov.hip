@@ -156,22 +156,6 @@ __global__ void k_div_vec(long n, const T *__restrict__ v, const T *__restrict__
     KGRID(q, n) w[q] = kdiv(v[q], d[q]);
 }
 
-// gn = gn - beta*Gi, then partials of conj(Gn) gn (next MGS dot) or |gn|^2
-template <class T>
-__global__ __launch_bounds__(256) void k_mgs(long n, T *gn, const T *__restrict__ Gi, T beta, const T *__restrict__ Gn,
-                                             T *partials) {
-    __shared__ T lds[4];
-    T e[1] = {T(0)};
-    KGRID(q, n) {
-        const T g = kfma(-beta, Gi[q], gn[q]);
-        gn[q] = g;
-        if (Gn) e[0] = kfma(Tr<T>::conj(Gn[q]), g, e[0]);
-        else e[0] = e[0] + T(Tr<T>::abs2(g));
-    }
-    block_sum<T, 1>(e, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = e[0];
-}
-
 // r = (b - t) + sigma*x, partials |r|^2 (msidrs.inc:318, msqmridr.inc:273-279)
 template <class T>
 __global__ __launch_bounds__(256) void k_shift_res(long n, const T *__restrict__ b, const T *__restrict__ t,
@@ -252,6 +236,42 @@ __global__ __launch_bounds__(256) void k_qmr_update(long n, int ns, int s, int h
     }
 }
 
+// Reduces the partials into device memory (a scalar a later kernel reads)
+template <class T>
+__global__ __launch_bounds__(FIN_THREADS) void k_red_dev(const T *partials, int nparts, int nv, T *out) {
+    __shared__ T red[MAX_RED];
+    reduce_partials(partials, nparts, nv, red);
+    if (threadIdx.x < nv) out[threadIdx.x] = red[threadIdx.x];
+}
+
+// Copies nv device scalars into pinned host memory, then the call's
+// sequence number (as k_red_host)
+template <class T>
+__global__ void k_fetch_host(const T *src, int nv, T *out, unsigned *seq_out, unsigned seq) {
+    if (threadIdx.x < nv) out[threadIdx.x] = src[threadIdx.x];
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// MGS step with beta read from device memory: gn -= beta Gi, then the
+// partials of Gn^H gn (or |gn|^2 without Gn)
+template <class T>
+__global__ __launch_bounds__(256) void k_mgs_d(long n, T *gn, const T *__restrict__ Gi, const T *beta_d,
+                                               const T *__restrict__ Gn, T *partials) {
+    __shared__ T lds[4];
+    const T beta = *beta_d;
+    T e[1] = {T(0)};
+    KGRID(q, n) {
+        const T g = kfma(-beta, Gi[q], gn[q]);
+        gn[q] = g;
+        if (Gn) e[0] = kfma(Tr<T>::conj(Gn[q]), g, e[0]);
+        else e[0] = e[0] + T(Tr<T>::abs2(g));
+    }
+    block_sum<T, 1>(e, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = e[0];
+}
+
 // Reduces the partials and writes the nv values straight into pinned host
 // memory, then the call's sequence number: the host waits for that number
 // instead of a copy and a stream synchronisation.  The values are made
@@ -277,7 +297,7 @@ struct Work {
     hipStream_t st;
     long n;
     int gv;
-    T *part = nullptr, *dcoef = nullptr, *hcoef = nullptr;
+    T *part = nullptr, *dcoef = nullptr, *hcoef = nullptr, *dsc = nullptr;
     T *hres = nullptr, *hres_d = nullptr;      // coherent pinned result slots (host / device view)
     unsigned *hseq = nullptr, *hseq_d = nullptr;
     unsigned seq = 0;
@@ -286,6 +306,7 @@ struct Work {
     explicit Work(long n_, hipStream_t s) : st(s), n(n_), gv(vec_grid(n_)) {
         part = alloc(MAX_NV * (size_t)MAX_PARTS);
         dcoef = alloc(NCOEF);
+        dsc = alloc(MAX_RED);
         HIP_TRY(hipHostMalloc((void **)&hres, sizeof(T) * MAX_RED, hipHostMallocCoherent | hipHostMallocMapped));
         HIP_TRY(hipHostMalloc((void **)&hseq, 64, hipHostMallocCoherent | hipHostMallocMapped));
         *hseq = 0;
@@ -329,6 +350,32 @@ struct Work {
     std::vector<H> get(int nv, int nparts) {
         const unsigned q = ++seq;
         hipLaunchKernelGGL((k_red_host<T>), dim3(1), dim3(FIN_THREADS), 0, st, part, nparts, nv, hres_d, hseq_d, q);
+        LAUNCH_CHECK();
+        wait_seq(q);
+        std::vector<H> out(nv);
+        for (int i = 0; i < nv; i++) out[i] = to_host(hres[i]);
+        return out;
+    }
+    // sum of the partials into a device scalar (no host round trip)
+    void reduce_to(T *out, int nparts) {
+        hipLaunchKernelGGL((k_red_dev<T>), dim3(1), dim3(FIN_THREADS), 0, st, part, nparts, 1, out);
+        LAUNCH_CHECK();
+    }
+    // u^H w into a device scalar
+    void dot_to(const T *u, const T *v, T *out) {
+        Dots<T> d{};
+        d.nu = 1;
+        d.u[0] = u;
+        d.w[0] = v;
+        hipLaunchKernelGGL((k_dots<T>), dim3(gv), dim3(256), 0, st, n, d, part, gv);
+        LAUNCH_CHECK();
+        reduce_to(out, gv);
+    }
+    // nv device scalars to the host
+    std::vector<H> fetch(const T *src, int nv) {
+        if (nv > MAX_RED) throw std::invalid_argument("too many scalars");
+        const unsigned q = ++seq;
+        hipLaunchKernelGGL((k_fetch_host<T>), dim3(1), dim3(64), 0, st, src, nv, hres_d, hseq_d, q);
         LAUNCH_CHECK();
         wait_seq(q);
         std::vector<H> out(nv);
@@ -895,18 +942,21 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                 LAUNCH_CHECK();
                 for (int i = 0; i < s + 2; i++) h[i] = c[i] / om;         // :197
                 if (k < s + 1) {                                          // :198-210 (modified Gram-Schmidt)
+                    // the betas stay on the device (db[i], i = i0..s; db[s+1]
+                    // = |gn|^2); the host reads them once, after the sweep
                     const int i0 = s - k + 1;
-                    H beta = w.dots({{Gcol(i0), gn}})[0];
+                    T *db = w.dsc;
+                    w.dot_to(Gcol(i0), gn, db + i0);
                     for (int i = i0; i <= s; i++) {
-                        h[i] = h[i] + beta;
                         const T *Gn = i < s ? Gcol(i + 1) : nullptr;
-                        hipLaunchKernelGGL((k_mgs<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, Gcol(i), to_dev(beta),
-                                           Gn, w.part);
+                        hipLaunchKernelGGL((k_mgs_d<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, Gcol(i), db + i, Gn,
+                                           w.part);
                         LAUNCH_CHECK();
-                        H red = w.get(1, w.gv)[0];
-                        if (i < s) beta = red;
-                        else normg = std::sqrt(reh(red));
+                        w.reduce_to(db + i + 1, w.gv);
                     }
+                    const std::vector<H> bv = w.fetch(db + i0, s + 2 - i0);
+                    for (int i = i0; i <= s; i++) h[i] = h[i] + bv[i - i0];
+                    normg = std::sqrt(reh(bv[s + 1 - i0]));
                 } else {
                     normg = w.norm(gn);
                 }
