@@ -307,3 +307,27 @@ def test_sliced_copy_complex_operator_and_refresh(gpu):
     assert _lib.load().idrs_user_matrix_set_values_f64(K4.handle, ctypes.c_void_p(nv.ctypes.data), 0) == 0
     c5 = P.Crs(c3.n, c3.row_ptr, c3.col_ind, nv)
     assert np.all(np.abs(K4 * x - oracle_crs(c5).user_mv(x)) <= _bound(c5, x))
+
+
+@pytest.mark.parametrize("case", ["2d_five", "band_hb5"])
+def test_scaled_dia_unit_diagonal_other_counts(gpu, case):
+    """The implicit unit diagonal in the 5-diagonal kernel and in the
+    generic-count kernel (an 11-diagonal CDS band), against the oracle's
+    scaled operator."""
+    rng = np.random.default_rng(30)
+    if case == "2d_five":
+        _, dia = P.convection_diffusion_2d(37, 20.0, angle=0.7)
+        n, off, vals = dia.vals.shape[0], np.asarray(dia.offsets), dia.vals
+        K = idrs_amd.dia_format(n, n, off, rvals=vals)
+        Ao = orc.OracleMatrix(orc.DIA, n, dia=(off, vals), which_precon=1)
+    else:
+        n, hb = 4099, 5
+        band = rng.standard_normal((n, 2 * hb + 1))
+        band[:, hb] = 4.0 + rng.uniform(0, 1, n)
+        K = idrs_amd.cds_format(n, n, hb, rband=band)
+        Ao = orc.OracleMatrix(orc.CDS, n, cds=(hb, band), which_precon=1)
+        off = np.arange(-hb, hb + 1)
+    A = idrs_amd.Matrix(K, which_precon=1)
+    assert A.stored_values() == ((len(off) - 1) * n, True)
+    x = rng.standard_normal(n)
+    assert rel(A * x, Ao.matvec(x)) < 1e-14
