@@ -486,7 +486,7 @@ def main():
     ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
     ap.add_argument("--small-grid", type=int, default=32, help="grid side of the small form")
     ap.add_argument("--batch-maxit", type=int, default=4000)
-    ap.add_argument("--no-autotune", action="store_true", help="CRS: keep the default lanes per row")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default product layout")
     ap.add_argument("--no-loop", action="store_true", help="batched form: skip the per-system loop (profiling)")
     ap.add_argument("--inner-tol", type=float, default=1e-9, help="inner tolerance of the mixed form")
     ap.add_argument("--mixed-vectors", choices=["fp64", "fp32"], default="fp64",
@@ -543,13 +543,15 @@ def main():
     formats = ["dia", "crs"] if args.format == "both" else [args.format]
 
     def run(fmt):
-        group = None
         if fmt == "crs":
             K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
-            # lanes per row (1: sliced-ELL copy) chosen by timing
-            group = K.autotune() if not args.no_autotune else None
         else:
             K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
+        # product layout chosen by timing: lanes per row (1: sliced-ELL copy)
+        # and row slabs of one-thread-per-row products
+        if not args.no_autotune:
+            K.autotune()
+        group, row_slabs = K.layout()
         A = idrs_amd.Matrix(K)
         A.set_precon(1)
         idrs_amd.real_preconditioner(A, 2, d, foci=foci)
@@ -608,6 +610,7 @@ def main():
                 "kernel": f"k_spmv_{fmt}",
                 "bytes_per_launch": round(tot_bytes / launches),
                 "stored_values": nvals, "implicit_unit_diagonal": unit, "lanes_per_row": group,
+                "row_slabs": row_slabs,
                 "avg_launch_us": round(spmv_ms * 1e3 / launches, 2),
             },
             "solve_gbs_model": round(solve_gbs, 1),

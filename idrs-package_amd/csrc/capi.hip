@@ -485,6 +485,21 @@ int idrs_user_matrix_autotune(idrs_user_matrix_t K, int reps, int *group) {
     });
 }
 
+int idrs_user_matrix_layout(idrs_user_matrix_t K, int *group, int *row_slabs) {
+    return guarded([&] {
+        need(K != nullptr, "null handle");
+        const DevMatrix &M = *K->M;
+        if (group) *group = M.fmt == DevFmt::CRS ? M.group : 0;
+        if (row_slabs) *row_slabs = M.row_slabs;
+    });
+}
+int idrs_user_matrix_set_row_slabs(idrs_user_matrix_t K, int row_slabs) {
+    return guarded([&] {
+        need(K != nullptr, "null handle");
+        need(row_slabs == 4096 || row_slabs == 8192 || row_slabs == 16384, "row_slabs must be 4096, 8192 or 16384");
+        K->M->row_slabs = row_slabs;
+    });
+}
 int idrs_user_matrix_set_group(idrs_user_matrix_t K, int group) {
     return guarded([&] {
         need(K != nullptr, "null handle");

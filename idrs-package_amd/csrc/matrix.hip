@@ -110,16 +110,16 @@ int vec_grid(long n) {
 // blocks of a one-thread-per-row product (DIA, sliced CRS): one per 256 rows
 // up to 16384, a multiple of the XCD count -- many short row slabs keep HBM
 // busy to the end of the launch (epilogue partials: one per block)
-static int wide_grid(long nrows) {
+static int wide_grid(long nrows, int cap) {
     long b = (nrows + SPMV_THREADS - 1) / SPMV_THREADS;
-    if (b > 16384) b = 16384;
+    if (b > cap) b = cap;
     if (b < 1) b = 1;
     if (b >= 64) b = (b / NUM_XCD) * NUM_XCD;
     return (int)b;
 }
 
 int spmv_grid(const DevMatrix &K) {
-    if (K.fmt != DevFmt::CRS || K.group == 1) return wide_grid(K.nrows);
+    if (K.fmt != DevFmt::CRS || K.group == 1) return wide_grid(K.nrows, K.row_slabs);
     long rows_per_unit = (K.fmt == DevFmt::CRS && K.group > 1) ? (long)(SPMV_THREADS / K.group) * 4 : SPMV_THREADS;
     long units = (K.nrows + rows_per_unit - 1) / rows_per_unit;
     long b = units;
@@ -1259,13 +1259,15 @@ void op_prepare(const Operator &op, hipStream_t st) {
     HIP_TRY(hipStreamSynchronize(st));
 }
 
-// Autotuner of the CRS kernel's lanes per row: times the product with the
-// candidate group sizes around the average-row-length choice (reps launches
-// each, HIP events on the library stream) and keeps the fastest.  Explicit,
-// so results stay reproducible: a group size changes the order in which a
-// row's partial sums are combined, i.e. the rounding.
+// Autotuner of the product layout: the CRS kernel's lanes per row (the
+// candidates around the average-row-length choice, and the sliced copy),
+// then, for one-thread-per-row products, the number of row slabs (reps
+// launches each, HIP events on the library stream); keeps the fastest.
+// Explicit, so results stay reproducible: a group size changes the order in
+// which a row's partial sums are combined, a slab count the order of the
+// epilogue's block sums, i.e. the rounding.
 int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair<int, float>> *times) {
-    if (K.fmt != DevFmt::CRS || K.nrows < 1) return K.group;
+    if (K.nrows < 1) return K.group;
     const bool cplx = K.kind == ValKind::Complex;
     const size_t vsz = cplx ? sizeof(c128) : sizeof(double);
     void *x = nullptr, *y = nullptr;
@@ -1275,48 +1277,69 @@ int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a));
     HIP_TRY(hipEventCreate(&b));
-    const int g0 = K.group == 1 ? 2 : K.group;
-    const int gin = K.group;
-    int best = gin;
-    float best_ms = 0.f;
-    // the sliced copy is a candidate when its padding stays within 1/4 of
-    // the stored entries (it is built for the timing and kept if chosen)
-    bool sell_ok = false;
-    if (!K.sell && K.nnz > 0) build_sell(K, st);
-    if (K.sell) {
-        sell_ok = K.sell->nent <= K.nnz + K.nnz / 4;
-    }
-    try {
-        for (int g : {1, g0 / 2, g0, g0 * 2, g0 * 4}) {
-            if (g == 1 ? !sell_ok : (g < 2 || g > 64)) continue;
-            K.group = g;
-            if (cplx) user_spmv<c128>(K, (const c128 *)x, (c128 *)y, st);   // warm-up
+    const int gin = K.group, sin = K.row_slabs;
+    // ms per product with the current layout (one warm-up product first)
+    auto timed = [&]() {
+        if (cplx) user_spmv<c128>(K, (const c128 *)x, (c128 *)y, st);
+        else user_spmv<double>(K, (const double *)x, (double *)y, st);
+        HIP_TRY(hipEventRecord(a, st));
+        for (int r = 0; r < reps; r++) {
+            if (cplx) user_spmv<c128>(K, (const c128 *)x, (c128 *)y, st);
             else user_spmv<double>(K, (const double *)x, (double *)y, st);
-            HIP_TRY(hipEventRecord(a, st));
-            for (int r = 0; r < reps; r++) {
-                if (cplx) user_spmv<c128>(K, (const c128 *)x, (c128 *)y, st);
-                else user_spmv<double>(K, (const double *)x, (double *)y, st);
+        }
+        HIP_TRY(hipEventRecord(b, st));
+        HIP_TRY(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        return ms / reps;
+    };
+    int best = gin;
+    try {
+        if (K.fmt == DevFmt::CRS) {
+            const int g0 = K.group == 1 ? 2 : K.group;
+            float best_ms = 0.f;
+            // the sliced copy is a candidate when its padding stays within
+            // 1/4 of the stored entries (built for the timing, kept if chosen)
+            bool sell_ok = false;
+            if (!K.sell && K.nnz > 0) build_sell(K, st);
+            if (K.sell) sell_ok = K.sell->nent <= K.nnz + K.nnz / 4;
+            for (int g : {1, g0 / 2, g0, g0 * 2, g0 * 4}) {
+                if (g == 1 ? !sell_ok : (g < 2 || g > 64)) continue;
+                K.group = g;
+                const float ms = timed();
+                if (times) times->push_back({g, ms});
+                if (best_ms == 0.f || ms < best_ms) {
+                    best_ms = ms;
+                    best = g;
+                }
             }
-            HIP_TRY(hipEventRecord(b, st));
-            HIP_TRY(hipEventSynchronize(b));
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, a, b));
-            if (times) times->push_back({g, ms / reps});
-            if (best_ms == 0.f || ms < best_ms) {
-                best_ms = ms;
-                best = g;
+            K.group = best;
+            if (best != 1) K.sell.reset();
+        }
+        // row slabs of a one-thread-per-row product
+        if (K.fmt != DevFmt::CRS || K.group == 1) {
+            int best_s = K.row_slabs;
+            float best_ms = 0.f;
+            for (int sl : {4096, 8192, 16384}) {
+                K.row_slabs = sl;
+                const float ms = timed();
+                if (times) times->push_back({-sl, ms});
+                if (best_ms == 0.f || ms < best_ms) {
+                    best_ms = ms;
+                    best_s = sl;
+                }
             }
+            K.row_slabs = best_s;
         }
     } catch (...) {
         K.group = gin;
+        K.row_slabs = sin;
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         (void)hipFree(x);
         (void)hipFree(y);
         throw;
     }
-    K.group = best;
-    if (best != 1) K.sell.reset();
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     (void)hipFree(x);

@@ -76,6 +76,9 @@ struct DevMatrix {
     int *row_ptr = nullptr, *col = nullptr;
     int group = 8;           // lanes that share one row in the CRS kernel; 1: the sliced copy
     std::unique_ptr<Sell> sell;   // built for group 1 (autotune_group)
+    // most row slabs (workgroups) of a one-thread-per-row product (DIA, the
+    // sliced copy): 4096, 8192 or 16384, chosen by autotune_group
+    int row_slabs = 16384;
     // DIA
     int ndiag = 0;
     int *offsets = nullptr;
@@ -202,8 +205,9 @@ DevMatrix *make_coo(int nrows, int ncols, long nnz, const int *indx1, const int 
 DevMatrix *make_cds(int nrows, int ncols, int half_band, const void *band, ValKind kind);
 DevMatrix *make_dia(int nrows, int ncols, int ndiag, const int *offsets, const void *vals, ValKind kind);
 // times the CRS product for lanes-per-row choices around K.group, and the
-// sliced copy (group 1) when its padding is small, and keeps the fastest
-// (returns it; times: (group, ms per product) if not null)
+// sliced copy (group 1) when its padding is small, and keeps the fastest;
+// for one-thread-per-row products (DIA, the sliced copy) then the row-slab
+// count.  Returns the group; times: (group or -slabs, ms per product)
 int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair<int, float>> *times);
 // builds / drops K's sliced copy (group 1)
 void build_sell(DevMatrix &K, hipStream_t st);

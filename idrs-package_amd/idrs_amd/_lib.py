@@ -105,6 +105,8 @@ _PROTOS = {
     "idrs_user_matrix_destroy": (c_int, [c_void_p]),
     "idrs_user_matrix_autotune": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "idrs_user_matrix_set_group": (c_int, [c_void_p, c_int]),
+    "idrs_user_matrix_layout": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int)]),
+    "idrs_user_matrix_set_row_slabs": (c_int, [c_void_p, c_int]),
     "idrs_user_matrix_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int64), POINTER(c_int)]),
     "idrs_user_mv_f64": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_user_mv_c128": (c_int, [c_void_p, c_void_p, c_void_p]),

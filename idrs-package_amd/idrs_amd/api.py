@@ -80,6 +80,17 @@ class UserMatrix:
         1 builds the sliced copy."""
         check(_lib.load().idrs_user_matrix_set_group(self._h, int(group)))
 
+    def layout(self):
+        """(lanes per row (1: sliced copy, 0: DIA), row slabs) of the products
+        (idrs_user_matrix_layout)."""
+        g, sl = ctypes.c_int(), ctypes.c_int()
+        check(_lib.load().idrs_user_matrix_layout(self._h, ctypes.byref(g), ctypes.byref(sl)))
+        return g.value, sl.value
+
+    def set_row_slabs(self, row_slabs):
+        """Row slabs of one-thread-per-row products: 4096, 8192 or 16384."""
+        check(_lib.load().idrs_user_matrix_set_row_slabs(self._h, int(row_slabs)))
+
     def __mul__(self, v):
         """w = K*v (ruser_mv / cuser_mv, user_module.f90:197-241)."""
         v = np.asarray(v)

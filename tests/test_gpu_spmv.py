@@ -179,6 +179,7 @@ def test_autotune_keeps_results(gpu):
     K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, rval=crs.val)
     g = K.autotune(reps=3)
     assert g in (1, 2, 4, 8, 16, 32, 64)
+    assert K.layout()[0] == g and K.layout()[1] in (4096, 8192, 16384)
     x = np.random.default_rng(4).standard_normal(crs.n)
     w = K * x
     ref = oracle_crs(crs).user_mv(x)
@@ -331,3 +332,33 @@ def test_scaled_dia_unit_diagonal_other_counts(gpu, case):
     assert A.stored_values() == ((len(off) - 1) * n, True)
     x = rng.standard_normal(n)
     assert rel(A * x, Ao.matvec(x)) < 1e-14
+
+
+def test_row_slabs_keep_results(gpu):
+    """The row-slab count of one-thread-per-row products (autotuned or set)
+    changes only the order of the epilogue's block sums: DIA products stay
+    bit-identical, and a solve stays within the parity bar."""
+    crs, dia = P.convection_diffusion_3d(24, 6.2, reaction=1.0)
+    n = crs.n
+    K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
+    assert K.layout() == (0, 16384)
+    x = np.random.default_rng(5).standard_normal(n)
+    w0 = K * x
+    K.autotune(reps=3)
+    assert K.layout()[1] in (4096, 8192, 16384)
+    for sl in (4096, 8192, 16384):
+        K.set_row_slabs(sl)
+        assert np.array_equal(K * x, w0)
+    with pytest.raises(idrs_amd.IdrsError):
+        K.set_row_slabs(1000)
+    b = np.random.default_rng(0).uniform(-1, 1, n)
+    PT = P.gaussian_shadow(n, 4, seed=0)
+    poly = (2, 8, [0.15, 1.85])
+    Ao = oracle_crs(crs, diag_precon=True, poly=poly)
+    go = orc.idrs(Ao, b, 4, tolerance=0.0, maximum_iterations=30, PT_raw=PT)
+    for sl in (4096, 16384):
+        K.set_row_slabs(sl)
+        A = idrs_amd.Matrix(K, which_precon=1)
+        idrs_amd.real_preconditioner(A, *poly[:2], foci=poly[2])
+        g = idrs_amd.idrs(A, b, 4, tolerance=0.0, maximum_iterations=30, PT_raw=PT)
+        assert rel(g.x, go[0]) < 1e-10
