@@ -9,14 +9,21 @@
 //   polpre        matrix_module.f90:266-304;  operator(/) :306-400
 //   chebyshev / neumann / compute_alpha  matrix_module.f90:851-1057
 //
-// Design: CRS rows are handled by groups of G lanes (G a power of two near
-// the average row length), so a wavefront streams the values and column
-// indices of 64/G consecutive rows as one coalesced stretch; DIA/CDS rows
-// are one thread each, every diagonal a coalesced stream.  Workgroups take
-// contiguous slabs of rows and are remapped so that each XCD streams one
-// contiguous part of the matrix (L2 reuse of the x entries neighbouring rows
-// share).  The operator's follow-up work (Horner term, shadow-space dots,
-// omega dots, norms) is fused into the product as an epilogue.
+// Design:
+// - CRS rows are handled by groups of G lanes (G a power of two near the
+//   average row length), so a wavefront streams the values and column
+//   indices of 64/G consecutive rows as one coalesced stretch.
+// - Group 1 uses a sliced-ELL copy instead, one lane per row, with each
+//   slice of 64 rows stored entry-major.
+// - DIA/CDS rows are one thread each, every diagonal a coalesced stream.
+//   The stream bases are wave-uniform, and with diagonal scaling an all-ones
+//   main diagonal is implicit.
+// - Workgroups take contiguous slabs of rows and are remapped so that each
+//   XCD streams one contiguous part of the matrix (L2 reuse of the x entries
+//   neighbouring rows share).  One-thread-per-row products use up to 16384
+//   short slabs.
+// - The operator's follow-up work (Horner term, shadow-space dots, omega
+//   dots, norms) is fused into the product as an epilogue.
 #include "matrix.hpp"
 #include "spmv_epi.hpp"
 
