@@ -360,6 +360,110 @@ def bench_small(args):
     print(json.dumps(out))
 
 
+def bench_adjoint(args):
+    """Differentiable solve: random sparse nonsymmetric A (N = 1e6, 16
+    off-diagonal entries per row at uniform random columns, N(0,1) values,
+    diagonal 1.2 x row abs-sum), L = 0.5 ||x - x_t||^2, x = A^-1 b.  A step
+    is one forward IDR(4) solve to 1e-10, the loss and its backward (adjoint
+    IDR(4) on A^T to 1e-10 and the pattern gradient) through torch
+    autograd.  The gradients wrt the nonzero values and b are checked
+    against central differences along seeded random directions."""
+    import numpy as np
+    import torch
+
+    import idrs_amd
+    from idrs_amd import problems as P
+
+    t0 = time.time()
+    n = args.adj_n
+    crs = P.random_dominant(n, k=16, seed=0)
+    print(f"# system built: n={n} nnz={crs.nnz} ({time.time() - t0:.1f} s)", file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    system = idrs_amd.SparseSystem(n, crs.row_ptr, crs.col_ind, crs.val)
+    groups = None
+    if not args.no_autotune:
+        groups = (system.K.autotune(), system.KT.autotune())
+    rng = np.random.default_rng(1)
+    b = torch.tensor(rng.standard_normal(n), device=dev)
+    xt = torch.tensor(rng.standard_normal(n), device=dev)
+    vals0 = torch.tensor(crs.val, device=dev)
+    tol = 1e-10
+
+    def step():
+        v = vals0.clone().requires_grad_(True)
+        bb = b.clone().requires_grad_(True)
+        x = idrs_amd.idrs_solve(system, v, bb, s=4, tolerance=tol)
+        L = 0.5 * torch.sum((x - xt) ** 2)
+        L.backward()
+        return v.grad, bb.grad, float(L.detach())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        gv, gb, L0 = step()
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t_start) * 1e3 / args.steps
+    # forward alone, for the split
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    with torch.no_grad():
+        rf = idrs_amd.idrs(system.A, b, 4, tolerance=tol, maximum_iterations=2 * n)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t_start) * 1e3
+    print(f"# forward+backward step: {step_ms:.1f} ms (forward alone {fwd_ms:.1f} ms, {rf.iterations} its); "
+          f"lanes per row {groups}", file=sys.stderr, flush=True)
+
+    # the finite differences need x far more accurately than 1e-10 (its
+    # error enters divided by h): one step of FP64 iterative refinement,
+    # x = x1 + A^-1 (b - A x1), with the residual from an FP64 product
+    fd_flags = set()
+
+    def solution(v, bb):
+        with torch.no_grad():
+            system.set_values(v)
+            r1 = idrs_amd.idrs(system.A, bb.contiguous(), 4, tolerance=tol, maximum_iterations=4000)
+            res = bb.cpu().numpy() - system.A * r1.x.cpu().numpy()
+            r2 = idrs_amd.idrs(system.A, torch.from_numpy(res).to(dev), 4, tolerance=tol, maximum_iterations=4000)
+            fd_flags.update((r1.flag, r2.flag))
+            return r1.x + r2.x
+
+    # central differences along seeded random directions (values scaled by
+    # their magnitude): truncation O(h^2), solve error O(1e-20/h).  The loss
+    # difference is formed as 0.5 (x+ - x-).(x+ + x- - 2 x_t), which does
+    # not cancel two large losses against each other
+    h, worst, worst_pair = 1e-5, 0.0, None
+    vabs = vals0.abs()
+    drng = np.random.default_rng(2)
+    for k in range(args.fd_dirs):
+        dv = torch.tensor(drng.standard_normal(crs.nnz), device=dev) * vabs
+        db = torch.tensor(drng.standard_normal(n), device=dev)
+        xp, xm = solution(vals0 + h * dv, b + h * db), solution(vals0 - h * dv, b - h * db)
+        fd = float(0.5 * torch.dot(xp - xm, xp + xm - 2.0 * xt)) / (2 * h)
+        an = float(torch.dot(gv, dv) + torch.dot(gb, db))
+        err = abs(fd - an) / max(abs(an), 1e-30)
+        if err >= worst:
+            worst, worst_pair = err, (fd, an)
+    system.set_values(vals0)
+    print(f"# gradient vs central differences on {args.fd_dirs} directions: worst rel. err {worst:.2e} "
+          f"(fd {worst_pair[0]:.9e}, autograd {worst_pair[1]:.9e}); flags of the difference solves {sorted(fd_flags)}",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "differentiable solve: forward + adjoint IDR(4) step time (ms) at 1e-10",
+        "value": round(step_ms, 2), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"random nonsymmetric N={n}, 16 off-diagonals per row, diagonal 1.2 row abs-sum, "
+                               "L = 0.5||x - x_t||^2, forward IDR(4) and adjoint IDR(4) on A^T to 1e-10, "
+                               "torch autograd", "nnz": int(crs.nnz), "forward_ms": round(fwd_ms, 2),
+                   "forward_iterations": rf.iterations, "loss": L0, "fd_directions": args.fd_dirs,
+                   "fd_worst_rel_err": worst, "fd_solve_flags": sorted(fd_flags), "lanes_per_row": groups},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def bench_multishift(args):
     """Shifted Helmholtz systems (A - sigma_k I) x_k = b, k = 1..16, complex128:
     MSIDRS(4) and MSQMRIDR(8) (one Krylov space for all shifts) against a
@@ -481,10 +585,12 @@ def main():
     ap.add_argument("--degree", type=int, default=8)
     ap.add_argument("--format", choices=["both", "crs", "dia"], default="both",
                     help="storage: both (headline DIA, then CRS), or one of them")
-    ap.add_argument("--form", choices=["solve", "batched", "mixed", "multishift", "small"], default="solve",
+    ap.add_argument("--form", choices=["solve", "batched", "mixed", "multishift", "small", "adjoint"], default="solve",
                     help="solve: the 200^3 IDR(4)+Chebyshev solve (default); batched: many small systems")
     ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
     ap.add_argument("--small-grid", type=int, default=32, help="grid side of the small form")
+    ap.add_argument("--adj-n", type=int, default=1000000, help="rows of the adjoint form")
+    ap.add_argument("--fd-dirs", type=int, default=32, help="finite-difference directions of the adjoint form")
     ap.add_argument("--batch-maxit", type=int, default=4000)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default product layout")
     ap.add_argument("--no-loop", action="store_true", help="batched form: skip the per-system loop (profiling)")
@@ -519,6 +625,9 @@ def main():
         return
     if args.form == "small":
         bench_small(args)
+        return
+    if args.form == "adjoint":
+        bench_adjoint(args)
         return
 
     import numpy as np

@@ -39,14 +39,22 @@ class SparseSystem:
 
     def set_values(self, values):
         """values: contiguous float64 CUDA tensor of nnz entries (CRS order)."""
+        import torch
+
         if values.numel() != self.nnz:
             raise ValueError(f"expected {self.nnz} values, got {values.numel()}")
         v = values.detach().contiguous()
+        # the library reads v on its own stream: torch's pending work on v
+        # (and on anything else the call reads) must have finished
+        torch.cuda.current_stream().synchronize()
         lib = _lib.load()
         check(lib.idrs_user_matrix_set_values_f64(self.K.handle, ctypes.c_void_p(v.data_ptr()), 1))
         check(lib.idrs_crs_transpose_refresh(self.K.handle, self.KT.handle))
 
     def pattern_grad(self, lam, x, out):
+        import torch
+
+        torch.cuda.current_stream().synchronize()
         check(_lib.load().idrs_crs_pattern_grad_f64(self.K.handle, ctypes.c_void_p(lam.data_ptr()),
                                                     ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
                                                     1))

@@ -121,3 +121,24 @@ def test_autograd_gradients(gpu):
         fd = (loss(vals.detach() + h * dv, b.detach() + h * db) - loss(vals.detach() - h * dv, b.detach() - h * db)) / (2 * h)
         an = float(np.dot(gv, dv.cpu().numpy()) + np.dot(gb, db.cpu().numpy()))
         assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-3), (k, fd, an)
+
+
+def test_set_values_waits_for_torch(gpu):
+    """Values computed by torch just before set_values (still queued on
+    torch's stream) reach the matrix: set_values waits for them, and the
+    product afterwards is the one of the new values (1e6 rows, so the torch
+    kernels take long enough to matter)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    crs = P.random_dominant(1000000, k=16, seed=3)
+    sysm = idrs_amd.SparseSystem(crs.n, crs.row_ptr, crs.col_ind, crs.val)
+    sysm.K.set_group(1)
+    v0 = torch.tensor(crs.val, device=dev)
+    x = np.random.default_rng(6).standard_normal(crs.n)
+    for k, scale in enumerate((0.5, 2.0, -1.5, 0.75)):
+        v = v0 * scale + float(k)      # queued on torch's stream
+        sysm.set_values(v)
+        del v
+        c2 = P.Crs(crs.n, crs.row_ptr, crs.col_ind, crs.val * scale + float(k))
+        assert rel(sysm.K * x, c2.matvec(x)) < 1e-14, k
