@@ -491,8 +491,8 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
 
 // Sliced copy of a CRS matrix (group 1): one thread per row.  Entry k of the
 // row is at p0 + SELL_C k, so each load instruction of a wavefront reads
-// SELL_C consecutive values (or column indices).  Rows of up to 8 entries
-// issue all their value and index loads, then all x gathers; the row sum
+// SELL_C consecutive values (or column indices).  Entries go in chunks of 8:
+// all value and index loads of a chunk, then all its x gathers; the row sum
 // runs over k in CRS order (the reference's sequential row sum).
 template <class VT, class T>
 __device__ inline T sell_row(const SpmvArgs<VT, T> &A, long row, long p0, int w) {
@@ -501,24 +501,23 @@ __device__ inline T sell_row(const SpmvArgs<VT, T> &A, long row, long p0, int w)
     const T *__restrict__ x = A.x;
     const int len = A.rp[row + 1] - A.rp[row];
     T acc = T(0);
-    if (w <= 8) {
+    // chunks of 8 entries (w, the slice width, is wave-uniform): all value
+    // and index loads of a chunk, then its x gathers, then the in-order sum
+    for (int k0 = 0; k0 < w; k0 += 8) {
         VT vv[8];
         int cc[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const bool ok = k < len;
-            vv[k] = ok ? ldnt(&val[p0 + (long)SELL_C * k]) : VT(0);
-            cc[k] = ok ? ldnt(&scol[p0 + (long)SELL_C * k]) : 0;
+            const bool ok = k0 + k < len;
+            vv[k] = ok ? ldnt(&val[p0 + (long)SELL_C * (k0 + k)]) : VT(0);
+            cc[k] = ok ? ldnt(&scol[p0 + (long)SELL_C * (k0 + k)]) : 0;
         }
         T xx[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) xx[k] = k < len ? x[cc[k]] : T(0);
+        for (int k = 0; k < 8; k++) xx[k] = k0 + k < len ? x[cc[k]] : T(0);
 #pragma unroll
         for (int k = 0; k < 8; k++)
-            if (k < len) acc = fmav(vv[k], xx[k], acc);
-    } else {
-        for (int k = 0; k < len; k++)
-            acc = fmav(ldnt(&val[p0 + (long)SELL_C * k]), x[ldnt(&scol[p0 + (long)SELL_C * k])], acc);
+            if (k0 + k < len) acc = fmav(vv[k], xx[k], acc);
     }
     return acc;
 }
