@@ -598,6 +598,10 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
     HIP_TRY(hipEventRecord(e0, stream));
     Work<T> w(n, stream);
     T *G = w.zeros((size_t)n * s), *W = w.zeros((size_t)n * s * ns);
+    // columns of G by pointer: storing gn as a column swaps buffers (gn is
+    // rewritten before its next read) instead of copying
+    std::vector<T *> Gc(s);
+    for (int i = 0; i < s; i++) Gc[i] = G + (long)n * i;
     T *gn = w.alloc(n), *v = w.alloc(n), *v0 = w.zeros(n), *t = w.alloc(n);
     T *w1 = w.alloc(n), *w2 = w.alloc(n), *tmp = w.alloc(n), *P = w.alloc((size_t)n * prow);
     const T *m1 = m1_diag(o, n, w);
@@ -639,7 +643,7 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
                         for (int j = k; j <= i - 1; j++) g = g - MM(i, j) * gamma[j - 1];
                         g = g / MM(i, i);
                         gamma[i - 1] = g;
-                        src.push_back(G + (long)n * (i - 1));
+                        src.push_back(Gc[i - 1]);
                         cof.push_back(-g);
                     }
                     w.comb(v, src, cof, false);
@@ -651,7 +655,7 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
                     info = 3;
                     break;
                 }
-                w.copy(G + (long)n * (k - 1), gn);                        // :156
+                std::swap(Gc[k - 1], gn);                                 // G(:,k) = gn (:156)
                 apply_m1<T>(A, v, t, m1, tmp, w1, w2, w);                 // :160-164
                 f = w.axpy_pdot(gn, v, t, om, P, prow);                   // :165-168
                 f.resize(s, H(0.0));
@@ -663,7 +667,7 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
                     for (int j = 1; j <= i - 1; j++) a = a - MM(i, j) * alpha[j - 1];
                     a = a / MM(i, i);
                     alpha[i - 1] = a;
-                    src.push_back(G + (long)n * (i - 1));
+                    src.push_back(Gc[i - 1]);
                     cof.push_back(-a);
                     for (int q = k + 1; q <= s; q++) f[q - 1] = f[q - 1] - MM(q, i) * a;
                 }
@@ -855,7 +859,10 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
         auto SN = [&](int i, int ks) -> H & { return sn[(i - 1) + (size_t)(s + 2) * ks]; };
         H om = multishift ? H((double)sqrtf(2.0f)) : H(1.0);              // qmridr.inc:116, msqmridr.inc:117
         int jj = 0, ghead = 0, whead = 0;
-        auto Gcol = [&](int i) { return G + (long)n * ((ghead + i - 1) % s); };   // logical column i (1-based)
+        // columns of G by pointer (see msidrs_solve): logical column i (1-based)
+        std::vector<T *> Gc(s);
+        for (int i = 0; i < s; i++) Gc[i] = G + (long)n * i;
+        auto Gcol = [&](int i) { return Gc[(ghead + i - 1) % s]; };
         std::vector<H> coef((size_t)ns * (s + 4));
         std::vector<T> sig0;
         if (multishift && in_it > 0) {
@@ -885,7 +892,7 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                 for (int j = 1; j <= s - 1; j++)                          // :155-158
                     for (int i = 1; i <= s; i++) MQ(i, j) = MQ(i, j + 1);
                 for (int i = 1; i <= s; i++) MQ(i, s) = mu[i - 1];
-                w.copy(Gcol(1), gn);                                      // oldest slot becomes column s
+                std::swap(Gc[ghead], gn);                                 // oldest slot becomes column s
                 ghead = (ghead + 1) % s;
                 const T *vtil = vt;
                 if (in_it == 0) {                                         // :161-175
