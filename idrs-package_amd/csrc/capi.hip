@@ -496,7 +496,8 @@ int idrs_user_matrix_layout(idrs_user_matrix_t K, int *group, int *row_slabs) {
 int idrs_user_matrix_set_row_slabs(idrs_user_matrix_t K, int row_slabs) {
     return guarded([&] {
         need(K != nullptr, "null handle");
-        need(row_slabs == 4096 || row_slabs == 8192 || row_slabs == 16384, "row_slabs must be 4096, 8192 or 16384");
+        need(row_slabs == 4096 || row_slabs == 8192 || row_slabs == 16384 || row_slabs == 32768,
+             "row_slabs must be 4096, 8192, 16384 or 32768");
         K->M->row_slabs = row_slabs;
     });
 }

@@ -6,7 +6,7 @@
 namespace idrs {
 
 constexpr int SMAX = 16;             // largest shadow-space dimension s
-constexpr int MAX_PARTS = 16384;     // upper bound of blocks in any reduction grid
+constexpr int MAX_PARTS = 32768;     // upper bound of blocks in any reduction grid
 constexpr int MAX_NV = SMAX + 3;     // values reduced per block
 
 struct SolveStats {

@@ -20,7 +20,7 @@
 //   main diagonal is implicit.
 // - Workgroups take contiguous slabs of rows and are remapped so that each
 //   XCD streams one contiguous part of the matrix (L2 reuse of the x entries
-//   neighbouring rows share).  One-thread-per-row products use up to 16384
+//   neighbouring rows share).  One-thread-per-row products use up to 32768
 //   short slabs.
 // - The operator's follow-up work (Horner term, shadow-space dots, omega
 //   dots, norms) is fused into the product as an epilogue.
@@ -115,8 +115,8 @@ int vec_grid(long n) {
 }
 
 // blocks of a one-thread-per-row product (DIA, sliced CRS): one per 256 rows
-// up to 16384, a multiple of the XCD count -- many short row slabs keep HBM
-// busy to the end of the launch (epilogue partials: one per block)
+// up to the row-slab cap, a multiple of the XCD count -- many short row slabs
+// keep HBM busy to the end of the launch (epilogue partials: one per block)
 static int wide_grid(long nrows, int cap) {
     long b = (nrows + SPMV_THREADS - 1) / SPMV_THREADS;
     if (b > cap) b = cap;
@@ -1326,7 +1326,7 @@ int autotune_group(DevMatrix &K, int reps, hipStream_t st, std::vector<std::pair
         if (K.fmt != DevFmt::CRS || K.group == 1) {
             int best_s = K.row_slabs;
             float best_ms = 0.f;
-            for (int sl : {4096, 8192, 16384}) {
+            for (int sl : {4096, 8192, 16384, 32768}) {
                 K.row_slabs = sl;
                 const float ms = timed();
                 if (times) times->push_back({-sl, ms});

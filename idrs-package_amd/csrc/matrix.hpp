@@ -77,8 +77,8 @@ struct DevMatrix {
     int group = 8;           // lanes that share one row in the CRS kernel; 1: the sliced copy
     std::unique_ptr<Sell> sell;   // built for group 1 (autotune_group)
     // most row slabs (workgroups) of a one-thread-per-row product (DIA, the
-    // sliced copy): 4096, 8192 or 16384, chosen by autotune_group
-    int row_slabs = 16384;
+    // sliced copy): 4096, 8192, 16384 or 32768, chosen by autotune_group
+    int row_slabs = 32768;
     // DIA
     int ndiag = 0;
     int *offsets = nullptr;

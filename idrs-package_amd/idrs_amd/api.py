@@ -88,7 +88,7 @@ class UserMatrix:
         return g.value, sl.value
 
     def set_row_slabs(self, row_slabs):
-        """Row slabs of one-thread-per-row products: 4096, 8192 or 16384."""
+        """Row slabs of one-thread-per-row products: 4096, 8192, 16384 or 32768."""
         check(_lib.load().idrs_user_matrix_set_row_slabs(self._h, int(row_slabs)))
 
     def __mul__(self, v):

@@ -82,8 +82,9 @@ int idrs_user_matrix_destroy(idrs_user_matrix_t A);
    at most 1/4 of the entries, with a sliced-ELL copy of the matrix (one
    lane per row, *group = 1), reps products each, and keeps the fastest
    for all later products of K; *group receives it.  For one-thread-per-row
-   products (DIA storage, the sliced copy) it then times 4096, 8192 and
-   16384 row slabs (workgroups) and keeps the fastest.  A different choice
+   products (DIA storage, the sliced copy) it then times 4096, 8192, 16384
+   and 32768 row slabs (workgroups, at most one per 256 rows) and keeps the
+   fastest.  A different choice
    changes the rounding order inside a row or of the epilogue's block sums,
    so the call is explicit; the sliced copy sums each row in CRS order like
    the reference's CRS product. */
@@ -91,8 +92,8 @@ int idrs_user_matrix_autotune(idrs_user_matrix_t K, int reps, int *group);
 /* The product layout of K (no counterpart in the reference): lanes per row
    (1: sliced copy; DIA: 0) and row slabs of one-thread-per-row products. */
 int idrs_user_matrix_layout(idrs_user_matrix_t K, int *group, int *row_slabs);
-/* Sets the row slabs of K's one-thread-per-row products: 4096, 8192 or
-   16384 (no counterpart in the reference; e.g. an earlier autotune's). */
+/* Sets the row slabs of K's one-thread-per-row products: 4096, 8192, 16384
+   or 32768 (no counterpart in the reference; e.g. an earlier autotune's). */
 int idrs_user_matrix_set_row_slabs(idrs_user_matrix_t K, int row_slabs);
 /* Sets the lanes per row of K's CRS products (no counterpart in the
    reference): 1 (sliced copy, built here), 2, 4, 8, 16, 32 or 64 -- e.g. a

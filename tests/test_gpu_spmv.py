@@ -179,7 +179,7 @@ def test_autotune_keeps_results(gpu):
     K = idrs_amd.crs_format(crs.n, crs.n, crs.row_ptr, crs.col_ind, rval=crs.val)
     g = K.autotune(reps=3)
     assert g in (1, 2, 4, 8, 16, 32, 64)
-    assert K.layout()[0] == g and K.layout()[1] in (4096, 8192, 16384)
+    assert K.layout()[0] == g and K.layout()[1] in (4096, 8192, 16384, 32768)
     x = np.random.default_rng(4).standard_normal(crs.n)
     w = K * x
     ref = oracle_crs(crs).user_mv(x)
@@ -341,12 +341,12 @@ def test_row_slabs_keep_results(gpu):
     crs, dia = P.convection_diffusion_3d(24, 6.2, reaction=1.0)
     n = crs.n
     K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
-    assert K.layout() == (0, 16384)
+    assert K.layout() == (0, 32768)
     x = np.random.default_rng(5).standard_normal(n)
     w0 = K * x
     K.autotune(reps=3)
-    assert K.layout()[1] in (4096, 8192, 16384)
-    for sl in (4096, 8192, 16384):
+    assert K.layout()[1] in (4096, 8192, 16384, 32768)
+    for sl in (4096, 8192, 16384, 32768):
         K.set_row_slabs(sl)
         assert np.array_equal(K * x, w0)
     with pytest.raises(idrs_amd.IdrsError):
