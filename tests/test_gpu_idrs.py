@@ -283,3 +283,27 @@ def test_full_size_200cube_formats_agree(gpu):
         del A, K
     assert out["dia"][0] == out["crs"][0]
     assert np.array_equal(out["dia"][1], out["crs"][1])
+
+
+@pytest.mark.parametrize("variant,s", [(3, 1), (3, 4), (4, 1), (4, 4)])
+def test_variants_3_and_4(gpu, variant, s):
+    """The one-row shadow-space variants (idrs.inc:109-125: variant 3 with
+    kappa = 0.7, variant 4 with kappa = 0) against the oracle: iteration
+    counts within +-2, and equal fixed-count solutions within 1e-10.  With
+    s > 1 the one-row shadow space makes M singular at the first step: both
+    stop there with flag 3."""
+    crs, _ = P.convection_diffusion_3d(16, 10.0, reaction=1.0)
+    b = np.random.default_rng(8).uniform(-1, 1, crs.n)
+    A, Ao = _pair(crs)
+    r = idrs_amd.idrs(A, b, s, tolerance=1e-9, maximum_iterations=3000, variant=variant)
+    xo, fo, ro, io, _ = orc.idrs(Ao, b, s, tolerance=1e-9, maximum_iterations=3000, variant=variant)
+    print(f"variant {variant} s={s}: gpu it={r.iterations} flag={r.flag}; oracle it={io} flag={fo}")
+    if s > 1:
+        assert r.flag == fo == 3 and r.iterations == io
+        return
+    assert r.flag == fo == 0 and abs(r.iterations - io) <= 2
+    k = min(r.iterations, io) // 2
+    g = idrs_amd.idrs(A, b, s, tolerance=0.0, maximum_iterations=k, variant=variant)
+    go = orc.idrs(Ao, b, s, tolerance=0.0, maximum_iterations=k, variant=variant)
+    assert g.iterations == go[3] == k
+    assert rel(g.x, go[0]) < 1e-10
