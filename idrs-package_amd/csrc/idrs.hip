@@ -270,6 +270,25 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_f(State<T> *st, const T *pa
 // ---------------------------------------------------------------------------
 // one-block reduction + scalar logic kernels
 // ---------------------------------------------------------------------------
+// First level of a two-level reduction of many block partials: block b sums
+// its contiguous share of every value's partials ([v][nparts] in, [v][PRE]
+// out), in a fixed order, so the one-workgroup kernel after it reads PRE.
+constexpr int PRE = 64;
+template <class T>
+__global__ __launch_bounds__(256) void k_prereduce(const State<T> *st, const T *partials, int nparts, int nv,
+                                                   T *out) {
+    if (st->info >= 0) return;
+    __shared__ T lds[4];
+    const long p0 = (long)nparts * blockIdx.x / PRE, p1 = (long)nparts * (blockIdx.x + 1) / PRE;
+    for (int v = 0; v < nv; v++) {
+        T e[1] = {T(0)};
+        for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x) e[0] = e[0] + partials[(long)v * nparts + p];
+        block_sum<T, 1>(e, lds);
+        if (threadIdx.x == 0) out[(long)v * PRE + blockIdx.x] = e[0];
+        __syncthreads();
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(FIN_THREADS) void k_fin_mu(int k, State<T> *st, const T *partials, int nparts) {
     if (st->info >= 0) return;
@@ -556,6 +575,7 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     const size_t vb = sizeof(T) * (size_t)n;
     Buf bG(vb * s), bU(vb * s), br(vb), bt(vb), bw1(vb), bw2(vb), btmp(vb), bP(vb * prow);
     Buf bpart(sizeof(T) * MAX_NV * MAX_PARTS), bst(sizeof(State<T>)), bres(sizeof(double) * (cap_it + 2));
+    Buf bpart2(sizeof(T) * MAX_NV * PRE);
     Buf bv, bm1, bom;
     T *G = bG.as<T>(), *U = bU.as<T>(), *r = br.as<T>(), *t = bt.as<T>(), *w1 = bw1.as<T>(), *w2 = bw2.as<T>();
     T *tmp = btmp.as<T>(), *P = bP.as<T>(), *part = bpart.as<T>();
@@ -618,13 +638,25 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     eom.partials = part;
     const int *info = &st->info;
     const int gs = spmv_grid(*A.K);
+    // the product epilogues' partials (one per row slab): above 4096 slabs a
+    // PRE-block first level runs before the one-workgroup scalar kernel
+    T *part2 = bpart2.as<T>();
+    const bool pre = gs > 4096;
+    const T *epart = pre ? part2 : part;
+    const int enparts = pre ? PRE : gs;
+    auto prereduce = [&](int nv) {
+        if (!pre) return;
+        hipLaunchKernelGGL((k_prereduce<T>), dim3(PRE), dim3(256), 0, stream, st, part, gs, nv, part2);
+        LAUNCH_CHECK();
+    };
 
     auto enqueue_cycle = [&]() {
         hipLaunchKernelGGL((k_form_u<T>), dim3(gv), dim3(256), 0, stream, n, 0, st, r, G, U, m1);
         LAUNCH_CHECK();
         for (int k = 0; k < s; k++) {
             op_apply<T>(A, U + (long)k * n, G + (long)k * n, w1, w2, EPI_PDOT, epd, info, stream);
-            hipLaunchKernelGGL((k_fin_mu<T>), dim3(1), dim3(FIN_THREADS), 0, stream, k, st, part, gs);
+            prereduce(prow);
+            hipLaunchKernelGGL((k_fin_mu<T>), dim3(1), dim3(FIN_THREADS), 0, stream, k, st, epart, enparts);
             LAUNCH_CHECK();
             hipLaunchKernelGGL((k_orth_update<T>), dim3(gv), dim3(256), 0, stream, n, k, st, G, U, r, x, m1, part);
             LAUNCH_CHECK();
@@ -636,7 +668,8 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
             LAUNCH_CHECK();
         }
         op_apply<T>(A, v, t, w1, w2, EPI_OMEGA, eom, info, stream);
-        hipLaunchKernelGGL((k_fin_om<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gs, omega_d);
+        prereduce(3);
+        hipLaunchKernelGGL((k_fin_om<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, epart, enparts, omega_d);
         LAUNCH_CHECK();
         launch_omega_update<T>(n, st, v, t, r, x, P, prow, conjP, part, stream);
         hipLaunchKernelGGL((k_fin_cycle<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv, resvec);
