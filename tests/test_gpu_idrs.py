@@ -280,6 +280,14 @@ def test_full_size_200cube_formats_agree(gpu):
         true = np.linalg.norm(bh - A * xh) / np.linalg.norm(bh)
         assert true <= 1.01e-8 and abs(true - r.relres) <= 1e-3 * r.relres
         out[fmt] = (r.iterations, xh)
+        if fmt == "dia":
+            # the mixed-precision solve at this size (FP32 values, FP64
+            # vectors, two-level reductions) meets the FP64 tolerance
+            m = idrs_amd.idrs_mixed(A, b, 4, tolerance=1e-8, maximum_iterations=2000, inner_tolerance=1e-9)
+            mh = m.x.cpu().numpy()
+            print(f"mixed: inner it={m.iterations} outer={m.outer} relres={m.relres:.3e}")
+            assert m.flag == 0 and np.linalg.norm(bh - A * mh) <= 1.01e-8 * np.linalg.norm(bh)
+            assert np.linalg.norm(mh - xh) <= 1e-6 * np.linalg.norm(xh)
         del A, K
     assert out["dia"][0] == out["crs"][0]
     assert np.array_equal(out["dia"][1], out["crs"][1])
