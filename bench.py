@@ -611,8 +611,9 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and not measured.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): sharding the rows over GPUs "
+              "(halo exchange per product, one all-reduce per reduction point) is designed, not built or "
+              "measured.", file=sys.stderr)
         sys.exit(2)
     if args.form == "batched":
         bench_batched(args)
