@@ -349,6 +349,21 @@ static int matvec_entry(idrs_matrix_t A, const double *v, double *w, int what) {
 }
 
 template <class T>
+static int matvec_block_entry(idrs_matrix_t A, int ncols, const double *V, double *W) {
+    return guarded([&] {
+        need(A && V && W, "null argument");
+        need(ncols >= 1, "ncols must be at least 1");
+        need(A->op.K->nrows == A->op.K->ncols, "block products need a square matrix");
+        hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
+        const size_t cnt = (size_t)A->op.K->nrows * ncols;
+        DVec<T> dv((const T *)V, cnt, st), dw(cnt), w1(cnt), w2(cnt);
+        op_apply_cols<T>(A->op, ncols, A->op.K->nrows, dv.p, dw.p, w1.p, w2.p, st);
+        dw.download((T *)W, st);
+    });
+}
+
+template <class T>
 static int ms_scaleback_entry(idrs_matrix_t A, int ns, int degree, const double *alpha_s, const double *Y, double *X,
                               int device) {
     return guarded([&] {
@@ -357,19 +372,26 @@ static int ms_scaleback_entry(idrs_matrix_t A, int ns, int degree, const double 
         hipStream_t st = lib_stream();
         op_prepare(A->op, st);
         const long n = A->op.K->nrows;
-        DVec<T> dy(device ? 0 : n), dx(device ? 0 : n), w1(n), w2(n);
-        for (int k = 0; k < ns; k++) {
-            std::vector<std::complex<double>> a(degree + 1);
-            for (int i = 0; i <= degree; i++)
-                a[i] = {alpha_s[2 * ((long)k * (degree + 1) + i)], alpha_s[2 * ((long)k * (degree + 1) + i) + 1]};
-            const T *yk = (const T *)Y + (long)k * n;
-            T *xk = (T *)X + (long)k * n;
+        // blocks of up to 8 shifts share each product's read of the matrix
+        const int nb = std::min(ns, SPMV_COLS);
+        DVec<T> dy(device ? 0 : n * nb), dx(device ? 0 : n * nb), w1(n * nb), w2(n * nb);
+        std::vector<std::vector<std::complex<double>>> a;
+        for (int k0 = 0; k0 < ns; k0 += nb) {
+            const int m = std::min(nb, ns - k0);
+            a.assign(m, std::vector<std::complex<double>>(degree + 1));
+            for (int j = 0; j < m; j++)
+                for (int i = 0; i <= degree; i++) {
+                    const long q = 2 * ((long)(k0 + j) * (degree + 1) + i);
+                    a[j][i] = {alpha_s[q], alpha_s[q + 1]};
+                }
+            const T *yk = (const T *)Y + (long)k0 * n;
+            T *xk = (T *)X + (long)k0 * n;
             if (device) {
-                op_scaleback_with<T>(A->op, a, yk, xk, w1.p, w2.p, st);
+                op_scaleback_cols<T>(A->op, m, n, a, yk, xk, w1.p, w2.p, st);
             } else {
-                HIP_TRY(hipMemcpyAsync(dy.p, yk, sizeof(T) * n, hipMemcpyHostToDevice, st));
-                op_scaleback_with<T>(A->op, a, dy.p, dx.p, w1.p, w2.p, st);
-                HIP_TRY(hipMemcpyAsync(xk, dx.p, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipMemcpyAsync(dy.p, yk, sizeof(T) * n * m, hipMemcpyHostToDevice, st));
+                op_scaleback_cols<T>(A->op, m, n, a, dy.p, dx.p, w1.p, w2.p, st);
+                HIP_TRY(hipMemcpyAsync(xk, dx.p, sizeof(T) * n * m, hipMemcpyDeviceToHost, st));
             }
         }
         HIP_TRY(hipStreamSynchronize(st));
@@ -699,6 +721,12 @@ int idrs_matrix_get_alpha(idrs_matrix_t A, double *alpha, int maxlen, int *degre
 
 int idrs_matvec_f64(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<double>(A, v, w, 0); }
 int idrs_matvec_c128(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<c128>(A, v, w, 0); }
+int idrs_matvec_block_f64(idrs_matrix_t A, int ncols, const double *V, double *W) {
+    return matvec_block_entry<double>(A, ncols, V, W);
+}
+int idrs_matvec_block_c128(idrs_matrix_t A, int ncols, const double *V, double *W) {
+    return matvec_block_entry<c128>(A, ncols, V, W);
+}
 int idrs_precon_f64(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<double>(A, v, w, 1); }
 int idrs_precon_c128(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<c128>(A, v, w, 1); }
 int idrs_scaleback_f64(idrs_matrix_t A, const double *y, double *x) { return matvec_entry<double>(A, y, x, 2); }

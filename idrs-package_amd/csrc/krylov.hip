@@ -96,6 +96,26 @@ __global__ __launch_bounds__(256) void k_dots(long n, Dots<T> d, T *partials, in
         for (int i = 0; i < d.nu; i++) partials[(long)i * nparts + blockIdx.x] = e[i];
 }
 
+// row q of P (prow values, row-major) into pv: a whole, even-length row of
+// doubles in 16-byte loads (every such row starts 16-byte aligned), else one
+// value at a time (zero beyond prow)
+template <class T, int NV>
+__device__ inline void load_prow(const T *__restrict__ P, long q, int prow, T (&pv)[NV]) {
+    if constexpr (std::is_same<T, double>::value && NV % 2 == 0) {
+        if (prow == NV) {
+#pragma unroll
+            for (int i = 0; i < NV; i += 2) {
+                const double2 v2 = *(const double2 *)(P + q * NV + i);
+                pv[i] = v2.x;
+                pv[i + 1] = v2.y;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) pv[i] = i < prow ? P[q * prow + i] : T(0);
+}
+
 // partials of P^T w (no conjugation), prow rows
 template <class T, int NV>
 __global__ __launch_bounds__(256) void k_pdotk(long n, const T *__restrict__ w, const T *__restrict__ P, int prow,
@@ -106,9 +126,11 @@ __global__ __launch_bounds__(256) void k_pdotk(long n, const T *__restrict__ w, 
     for (int i = 0; i < NV; i++) e[i] = T(0);
     KGRID(q, n) {
         const T wq = w[q];
+        T pv[NV];
+        load_prow<T, NV>(P, q, prow, pv);
 #pragma unroll
         for (int i = 0; i < NV; i++)
-            if (i < prow) e[i] = kfma(P[q * prow + i], wq, e[i]);
+            if (i < prow) e[i] = kfma(pv[i], wq, e[i]);
     }
     block_sum<T, NV>(e, lds);
     if (threadIdx.x == 0)
@@ -126,9 +148,11 @@ __global__ __launch_bounds__(256) void k_axpy_pdot(long n, T *gn, const T *__res
     KGRID(q, n) {
         const T g = kfma(-om, t[q], v[q]);
         gn[q] = g;
+        T pv[NV];
+        load_prow<T, NV>(P, q, prow, pv);
 #pragma unroll
         for (int i = 0; i < NV; i++)
-            if (i < prow) e[i] = kfma(P[q * prow + i], g, e[i]);
+            if (i < prow) e[i] = kfma(pv[i], g, e[i]);
     }
     block_sum<T, NV>(e, lds);
     if (threadIdx.x == 0)
@@ -144,6 +168,20 @@ __global__ void k_sub_div(long n, T *gn, const T *__restrict__ v, T om) {
 // w = w / c for a real c
 template <class T>
 __global__ void k_div_real(long n, T *w, double c) {
+    KGRID(q, n) {
+        if constexpr (Tr<T>::is_complex) w[q] = T(w[q].re / c, w[q].im / c);
+        else w[q] = w[q] / c;
+    }
+}
+
+// w = w / sqrt(re(nrm[0])) with the square root computed on the device;
+// nrm[1] receives it (the host reads it back with the other step scalars)
+__device__ inline double real_part(double v) { return v; }
+__device__ inline double real_part(c128 v) { return v.re; }
+template <class T>
+__global__ void k_norm_div(long n, T *w, T *nrm) {
+    const double c = sqrt(real_part(nrm[0]));
+    if (blockIdx.x == 0 && threadIdx.x == 0) nrm[1] = T(c);
     KGRID(q, n) {
         if constexpr (Tr<T>::is_complex) w[q] = T(w[q].re / c, w[q].im / c);
         else w[q] = w[q] / c;
@@ -307,7 +345,7 @@ struct Work {
         part = alloc(MAX_NV * (size_t)MAX_PARTS);
         dcoef = alloc(NCOEF);
         dsc = alloc(MAX_RED);
-        HIP_TRY(hipHostMalloc((void **)&hres, sizeof(T) * MAX_RED, hipHostMallocCoherent | hipHostMallocMapped));
+        HIP_TRY(hipHostMalloc((void **)&hres, sizeof(T) * 2 * MAX_RED, hipHostMallocCoherent | hipHostMallocMapped));
         HIP_TRY(hipHostMalloc((void **)&hseq, 64, hipHostMallocCoherent | hipHostMallocMapped));
         *hseq = 0;
         HIP_TRY(hipHostGetDevicePointer((void **)&hres_d, hres, 0));
@@ -345,6 +383,29 @@ struct Work {
         T *p = alloc(count);
         HIP_TRY(hipMemsetAsync(p, 0, sizeof(T) * count, st));
         return p;
+    }
+    // Posted transfers: each lands in its own region of the result slots
+    // (off 0 or MAX_RED), and one wait on the last post's sequence number
+    // covers all posts before it (stream order).
+    unsigned post_fetch(const T *src, int nv, int off) {
+        if (nv > MAX_RED) throw std::invalid_argument("too many scalars");
+        const unsigned q = ++seq;
+        hipLaunchKernelGGL((k_fetch_host<T>), dim3(1), dim3(64), 0, st, src, nv, hres_d + off, hseq_d, q);
+        LAUNCH_CHECK();
+        return q;
+    }
+    unsigned post_pdot(const T *w, const T *P, int prow, int off) {
+        launch_pdotk(w, P, prow);
+        const unsigned q = ++seq;
+        hipLaunchKernelGGL((k_red_host<T>), dim3(1), dim3(FIN_THREADS), 0, st, part, gv, prow, hres_d + off, hseq_d,
+                           q);
+        LAUNCH_CHECK();
+        return q;
+    }
+    std::vector<H> read(int nv, int off) const {
+        std::vector<H> out(nv);
+        for (int i = 0; i < nv; i++) out[i] = to_host(hres[off + i]);
+        return out;
     }
     // reduce nv values of partials laid out [v][nparts] and bring them to the host
     std::vector<H> get(int nv, int nparts) {
@@ -441,7 +502,7 @@ struct Work {
         LAUNCH_CHECK();
         return get(d.nu, gv);
     }
-    std::vector<H> pdot(const T *w, const T *P, int prow) {
+    void launch_pdotk(const T *w, const T *P, int prow) {
 #define PD(NV) hipLaunchKernelGGL((k_pdotk<T, NV>), dim3(gv), dim3(256), 0, st, n, w, P, prow, part, gv)
         if (prow <= 1) PD(1);
         else if (prow <= 2) PD(2);
@@ -450,6 +511,9 @@ struct Work {
         else PD(16);
 #undef PD
         LAUNCH_CHECK();
+    }
+    std::vector<H> pdot(const T *w, const T *P, int prow) {
+        launch_pdotk(w, P, prow);
         return get(prow, gv);
     }
     std::vector<H> axpy_pdot(T *gn, const T *v, const T *t, H om, const T *P, int prow) {
@@ -557,6 +621,37 @@ void apply_m1(const Operator &A, const T *v, T *w, const T *m1, T *tmp, T *w1, T
         op_apply<T>(A, tmp, w, w1, w2, EPI_NONE, none, nullptr, wk.st);
     } else {
         op_apply<T>(A, v, w, w1, w2, EPI_NONE, none, nullptr, wk.st);
+    }
+}
+
+// True residual norms |b - (A x_k - sigma_k x_k)| of the shifted systems
+// (A x = K'(x / M1) with M1), with A applied to the shifts' solutions as
+// block products (one read of the matrix per 8 shifts) and one host read of
+// the norms per block (msidrs.inc:312-320, msqmridr.inc:273-291).
+template <class T>
+void shifted_residuals(const Operator &A, long n, int ns, const std::vector<typename HostT<T>::type> &sigma,
+                       const T *b, const T *X, const T *m1, Work<T> &w, std::vector<double> &normr) {
+    const int nb = std::min(ns, SPMV_COLS);
+    T *Tb = w.alloc((size_t)n * nb), *W1 = w.alloc((size_t)n * nb), *W2 = w.alloc((size_t)n * nb);
+    T *Mb = m1 ? w.alloc((size_t)n * nb) : nullptr;
+    for (int k0 = 0; k0 < ns; k0 += nb) {
+        const int m = std::min(nb, ns - k0);
+        const T *src = X + n * k0;
+        if (m1) {
+            for (int j = 0; j < m; j++) {
+                hipLaunchKernelGGL((k_div_vec<T>), dim3(w.gv), dim3(256), 0, w.st, n, X + n * (k0 + j), m1, Mb + n * j);
+                LAUNCH_CHECK();
+            }
+            src = Mb;
+        }
+        op_apply_cols<T>(A, m, n, src, Tb, W1, W2, w.st);
+        for (int j = 0; j < m; j++) {
+            hipLaunchKernelGGL((k_shift_res<T>), dim3(w.gv), dim3(256), 0, w.st, n, b, Tb + n * j, X + n * (k0 + j),
+                               to_dev(sigma[k0 + j]), w.part + (long)j * w.gv);
+            LAUNCH_CHECK();
+        }
+        const std::vector<typename HostT<T>::type> r2 = w.get(m, w.gv);
+        for (int j = 0; j < m; j++) normr[k0 + j] = std::sqrt(reh(r2[j]));
     }
 }
 
@@ -755,13 +850,7 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
             }
         }
     }
-    for (int ks = 0; ks < ns; ks++) {                                     // :312-320
-        T *xk = X + (long)n * ks;
-        apply_m1<T>(A, xk, t, m1, tmp, w1, w2, w);
-        hipLaunchKernelGGL((k_shift_res<T>), dim3(w.gv), dim3(256), 0, stream, n, b, t, xk, to_dev(sigma[ks]), w.part);
-        LAUNCH_CHECK();
-        normr[ks] = std::sqrt(reh(w.get(1, w.gv)[0]));
-    }
+    shifted_residuals<T>(A, n, ns, sigma, b, X, m1, w, normr);            // :312-320
     if (m1 && !o.want_colfac)                                             // :323-327
         for (int ks = 0; ks < ns; ks++) {
             T *xk = X + (long)n * ks;
@@ -850,7 +939,8 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
         make_p_device<T>(n, s, 1, o.PT_raw, gn, P, o.p_seed, stream);     // :108
         hipLaunchKernelGGL((k_div_real<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, normg);   // :112
         LAUNCH_CHECK();
-        std::vector<H> M((size_t)s * s, H(0.0)), gamma(s), c(s + 2), h(s + 2), rs(s + 3);
+        std::vector<H> M((size_t)s * s, H(0.0)), gamma(s), c(s + 2), h(s + 2), rs(s + 3), mu;
+        bool mu_ready = false;
         std::vector<H> cs((size_t)(s + 2) * ns, H(0.0)), sn((size_t)(s + 2) * ns, H(0.0));
         std::vector<H> phn(ns, H(0.0)), phn1(ns, H(normg)), eta(ns, H(1.0));
         std::vector<char> converged(ns, 0);
@@ -875,7 +965,10 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                 iter = iter + 1;
                 for (auto &q : c) q = H(0.0);
                 c[s] = H(1.0);
-                std::vector<H> mu = w.pdot(gn, P, s);                     // :139-140
+                // P^T gn (:139-140): after the first step it came back with
+                // the previous step's norm
+                if (!mu_ready) mu = w.pdot(gn, P, s);
+                mu_ready = false;
                 if (iter > s) {
                     gamma = gepp<H>(s, M, mu);                            // :144-150
                     std::vector<const T *> src{gn};
@@ -948,27 +1041,37 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                 hipLaunchKernelGGL((k_sub_div<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, v, to_dev(om));  // :194
                 LAUNCH_CHECK();
                 for (int i = 0; i < s + 2; i++) h[i] = c[i] / om;         // :197
-                if (k < s + 1) {                                          // :198-210 (modified Gram-Schmidt)
-                    // the betas stay on the device (db[i], i = i0..s; db[s+1]
-                    // = |gn|^2); the host reads them once, after the sweep
-                    const int i0 = s - k + 1;
+                // The step's scalars stay on the device until one host read:
+                // the MGS betas db[i], i = i0..s (:198-205, modified
+                // Gram-Schmidt; none in the last step), |gn|^2 in db[s+1],
+                // normg = sqrt of it in db[s+2] (gn /= normg on the device,
+                // :206-208).  The next step's P^T gn is posted behind them,
+                // so both come back in one round trip.
+                {
                     T *db = w.dsc;
-                    w.dot_to(Gcol(i0), gn, db + i0);
-                    for (int i = i0; i <= s; i++) {
-                        const T *Gn = i < s ? Gcol(i + 1) : nullptr;
-                        hipLaunchKernelGGL((k_mgs_d<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, Gcol(i), db + i, Gn,
-                                           w.part);
-                        LAUNCH_CHECK();
-                        w.reduce_to(db + i + 1, w.gv);
+                    const int i0 = k < s + 1 ? s - k + 1 : s + 1;
+                    if (k < s + 1) {
+                        w.dot_to(Gcol(i0), gn, db + i0);
+                        for (int i = i0; i <= s; i++) {
+                            const T *Gn = i < s ? Gcol(i + 1) : nullptr;
+                            hipLaunchKernelGGL((k_mgs_d<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, Gcol(i), db + i,
+                                               Gn, w.part);
+                            LAUNCH_CHECK();
+                            w.reduce_to(db + i + 1, w.gv);
+                        }
+                    } else {
+                        w.dot_to(gn, gn, db + s + 1);
                     }
-                    const std::vector<H> bv = w.fetch(db + i0, s + 2 - i0);
+                    hipLaunchKernelGGL((k_norm_div<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, db + s + 1);
+                    LAUNCH_CHECK();
+                    w.post_fetch(db + i0, s + 3 - i0, 0);
+                    w.wait_seq(w.post_pdot(gn, P, s, MAX_RED));
+                    const std::vector<H> bv = w.read(s + 3 - i0, 0);
                     for (int i = i0; i <= s; i++) h[i] = h[i] + bv[i - i0];
-                    normg = std::sqrt(reh(bv[s + 1 - i0]));
-                } else {
-                    normg = w.norm(gn);
+                    normg = reh(bv[s + 2 - i0]);
+                    mu = w.read(s, MAX_RED);
+                    mu_ready = true;
                 }
-                hipLaunchKernelGGL((k_div_real<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, normg);    // :206-208
-                LAUNCH_CHECK();
                 h[s + 1] = normg;
                 bool broke = false;
                 for (int ks = 0; ks < ns; ks++) {                         // :211-243 / :217-259
@@ -1034,14 +1137,13 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
         }
     }
     // final true residuals (qmridr.inc:256-259; msqmridr.inc:273-291)
-    for (int ks = 0; ks < ns; ks++) {
-        T *xk = X + (long)n * ks;
-        if (multishift) apply_m1<T>(A, xk, t, m1, tmp, w1, w2, w);
-        else op_apply<T>(A, xk, t, w1, w2, EPI_NONE, none, nullptr, stream);
-        hipLaunchKernelGGL((k_shift_res<T>), dim3(w.gv), dim3(256), 0, stream, n, b, t, xk,
-                           to_dev(multishift ? sigma[ks] : H(0.0)), w.part);
+    if (multishift) {
+        shifted_residuals<T>(A, n, ns, sigma, b, X, m1, w, normr);
+    } else {
+        op_apply<T>(A, X, t, w1, w2, EPI_NONE, none, nullptr, stream);
+        hipLaunchKernelGGL((k_shift_res<T>), dim3(w.gv), dim3(256), 0, stream, n, b, t, X, T(0), w.part);
         LAUNCH_CHECK();
-        normr[ks] = std::sqrt(reh(w.get(1, w.gv)[0]));
+        normr[0] = std::sqrt(reh(w.get(1, w.gv)[0]));
     }
     if (multishift && m1)
         for (int ks = 0; ks < ns; ks++) {

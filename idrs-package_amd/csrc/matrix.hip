@@ -606,6 +606,125 @@ static void dispatch_epi(const DevMatrix &K, const SpmvArgs<VT, T> &A, const Epi
 }
 
 // ---------------------------------------------------------------------------
+// block products y_j = xs (K' x_j - seed x_j / sdiag) + a_j z_j over the
+// columns of a block (the multishift solvers' per-shift products and
+// scale-backs).  A row's entries are loaded once for all columns of the
+// launch; each column's row sum and finish are those of k_spmv_dia /
+// k_spmv_sell, so column j equals the one-column product of x_j bit for bit.
+// ---------------------------------------------------------------------------
+template <class VT, class T, int ND>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia_cols(SpmvArgs<VT, T> A, ColArgs<T> C) {
+    static_assert(ND > 0 && ND <= 8, "known diagonal counts only");
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const long n = A.nrows, ncols = A.ncols;
+    const long nchunk = (n + SPMV_THREADS - 1) / SPMV_THREADS;
+    const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
+    const int ud = A.ud;
+    for (long c = c0; c < c1; c++) {
+        const long row = c * SPMV_THREADS + threadIdx.x;
+        if (row >= n) continue;
+        VT vv[ND];
+        bool ok[ND];
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+            const long cc = row + A.off8[d];
+            ok[d] = cc >= 0 && cc < ncols;
+            const long sd = d - (d > ud ? 1 : 0);
+            vv[d] = !ok[d] ? VT(0) : (d == ud ? VT(1) : ldnt(A.val + sd * n + row));
+        }
+#pragma unroll
+        for (int j = 0; j < SPMV_COLS; j++) {
+            if (j < C.nc) {
+                const T *xj = C.x + j * C.ld;
+                T xx[ND];
+#pragma unroll
+                for (int d = 0; d < ND; d++) xx[d] = ok[d] ? xj[row + A.off8[d]] : T(0);
+                T acc = T(0);
+#pragma unroll
+                for (int d = 0; d < ND; d++) acc = fmav(vv[d], xx[d], acc);
+                const T *zj = C.z ? C.z + j * C.ld : nullptr;
+                const T zq = zj ? zj[row] : T(0);
+                C.y[j * C.ld + row] = finish_col(A, xj, row, acc, zj, zq, C.a[j]);
+            }
+        }
+    }
+}
+
+template <class VT, class T>
+__global__ __launch_bounds__(SPMV_THREADS) void k_spmv_sell_cols(SpmvArgs<VT, T> A, ColArgs<T> C) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const long n = A.nrows;
+    const long nchunk = (n + SPMV_THREADS - 1) / SPMV_THREADS;
+    const long c0 = nchunk * lb / gridDim.x, c1 = nchunk * (lb + 1) / gridDim.x;
+    const int wv = threadIdx.x / WAVE;
+    const VT *__restrict__ val = A.val;
+    const int *__restrict__ scol = A.sl_col;
+    for (long c = c0; c < c1; c++) {
+        const int slice = __builtin_amdgcn_readfirstlane((int)(c * (SPMV_THREADS / SELL_C) + wv));
+        const long row = c * SPMV_THREADS + threadIdx.x;
+        if (row >= n) continue;
+        const long sp = A.sl_ptr[slice];
+        const int w = (int)((A.sl_ptr[slice + 1] - sp) / SELL_C);
+        const long p0 = sp + (row % SELL_C);
+        const int len = A.rp[row + 1] - A.rp[row];
+        T acc[SPMV_COLS];
+#pragma unroll
+        for (int j = 0; j < SPMV_COLS; j++) acc[j] = T(0);
+        // the chunks of sell_row, each used for every column
+        for (int k0 = 0; k0 < w; k0 += 8) {
+            VT vv[8];
+            int cc[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const bool ok = k0 + k < len;
+                vv[k] = ok ? ldnt(&val[p0 + (long)SELL_C * (k0 + k)]) : VT(0);
+                cc[k] = ok ? ldnt(&scol[p0 + (long)SELL_C * (k0 + k)]) : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < SPMV_COLS; j++) {
+                if (j < C.nc) {
+                    const T *xj = C.x + j * C.ld;
+                    T xx[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) xx[k] = k0 + k < len ? xj[cc[k]] : T(0);
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        if (k0 + k < len) acc[j] = fmav(vv[k], xx[k], acc[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SPMV_COLS; j++) {
+            if (j < C.nc) {
+                const T *zj = C.z ? C.z + j * C.ld : nullptr;
+                const T zq = zj ? zj[row] : T(0);
+                C.y[j * C.ld + row] = finish_col(A, C.x + j * C.ld, row, acc[j], zj, zq, C.a[j]);
+            }
+        }
+    }
+}
+
+// one block product of at most SPMV_COLS columns; false when K's layout has
+// no block kernel (generic DIA, CRS with lanes per row)
+template <class VT, class T>
+static bool launch_cols(const DevMatrix &K, const SpmvArgs<VT, T> &A, const ColArgs<T> &C, hipStream_t st) {
+    dim3 gd(spmv_grid(K)), bd(SPMV_THREADS);
+    if (K.fmt == DevFmt::CRS) {
+        if (!A.sl_ptr) return false;
+        hipLaunchKernelGGL((k_spmv_sell_cols<VT, T>), gd, bd, 0, st, A, C);
+    } else {
+        switch (K.ndiag) {
+        case 3: hipLaunchKernelGGL((k_spmv_dia_cols<VT, T, 3>), gd, bd, 0, st, A, C); break;
+        case 5: hipLaunchKernelGGL((k_spmv_dia_cols<VT, T, 5>), gd, bd, 0, st, A, C); break;
+        case 7: hipLaunchKernelGGL((k_spmv_dia_cols<VT, T, 7>), gd, bd, 0, st, A, C); break;
+        default: return false;
+        }
+    }
+    LAUNCH_CHECK();
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // batched (tile) SpMV: one workgroup per tile of the block-diagonal matrix's
 // partition.  Same per-row arithmetic and group layout as k_spmv_crs; the
 // epilogue partials are per tile, and a tile whose system has finished
@@ -811,9 +930,11 @@ static void use_sliced(const Operator &op, SpmvArgs<VT, T> &A) {
     A.sl_col = K.sell->sl_col;
 }
 
-template <class T>
-void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int epi, const EpiArgs<T> &E,
-             const int *info, hipStream_t st, const TileCtx *tc) {
+// The argument block of one product of op: its values (complex, FP32 copy
+// or real; scaled or not), layout (sliced copy, implicit unit diagonal) and
+// seed shift.  f receives it and fills the vector fields.
+template <class T, class F>
+static void with_args(const Operator &op, F &&f) {
     const DevMatrix &K = *op.K;
     const bool cv = op.values_complex();
     const bool has_seed = op.seed != std::complex<double>(0.0, 0.0);
@@ -827,12 +948,10 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
             auto A = base_args<c128, T>(K, (const c128 *)op.values());
             if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
             use_sliced(op, A);
-            A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
             A.has_seed = has_seed;
             A.sdiag = op.which_precon == 1 ? (const c128 *)op.pdiag : nullptr;
             A.seed = to_T<T>(op.seed);
-            if (tc) dispatch_epi_tiles<c128, T>(K, A, E, epi, *tc, st);
-            else dispatch_epi<c128, T>(K, A, E, epi, info, st);
+            f(A);
         } else {
             throw std::invalid_argument("complex matrix values need complex128 vectors");
         }
@@ -844,12 +963,10 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
             auto A = base_args<float, T>(K, v32);
             if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
             use_sliced(op, A);
-            A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
             A.has_seed = 0;
             A.sdiag = nullptr;
             A.seed = T(0);
-            if (tc) dispatch_epi_tiles<float, T>(K, A, E, epi, *tc, st);
-            else dispatch_epi<float, T>(K, A, E, epi, info, st);
+            f(A);
         } else {
             throw std::invalid_argument("single-precision values with double vectors: real systems only");
         }
@@ -859,12 +976,47 @@ void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int ep
         auto A = base_args<R, T>(K, vals);
         if (op.sval && op.unit_diag >= 0) A.ud = op.unit_diag;
         use_sliced(op, A);
-        A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
         A.has_seed = has_seed;
         A.sdiag = op.which_precon == 1 ? (const R *)op.pdiag : nullptr;
         A.seed = to_T<T>(op.seed);
-        if (tc) dispatch_epi_tiles<R, T>(K, A, E, epi, *tc, st);
-        else dispatch_epi<R, T>(K, A, E, epi, info, st);
+        f(A);
+    }
+}
+
+template <class T>
+void op_spmv(const Operator &op, const T *x, T xs, const T *z, T a, T *y, int epi, const EpiArgs<T> &E,
+             const int *info, hipStream_t st, const TileCtx *tc) {
+    const DevMatrix &K = *op.K;
+    with_args<T>(op, [&](auto &A) {
+        A.x = x; A.xs = xs; A.z = z; A.a = a; A.y = y;
+        if (tc) dispatch_epi_tiles(K, A, E, epi, *tc, st);
+        else dispatch_epi(K, A, E, epi, info, st);
+    });
+}
+
+template <class T>
+void op_spmv_cols(const Operator &op, int nc, long ld, const T *x, T xs, const T *z, const T *a, T *y,
+                  hipStream_t st) {
+    const DevMatrix &K = *op.K;
+    EpiArgs<T> none{};
+    for (int j0 = 0; j0 < nc; j0 += SPMV_COLS) {
+        const int m = std::min(SPMV_COLS, nc - j0);
+        bool done = false;
+        with_args<T>(op, [&](auto &A) {
+            A.xs = xs;
+            ColArgs<T> C{};
+            C.nc = m;
+            C.ld = ld;
+            C.x = x + (long)j0 * ld;
+            C.z = z ? z + (long)j0 * ld : nullptr;
+            C.y = y + (long)j0 * ld;
+            for (int j = 0; j < m; j++) C.a[j] = a ? a[j0 + j] : T(0);
+            done = launch_cols(K, A, C, st);
+        });
+        if (!done)
+            for (int j = j0; j < j0 + m; j++)
+                op_spmv<T>(op, x + (long)j * ld, xs, z ? z + (long)j * ld : nullptr, a ? a[j] : T(0),
+                           y + (long)j * ld, EPI_NONE, none, nullptr, st);
     }
 }
 
@@ -894,6 +1046,27 @@ void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const
     }
     op_spmv<T>(op, cur, T(1), nullptr, T(0), w, epi, E, info, st, tc);
     timer_mark(st, d + 1);
+}
+
+template <class T>
+void op_apply_cols(const Operator &op, int nc, long ld, const T *V, T *W, T *W1, T *W2, hipStream_t st) {
+    const int d = op.degree;
+    if (d == 0) {
+        op_spmv_cols<T>(op, nc, ld, V, T(1), nullptr, nullptr, W, st);
+        return;
+    }
+    std::vector<T> a((size_t)nc);
+    auto all = [&](T c) {
+        std::fill(a.begin(), a.end(), c);
+        return a.data();
+    };
+    T *cur = W1, *nxt = W2;
+    op_spmv_cols<T>(op, nc, ld, V, coef<T>(op, d), V, all(coef<T>(op, d - 1)), cur, st);
+    for (int i = d - 2; i >= 0; i--) {
+        op_spmv_cols<T>(op, nc, ld, cur, T(1), V, all(coef<T>(op, i)), nxt, st);
+        std::swap(cur, nxt);
+    }
+    op_spmv_cols<T>(op, nc, ld, cur, T(1), nullptr, nullptr, W, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1181,6 +1354,35 @@ void op_scaleback_with(const Operator &op, const std::vector<std::complex<double
         p = cur;
     }
     op_precon<T>(op, p, x, st);
+}
+
+template <class T>
+void op_scaleback_cols(const Operator &op, int nc, long ld, const std::vector<std::vector<std::complex<double>>> &alpha,
+                       const T *Y, T *X, T *W1, T *W2, hipStream_t st) {
+    if ((int)alpha.size() < nc) throw std::invalid_argument("one coefficient set per column");
+    const int d = nc > 0 ? (int)alpha[0].size() - 1 : 0;
+    for (int j = 0; j < nc; j++)
+        if ((int)alpha[j].size() != d + 1) throw std::invalid_argument("coefficient sets of unequal degree");
+    const long n = op.K->nrows;
+    const T *P = Y;
+    if (d > 0) {
+        // polpre on every column: W = alpha_d y; for i = d-1..0: W = K'(W) + alpha_i y
+        const int g = vec_grid(n);
+        T *cur = W1, *nxt = W2;
+        for (int j = 0; j < nc; j++) {
+            hipLaunchKernelGGL((k_scale_copy<T>), dim3(g), dim3(256), 0, st, n, to_T<T>(alpha[j][d]), Y + (long)j * ld,
+                               cur + (long)j * ld);
+            LAUNCH_CHECK();
+        }
+        std::vector<T> a((size_t)nc);
+        for (int i = d - 1; i >= 0; i--) {
+            for (int j = 0; j < nc; j++) a[j] = to_T<T>(alpha[j][i]);
+            op_spmv_cols<T>(op, nc, ld, cur, T(1), Y, a.data(), nxt, st);
+            std::swap(cur, nxt);
+        }
+        P = cur;
+    }
+    for (int j = 0; j < nc; j++) op_precon<T>(op, P + (long)j * ld, X + (long)j * ld, st);
 }
 
 template <class T>
@@ -1716,6 +1918,11 @@ void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> 
     template void op_scaleback<T>(const Operator &, const T *, T *, T *, T *, T *, hipStream_t);             \
     template void op_scaleback_with<T>(const Operator &, const std::vector<std::complex<double>> &, const T *, T *,  \
                                        T *, T *, hipStream_t);                                                 \
+    template void op_spmv_cols<T>(const Operator &, int, long, const T *, T, const T *, const T *, T *, hipStream_t); \
+    template void op_apply_cols<T>(const Operator &, int, long, const T *, T *, T *, T *, hipStream_t);         \
+    template void op_scaleback_cols<T>(const Operator &, int, long,                                             \
+                                       const std::vector<std::vector<std::complex<double>>> &, const T *, T *,  \
+                                       T *, T *, hipStream_t);                                                  \
     template void user_spmv<T>(const DevMatrix &, const T *, T *, hipStream_t);
 INST(double)
 INST(c128)

@@ -189,6 +189,23 @@ template <class T>
 void op_scaleback_with(const Operator &op, const std::vector<std::complex<double>> &alpha, const T *y, T *x, T *w1,
                        T *w2, hipStream_t st);
 
+// Block forms over the nc columns of V, X, Y, W (column stride ld), one read
+// of the matrix per up to 8 columns; column j is bit-identical to the
+// one-column call on column j.  Layouts without a block kernel (generic DIA,
+// CRS with lanes per row) run the one-column product per column.
+//   op_spmv_cols:      y_j = xs * (K' x_j) + a[j] z_j  (z nullptr: no z term)
+//   op_apply_cols:     W = A V (op_apply per column); W1, W2: blocks like V
+//   op_scaleback_cols: X(:,j) = p_j(K') Y(:,j) / D, p_j's coefficients alpha[j]
+constexpr int SPMV_COLS = 8;   // columns per block-product launch
+template <class T>
+void op_spmv_cols(const Operator &op, int nc, long ld, const T *x, T xs, const T *z, const T *a, T *y,
+                  hipStream_t st);
+template <class T>
+void op_apply_cols(const Operator &op, int nc, long ld, const T *V, T *W, T *W1, T *W2, hipStream_t st);
+template <class T>
+void op_scaleback_cols(const Operator &op, int nc, long ld, const std::vector<std::vector<std::complex<double>>> &alpha,
+                       const T *Y, T *X, T *W1, T *W2, hipStream_t st);
+
 // plain user-matrix product w = K v (no scaling, no polynomial)
 template <class T>
 void user_spmv(const DevMatrix &K, const T *v, T *w, hipStream_t st);

@@ -49,15 +49,33 @@ struct SpmvArgs {
     T *y;
 };
 
-// y(row) = xs*(acc - seed*x(row)/sdiag(row)) + a*z(row); zq holds z(row)
-// (ignored without z)
+// y(row) = xs*(acc - seed*x(row)/sdiag(row)) + a*z(row) for the vectors x
+// and z (z nullptr: no z term); zq holds z(row)
 template <class VT, class T>
-__device__ inline T finish_value(const SpmvArgs<VT, T> &A, long row, T acc, T zq) {
-    if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(A.x[row], A.sdiag[row]) : A.x[row]);
+__device__ inline T finish_col(const SpmvArgs<VT, T> &A, const T *x, long row, T acc, const T *z, T zq, T a) {
+    if (A.has_seed) acc = acc - A.seed * (A.sdiag ? divv(x[row], A.sdiag[row]) : x[row]);
     T yv = A.xs * acc;
-    if (A.z) yv = Tr<T>::fma(A.a, zq, yv);
+    if (z) yv = Tr<T>::fma(a, zq, yv);
     return yv;
 }
+
+// the finish of a product of A.x (Horner vector A.z with coefficient A.a)
+template <class VT, class T>
+__device__ inline T finish_value(const SpmvArgs<VT, T> &A, long row, T acc, T zq) {
+    return finish_col(A, A.x, row, acc, A.z, zq, A.a);
+}
+
+// Block products: the columns j < nc of x, z and y (column stride ld), with
+// coefficient a[j] of z's column j.  One launch holds up to SPMV_COLS columns.
+template <class T>
+struct ColArgs {
+    int nc;
+    long ld;
+    const T *x;
+    const T *z;
+    T *y;
+    T a[SPMV_COLS];
+};
 
 // Operands of one row's epilogue: PC components of its P row (PDOT: this
 // lane's components c, c+step, ...), or r(row) (OMEGA).

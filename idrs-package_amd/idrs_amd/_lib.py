@@ -128,6 +128,8 @@ _PROTOS = {
     "idrs_compute_alpha_coeffs": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p]),
     "idrs_matvec_f64": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_matvec_c128": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "idrs_matvec_block_f64": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "idrs_matvec_block_c128": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "idrs_precon_f64": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_precon_c128": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_scaleback_f64": (c_int, [c_void_p, c_void_p, c_void_p]),

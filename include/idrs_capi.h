@@ -163,6 +163,14 @@ int idrs_matrix_get_alpha(idrs_matrix_t A, double *alpha, int maxlen, int *degre
 int idrs_matvec_f64(idrs_matrix_t A, const double *v, double *w);
 /* w = A*v: cmatvec, matrix_module.f90:135-150 */
 int idrs_matvec_c128(idrs_matrix_t A, const double *v, double *w);
+/* W(:,j) = A*V(:,j), j = 1..ncols (V, W: n x ncols, column-major): rmatvec /
+   cmatvec (matrix_module.f90:135-167) over a block of vectors, each column
+   bit-identical to idrs_matvec_* on it.  No single counterpart in the
+   reference, whose multishift solvers apply A once per shift
+   (msidrs.inc:312-320): here up to 8 columns share each product's read of
+   the matrix.  A must be square. */
+int idrs_matvec_block_f64(idrs_matrix_t A, int ncols, const double *V, double *W);
+int idrs_matvec_block_c128(idrs_matrix_t A, int ncols, const double *V, double *W);
 /* w = v/A: rprecon, matrix_module.f90:306-352 */
 int idrs_precon_f64(idrs_matrix_t A, const double *v, double *w);
 /* w = v/A: cprecon, matrix_module.f90:354-400 */

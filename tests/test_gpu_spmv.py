@@ -362,3 +362,107 @@ def test_row_slabs_keep_results(gpu):
         idrs_amd.real_preconditioner(A, *poly[:2], foci=poly[2])
         g = idrs_amd.idrs(A, b, 4, tolerance=0.0, maximum_iterations=30, PT_raw=PT)
         assert rel(g.x, go[0]) < 1e-10
+
+
+def _block_operator(case, rng):
+    if case == "dia7_scaled_cheb":
+        crs, dia = P.convection_diffusion_3d(12, 10.0, reaction=1.0)
+        n = crs.n
+        A = idrs_amd.Matrix(idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals), which_precon=1)
+        idrs_amd.real_preconditioner(A, 2, 8, foci=[0.15, 1.85])
+    elif case == "dia5_seed":
+        _, dia = P.convection_diffusion_2d(37, 20.0, angle=0.7)
+        n = dia.vals.shape[0]
+        A = idrs_amd.Matrix(idrs_amd.dia_format(n, n, np.asarray(dia.offsets), rvals=dia.vals))
+        A.set_seed(0.3)
+    elif case == "dia3_complex_vectors":
+        n = 4099
+        vals = rng.standard_normal((n, 3))
+        A = idrs_amd.Matrix(idrs_amd.dia_format(n, n, np.array([-1, 0, 1]), rvals=vals))
+    elif case == "dia_generic":
+        n, hb = 4099, 5
+        band = rng.standard_normal((n, 2 * hb + 1))
+        A = idrs_amd.Matrix(idrs_amd.cds_format(n, n, hb, rband=band))
+    elif case == "sliced_scaled_seed_cheb":
+        crs, _ = P.convection_diffusion_3d(12, 10.0, reaction=1.0)
+        n = crs.n
+        K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+        K.set_group(1)
+        A = idrs_amd.Matrix(K, which_precon=1)
+        A.set_seed(0.2)
+        idrs_amd.real_preconditioner(A, 2, 4, foci=[0.15, 1.85])
+    elif case == "sliced_ragged":
+        crs = _ragged(5000, 2, (7, 4000))
+        n = crs.n
+        K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+        K.set_group(1)
+        A = idrs_amd.Matrix(K)
+    elif case == "sliced_complex":
+        crs = _ragged(3000, 5)
+        n = crs.n
+        K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, cval=crs.val * (1.0 + 0.5j))
+        K.set_group(1)
+        A = idrs_amd.Matrix(K)
+    else:  # lanes4
+        crs = _ragged(3000, 6)
+        n = crs.n
+        K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+        K.set_group(4)
+        A = idrs_amd.Matrix(K)
+    return A, n
+
+
+@pytest.mark.parametrize("case", ["dia7_scaled_cheb", "dia5_seed", "dia3_complex_vectors", "dia_generic",
+                                  "sliced_scaled_seed_cheb", "sliced_ragged", "sliced_complex", "lanes4"])
+def test_block_products_match_columns(gpu, case):
+    """A*V for a block of 11 vectors (launches of 8 + 3 columns) equals A*v
+    one column at a time, bit for bit: the block kernels (DIA with 3, 5 or 7
+    diagonals, the sliced CRS copy) repeat the one-column row sum, Horner
+    term, seed shift and scaling; the other layouts (generic DIA, lanes per
+    row) run the one-column product per column."""
+    rng = np.random.default_rng(40)
+    A, n = _block_operator(case, rng)
+    V = rng.standard_normal((n, 11))
+    if case in ("dia3_complex_vectors", "sliced_complex"):
+        V = V + 1j * rng.standard_normal((n, 11))
+    W = A * V
+    assert W.shape == V.shape
+    for j in range(11):
+        assert np.array_equal(W[:, j], A * V[:, j]), j
+    assert np.array_equal(A * V[:, 4:5], W[:, 4:5])
+    with pytest.raises(idrs_amd.IdrsError):
+        A * V[:, :0]
+
+
+@pytest.mark.parametrize("fmt", ["sliced", "dia"])
+def test_multishift_blocks_of_shifts(gpu, fmt):
+    """Eleven shifts of the Chebyshev-preconditioned 3-D Laplacian (block
+    products of 8 + 3 shifts in the final residuals and the scale-back):
+    ms_scaleback of all shifts equals each shift's scale-back on its own bit
+    for bit, every (K - sigma I) x = b holds, and the residual norms the
+    solvers return agree with norms recomputed one shift at a time."""
+    crs, dia = P.helmholtz_3d(12)
+    n = crs.n
+    if fmt == "sliced":
+        K = idrs_amd.crs_format(n, n, crs.row_ptr, crs.col_ind, rval=crs.val)
+        K.set_group(1)
+    else:
+        K = idrs_amd.dia_format(n, n, dia.offsets, rvals=dia.vals)
+    A = idrs_amd.Matrix(K)
+    idrs_amd.real_preconditioner(A, 2, 4, foci=(0.05, 11.95))
+    b = np.random.default_rng(22).uniform(-1, 1, n)
+    sig = -0.25 * np.arange(11)
+    sig_p, alpha_s = idrs_amd.ms_parameters(A, sig)
+    Ks = crs.to_scipy()
+    for solver in (idrs_amd.msidrs, idrs_amd.msqmridr):
+        r = solver(A, b, sig_p, 4, tolerance=1e-10, maximum_iterations=2000)
+        assert r.flag == 0
+        for k in range(11):
+            res = b - (A * r.x[:, k] - sig_p[k] * r.x[:, k])
+            assert abs(np.linalg.norm(res) / np.linalg.norm(b) - r.relres[k]) <= 1e-12 * max(r.relres[k], 1e-300) + 1e-15
+        X = idrs_amd.ms_scaleback(A, alpha_s, r.x)
+        for k in range(11):
+            Xk = idrs_amd.ms_scaleback(A, alpha_s[:, k:k + 1], r.x[:, k:k + 1])
+            assert np.array_equal(X[:, k], Xk[:, 0]), k
+            res = b - (Ks @ X[:, k] - sig[k] * X[:, k])
+            assert np.linalg.norm(res) / np.linalg.norm(b) < 1e-8
