@@ -208,16 +208,26 @@ __global__ __launch_bounds__(256) void k_shift_res(long n, const T *__restrict__
     if (threadIdx.x == 0) partials[blockIdx.x] = e[0];
 }
 
+// Per-shift coefficients of the update kernels, passed by value in the
+// launch when they fit (no copy to device memory before each update);
+// larger sets come through device memory (the kernel's coef pointer).
+template <class T>
+struct CoefArg {
+    static constexpr int CAP = 3072 / (int)sizeof(T);
+    T c[CAP];
+};
+
 // MSIDRS dimension-reduction update of every shift (msidrs.inc:207-226):
 //   t = v - om_s*v0*u(s+2-k) - sum_i W(:,i,s)*c_i;  W(:,k,s) = t/u(s+2);
 //   x(:,s) = x(:,s) + phi_s*W(:,k,s)
 // coef per shift: [om_s, u(s+2-k), c_1..c_s, u(s+2), phi_s]
 template <class T>
 __global__ __launch_bounds__(256) void k_msidrs_update(long n, int ns, int s, int k, const T *__restrict__ v,
-                                                       const T *__restrict__ v0, T *W, T *X, const T *coef) {
+                                                       const T *__restrict__ v0, T *W, T *X, const T *coef,
+                                                       const CoefArg<T> ca) {
     __shared__ T sc[MAXS * (SMAX + 4)];
     const int stride = s + 4;
-    for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef[i];
+    for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef ? coef[i] : ca.c[i];
     __syncthreads();
     KGRID(q, n) {
         const T vq = v[q], v0q = v0[q];
@@ -235,9 +245,9 @@ __global__ __launch_bounds__(256) void k_msidrs_update(long n, int ns, int s, in
 
 // x(:,s) = x(:,s) + c_s*v0 for every shift (msidrs.inc:294)
 template <class T>
-__global__ void k_ms_xaxpy(long n, int ns, const T *__restrict__ v0, T *X, const T *coef) {
+__global__ void k_ms_xaxpy(long n, int ns, const T *__restrict__ v0, T *X, const T *coef, const CoefArg<T> ca) {
     __shared__ T sc[MAXS];
-    for (int i = threadIdx.x; i < ns; i += blockDim.x) sc[i] = coef[i];
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) sc[i] = coef ? coef[i] : ca.c[i];
     __syncthreads();
     KGRID(q, n) {
         const T v0q = v0[q];
@@ -252,10 +262,10 @@ __global__ void k_ms_xaxpy(long n, int ns, const T *__restrict__ v0, T *X, const
 // active (real part != 0)].  vt_s is vt + ks*vt_stride.
 template <class T>
 __global__ __launch_bounds__(256) void k_qmr_update(long n, int ns, int s, int head, const T *vt, long vt_stride,
-                                                    T *W, T *X, const T *coef) {
+                                                    T *W, T *X, const T *coef, const CoefArg<T> ca) {
     __shared__ T sc[MAXS * (SMAX + 4)];
     const int stride = s + 4, nw = s + 1;
-    for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef[i];
+    for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef ? coef[i] : ca.c[i];
     __syncthreads();
     KGRID(q, n) {
         for (int ks = 0; ks < ns; ks++) {
@@ -452,6 +462,16 @@ struct Work {
         cslot ^= 1;
         for (size_t i = 0; i < c.size(); i++) h[i] = to_dev(c[i]);
         HIP_TRY(hipMemcpyAsync(dcoef, h, sizeof(T) * c.size(), hipMemcpyHostToDevice, st));
+    }
+    // coefficients for an update kernel: in ca when they fit (nullptr
+    // returned), else in device memory (its pointer returned)
+    const T *coefs(const std::vector<H> &c, CoefArg<T> &ca) {
+        if (c.size() <= (size_t)CoefArg<T>::CAP) {
+            for (size_t i = 0; i < c.size(); i++) ca.c[i] = to_dev(c[i]);
+            return nullptr;
+        }
+        put_coef(c);
+        return dcoef;
     }
     // y = sum c_i src_i; returns ||y|| when want_norm
     double comb(T *y, const std::vector<const T *> &src, const std::vector<H> &cof, bool want_norm) {
@@ -792,9 +812,10 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
                     phi[ks] = -LS(s + 2, ks) * phi[ks];
                     normr[ks] = normg * absh(phi[ks]);
                 }
-                w.put_coef(coef);
+                CoefArg<T> ca{};
+                const T *dc = w.coefs(coef, ca);
                 hipLaunchKernelGGL((k_msidrs_update<T>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, k - 1, v, v0, W,
-                                   X, w.dcoef);
+                                   X, dc, ca);
                 LAUNCH_CHECK();
                 rv(iter);
                 double mx = 0.0;
@@ -835,8 +856,9 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
                 phi[ks] = -LS(s + 2, ks) * phi[ks];
                 normr[ks] = normg * absh(phi[ks]);
             }
-            w.put_coef(xc);
-            hipLaunchKernelGGL((k_ms_xaxpy<T>), dim3(w.gv), dim3(256), 0, stream, n, ns, v0, X, w.dcoef);
+            CoefArg<T> ca{};
+            const T *dc = w.coefs(xc, ca);
+            hipLaunchKernelGGL((k_ms_xaxpy<T>), dim3(w.gv), dim3(256), 0, stream, n, ns, v0, X, dc, ca);
             LAUNCH_CHECK();
             rv(iter);
             double mx = 0.0;
@@ -1115,11 +1137,12 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                     normr[ks] = absh(phn1[ks]) * (double)sqrtf((float)(jj + 1));
                     if (multishift) converged[ks] = normr[ks] < tolb;
                 }
-                w.put_coef(coef);
+                CoefArg<T> ca{};
+                const T *dc = w.coefs(coef, ca);
                 const T *vsrc = (multishift && in_it > 0) ? Vt : vtil;
                 const long vstride = (multishift && in_it > 0) ? n : 0;
                 hipLaunchKernelGGL((k_qmr_update<T>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, whead, vsrc, vstride,
-                                   W, X, w.dcoef);
+                                   W, X, dc, ca);
                 LAUNCH_CHECK();
                 whead = (whead + 1) % nw;
                 if (!multishift && broke) break;                          // qmridr.inc:234-236
