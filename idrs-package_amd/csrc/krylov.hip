@@ -174,14 +174,21 @@ __global__ void k_div_real(long n, T *w, double c) {
     }
 }
 
-// w = w / sqrt(re(nrm[0])) with the square root computed on the device;
-// nrm[1] receives it (the host reads it back with the other step scalars)
+// |w|^2 = the sum of the partials pin (reduced in every workgroup in one
+// fixed order, so all hold the same value), then w = w / sqrt(|w|^2) with
+// the square root taken on the device; workgroup 0 stores |w|^2 in nrm[0]
+// and the root in nrm[1] (the host reads them with the other step scalars)
 __device__ inline double real_part(double v) { return v; }
 __device__ inline double real_part(c128 v) { return v.re; }
 template <class T>
-__global__ void k_norm_div(long n, T *w, T *nrm) {
-    const double c = sqrt(real_part(nrm[0]));
-    if (blockIdx.x == 0 && threadIdx.x == 0) nrm[1] = T(c);
+__global__ __launch_bounds__(256) void k_norm_div(long n, T *w, const T *__restrict__ pin, int npin, T *nrm) {
+    __shared__ T red[1];
+    reduce_partials(pin, npin, 1, red);
+    const double c = sqrt(real_part(red[0]));
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        nrm[0] = red[0];
+        nrm[1] = T(c);
+    }
     KGRID(q, n) {
         if constexpr (Tr<T>::is_complex) w[q] = T(w[q].re / c, w[q].im / c);
         else w[q] = w[q] / c;
@@ -284,14 +291,6 @@ __global__ __launch_bounds__(256) void k_qmr_update(long n, int ns, int s, int h
     }
 }
 
-// Reduces the partials into device memory (a scalar a later kernel reads)
-template <class T>
-__global__ __launch_bounds__(FIN_THREADS) void k_red_dev(const T *partials, int nparts, int nv, T *out) {
-    __shared__ T red[MAX_RED];
-    reduce_partials(partials, nparts, nv, red);
-    if (threadIdx.x < nv) out[threadIdx.x] = red[threadIdx.x];
-}
-
 // Copies nv device scalars into pinned host memory, then the call's
 // sequence number (as k_red_host)
 template <class T>
@@ -302,13 +301,18 @@ __global__ void k_fetch_host(const T *src, int nv, T *out, unsigned *seq_out, un
     if (threadIdx.x == 0) __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// MGS step with beta read from device memory: gn -= beta Gi, then the
-// partials of Gn^H gn (or |gn|^2 without Gn)
+// MGS step: beta = the sum of the previous dot's partials pin (reduced in
+// every workgroup in one fixed order, so all hold the same value; workgroup
+// 0 stores it in beta_out for the host); gn -= beta Gi, then the partials of
+// Gn^H gn (or |gn|^2 without Gn) in pout (a buffer other than pin)
 template <class T>
-__global__ __launch_bounds__(256) void k_mgs_d(long n, T *gn, const T *__restrict__ Gi, const T *beta_d,
-                                               const T *__restrict__ Gn, T *partials) {
+__global__ __launch_bounds__(256) void k_mgs_r(long n, T *gn, const T *__restrict__ Gi, const T *__restrict__ pin,
+                                               int npin, T *beta_out, const T *__restrict__ Gn, T *partials) {
     __shared__ T lds[4];
-    const T beta = *beta_d;
+    __shared__ T red[1];
+    reduce_partials(pin, npin, 1, red);
+    const T beta = red[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *beta_out = beta;
     T e[1] = {T(0)};
     KGRID(q, n) {
         const T g = kfma(-beta, Gi[q], gn[q]);
@@ -345,7 +349,7 @@ struct Work {
     hipStream_t st;
     long n;
     int gv;
-    T *part = nullptr, *dcoef = nullptr, *hcoef = nullptr, *dsc = nullptr;
+    T *part = nullptr, *part2 = nullptr, *dcoef = nullptr, *hcoef = nullptr, *dsc = nullptr;
     T *hres = nullptr, *hres_d = nullptr;      // coherent pinned result slots (host / device view)
     unsigned *hseq = nullptr, *hseq_d = nullptr;
     unsigned seq = 0;
@@ -353,6 +357,7 @@ struct Work {
     std::vector<void *> bufs;
     explicit Work(long n_, hipStream_t s) : st(s), n(n_), gv(vec_grid(n_)) {
         part = alloc(MAX_NV * (size_t)MAX_PARTS);
+        part2 = alloc(MAX_PARTS);
         dcoef = alloc(NCOEF);
         dsc = alloc(MAX_RED);
         HIP_TRY(hipHostMalloc((void **)&hres, sizeof(T) * 2 * MAX_RED, hipHostMallocCoherent | hipHostMallocMapped));
@@ -427,20 +432,14 @@ struct Work {
         for (int i = 0; i < nv; i++) out[i] = to_host(hres[i]);
         return out;
     }
-    // sum of the partials into a device scalar (no host round trip)
-    void reduce_to(T *out, int nparts) {
-        hipLaunchKernelGGL((k_red_dev<T>), dim3(1), dim3(FIN_THREADS), 0, st, part, nparts, 1, out);
-        LAUNCH_CHECK();
-    }
-    // u^H w into a device scalar
-    void dot_to(const T *u, const T *v, T *out) {
+    // partials of u^H v into out (gv of them)
+    void dot_parts(const T *u, const T *v, T *out) {
         Dots<T> d{};
         d.nu = 1;
         d.u[0] = u;
         d.w[0] = v;
-        hipLaunchKernelGGL((k_dots<T>), dim3(gv), dim3(256), 0, st, n, d, part, gv);
+        hipLaunchKernelGGL((k_dots<T>), dim3(gv), dim3(256), 0, st, n, d, out, gv);
         LAUNCH_CHECK();
-        reduce_to(out, gv);
     }
     // nv device scalars to the host
     std::vector<H> fetch(const T *src, int nv) {
@@ -1072,19 +1071,22 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                 {
                     T *db = w.dsc;
                     const int i0 = k < s + 1 ? s - k + 1 : s + 1;
+                    // each kernel reduces the partials of the dot before it
+                    // (two partial buffers in turn)
+                    T *pa = w.part, *pb = w.part2;
                     if (k < s + 1) {
-                        w.dot_to(Gcol(i0), gn, db + i0);
+                        w.dot_parts(Gcol(i0), gn, pa);
                         for (int i = i0; i <= s; i++) {
                             const T *Gn = i < s ? Gcol(i + 1) : nullptr;
-                            hipLaunchKernelGGL((k_mgs_d<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, Gcol(i), db + i,
-                                               Gn, w.part);
+                            hipLaunchKernelGGL((k_mgs_r<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, Gcol(i), pa, w.gv,
+                                               db + i, Gn, pb);
                             LAUNCH_CHECK();
-                            w.reduce_to(db + i + 1, w.gv);
+                            std::swap(pa, pb);
                         }
                     } else {
-                        w.dot_to(gn, gn, db + s + 1);
+                        w.dot_parts(gn, gn, pa);
                     }
-                    hipLaunchKernelGGL((k_norm_div<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, db + s + 1);
+                    hipLaunchKernelGGL((k_norm_div<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, pa, w.gv, db + s + 1);
                     LAUNCH_CHECK();
                     w.post_fetch(db + i0, s + 3 - i0, 0);
                     w.wait_seq(w.post_pdot(gn, P, s, MAX_RED));
