@@ -14,6 +14,7 @@
 #include "batch.hpp"
 
 #include "idrs_state.hpp"
+#include "mixed.hpp"
 #include "spmv_epi.hpp"
 
 #include <algorithm>
@@ -111,6 +112,22 @@ __global__ __launch_bounds__(BT) void k_bfin_scalar(BV bv, State<T> *st, const T
         S->normr = nr;
         if (S->info == 0 && nr > S->tolb) S->info = 2;
     }
+}
+
+// per-system 2-norms from tile partials of |v|^2 (one workgroup per system)
+template <class T>
+__global__ __launch_bounds__(BT) void k_bsys_norm(BV bv, const T *partials, double *out) {
+    __shared__ T red[1];
+    reduce_system(bv, partials, 1, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = sqrt(sre(red[0]));
+}
+
+// y = c[k] v on the rows of every system k; y += c[k] v with accumulate
+template <class T>
+__global__ __launch_bounds__(BT) void k_bscale(BV bv, const double *__restrict__ c, const T *__restrict__ v, T *y,
+                                               int accumulate) {
+    const double ck = c[bv.tile_sys[blockIdx.x]];
+    TILE_ROWS(q) y[q] = accumulate ? fma(ck, v[q], y[q]) : ck * v[q];
 }
 
 // MAKE_P (idrs_module.f90:839-937) per system: each system's rows draw the
@@ -675,6 +692,125 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
     cleanup();
 }
 #undef NV_SWITCH
+
+void batch_mixed_solve(const Operator &A, int s, const double *b, double *x, const SolveOpts<double> &o,
+                       double inner_tol, int max_outer, std::vector<MixedStats> &stats, double &solve_ms,
+                       hipStream_t stream) {
+    const Partition *part = A.K->part.get();
+    if (!part) throw std::invalid_argument("the operator's matrix is not a batch (block-diagonal with a partition)");
+    if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..16");
+    if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
+    if (!(inner_tol > 0 && inner_tol < 1)) throw std::invalid_argument("inner tolerance must be in (0, 1)");
+    if (max_outer < 1) throw std::invalid_argument("at least one outer step");
+    if (o.has_m1 || o.U0 || o.resvec || o.omega || o.x0 || o.PT_raw)
+        throw std::invalid_argument("M1, U0, omega, resvec, x0 and PT_raw are not offered here");
+    if (A.values_complex()) throw std::invalid_argument("mixed precision needs real matrix values");
+    if (A.seed != std::complex<double>(0.0, 0.0)) throw std::invalid_argument("mixed precision without a seed shift");
+    Operator &op = const_cast<Operator &>(A);
+    op_make_lowprec(op);
+    const int nsys = part->nsys, ntiles = part->ntiles;
+    const long ntot = part->h_row0[nsys];
+    long nmax = 0;
+    for (int q = 0; q < nsys; q++) nmax = std::max(nmax, part->h_row0[q + 1] - part->h_row0[q]);
+    const int maxit = o.has_maxit ? o.maxit : (int)std::min<long>(2 * nmax, 1000);
+    const size_t vb = sizeof(double) * (size_t)ntot;
+    DBuf br(vb), bt(vb), bw1(vb), bw2(vb), brh(vb), bdh(vb), bpart(sizeof(double) * (size_t)ntiles);
+    DBuf bnorm(sizeof(double) * nsys), bfac(sizeof(double) * nsys);
+    double *r = br.as<double>(), *t = bt.as<double>(), *w1 = bw1.as<double>(), *w2 = bw2.as<double>();
+    double *rh = brh.as<double>(), *dh = bdh.as<double>(), *part_d = bpart.as<double>();
+    double *norm_d = bnorm.as<double>(), *fac_d = bfac.as<double>();
+    const BV bv{nsys, ntiles, part->tile_sys, part->tile_r0, part->tile_r0 + 1, part->tile0, part->row0};
+    const dim3 gt(ntiles), gsys(nsys);
+    TileCtx all{part, nullptr, 0};
+    EpiArgs<double> none{};
+    std::vector<double> normb(nsys), normr(nsys), prev(nsys), fac(nsys);
+    std::vector<int> its(nsys, 0);
+    stats.assign(nsys, MixedStats{});
+    // per-system |b - ax| (ax null: |b|), r = b - ax when r is given
+    auto sys_norms = [&](const double *ax, double *rv, std::vector<double> &out) {
+        hipLaunchKernelGGL((k_bresidual<double>), gt, dim3(BT), 0, stream, bv, b, ax, rv, part_d);
+        LAUNCH_CHECK();
+        hipLaunchKernelGGL((k_bsys_norm<double>), gsys, dim3(BT), 0, stream, bv, (const double *)part_d, norm_d);
+        LAUNCH_CHECK();
+        HIP_TRY(hipMemcpyAsync(out.data(), norm_d, sizeof(double) * nsys, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+    };
+    auto per_system = [&](const std::vector<double> &c, const double *v, double *y, int accumulate) {
+        HIP_TRY(hipMemcpyAsync(fac_d, c.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL((k_bscale<double>), gt, dim3(BT), 0, stream, bv, (const double *)fac_d, v, y, accumulate);
+        LAUNCH_CHECK();
+    };
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, stream));
+    HIP_TRY(hipMemsetAsync(x, 0, vb, stream));
+    sys_norms(nullptr, nullptr, normb);
+    sys_norms(nullptr, r, normr);       // r = b (x = 0)
+    while (true) {
+        // a system stops at its FP64 tolerance, at max_outer or at its share of maxit
+        int active = 0;
+        double itol = 1.0;
+        for (int q = 0; q < nsys; q++) {
+            MixedStats &S = stats[q];
+            if (S.flag >= 0) continue;
+            if (normr[q] <= o.tol * normb[q]) S.flag = 0;
+            else if (S.outer == max_outer || its[q] >= maxit) S.flag = 1;
+            else {
+                active++;
+                // no tighter than the FP64 target needs (half of it, for the FP32 error)
+                itol = std::min(itol, std::max(inner_tol, 0.5 * o.tol * normb[q] / normr[q]));
+            }
+        }
+        if (active == 0) break;
+        // inner right-hand sides r_k/|r_k|; a finished system gets zero,
+        // which the inner solve finishes at once
+        for (int q = 0; q < nsys; q++) fac[q] = stats[q].flag < 0 ? 1.0 / normr[q] : 0.0;
+        per_system(fac, r, rh, 0);
+        SolveOpts<double> io;
+        io.tol = itol;
+        io.has_maxit = true;
+        io.maxit = maxit;
+        io.variant = o.variant;
+        io.p_seed = o.p_seed;
+        io.graph_mode = o.graph_mode;
+        std::vector<SolveStats> is;
+        double ims = 0.0;
+        op.values32 = true;
+        try {
+            batch_idrs_solve<double>(A, s, rh, dh, io, is, ims, stream);
+        } catch (...) {
+            op.values32 = false;
+            throw;
+        }
+        op.values32 = false;
+        for (int q = 0; q < nsys; q++) fac[q] = stats[q].flag < 0 ? normr[q] : 0.0;
+        per_system(fac, dh, x, 1);
+        // true FP64 residuals
+        op_apply<double>(A, x, t, w1, w2, EPI_NONE, none, nullptr, stream, &all);
+        prev = normr;
+        sys_norms(t, r, normr);
+        for (int q = 0; q < nsys; q++) {
+            MixedStats &S = stats[q];
+            if (S.flag >= 0) continue;
+            its[q] += is[q].iterations;
+            S.outer++;
+            if (is[q].flag == 3 || !(normr[q] < prev[q])) S.flag = normr[q] <= o.tol * normb[q] ? 0 : 3;
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    solve_ms = ms;
+    for (int q = 0; q < nsys; q++) {
+        stats[q].iterations = its[q];
+        stats[q].relres = normr[q] / normb[q];
+        stats[q].solve_ms = ms;
+    }
+}
 
 template void batch_idrs_solve<double>(const Operator &, int, const double *, double *, const SolveOpts<double> &,
                                        std::vector<SolveStats> &, double &, hipStream_t);

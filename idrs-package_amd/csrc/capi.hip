@@ -281,6 +281,52 @@ static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opt
     });
 }
 
+static int batched_mixed_entry(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double inner_tol,
+                               int max_outer, double *x, idrs_stats_t *stats, int *outer_steps) {
+    return guarded([&] {
+        need(A && b && x && stats && outer_steps, "null argument");
+        const Partition *part = A->op.K->part.get();
+        need(part != nullptr, "not a batch matrix (idrs_batch_format)");
+        hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
+        idrs_opts_t o{};
+        if (opts) o = *opts;
+        need(!o.has_M1 && !o.U0 && !o.resvec && !o.omega && !o.x0 && !o.PT_raw,
+             "M1, U0, omega, resvec, x0 and PT_raw are not offered here");
+        need(o.graph_mode >= 0 && o.graph_mode <= 2, "graph_mode must be 0, 1 or 2");
+        SolveOpts<double> so;
+        if (o.has_tolerance) so.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            so.has_maxit = true;
+            so.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) so.variant = o.variant;
+        so.p_seed = o.p_seed ? o.p_seed : 1;
+        so.graph_mode = o.graph_mode;
+        const int nsys = part->nsys;
+        const long ntot = part->h_row0[nsys];
+        std::vector<MixedStats> ms;
+        double solve_ms = 0.0;
+        if (o.device_pointers) {
+            batch_mixed_solve(A->op, s, b, x, so, inner_tol, max_outer, ms, solve_ms, st);
+        } else {
+            DVec<double> db(b, ntot, st), dx(ntot);
+            batch_mixed_solve(A->op, s, db.p, dx.p, so, inner_tol, max_outer, ms, solve_ms, st);
+            dx.download(x, st);
+        }
+        for (int k = 0; k < nsys; k++) {
+            stats[k].flag = ms[k].flag;
+            stats[k].iterations = ms[k].iterations;
+            stats[k].relres = ms[k].relres;
+            stats[k].solve_ms = solve_ms;
+            stats[k].spmv_ms = 0.0;
+            stats[k].spmv_launches = 0;
+            stats[k].reserved = 0;
+            outer_steps[k] = ms[k].outer;
+        }
+    });
+}
+
 template <class T>
 static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double inner_tol,
                        int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats, int *outer_steps) {
@@ -848,6 +894,11 @@ int idrs_batch_info(idrs_user_matrix_t A, int *nsys, int64_t *row0) {
         if (p && row0)
             for (int k = 0; k <= p->nsys; k++) row0[k] = p->h_row0[k];
     });
+}
+int idrs_idrs_batched_mixed_f64(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts,
+                                double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats,
+                                int *outer_steps) {
+    return batched_mixed_entry(A, b, s, opts, inner_tolerance, max_outer, x, stats, outer_steps);
 }
 int idrs_idrs_batched_f64(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
                           idrs_stats_t *stats) {

@@ -6,6 +6,7 @@ the IDRS solver); see api.py.
 """
 from ._lib import IdrsError, load as load_library  # noqa: F401
 from .api import (  # noqa: F401
+    BatchMixedResult,
     BatchResult,
     IdrsResult,
     KrylovResult,
@@ -22,6 +23,7 @@ from .api import (  # noqa: F401
     dia_format,
     idrs,
     idrs_batched,
+    idrs_batched_mixed,
     idrs_mixed,
     make_p,
     ms_parameters,

@@ -160,6 +160,8 @@ _PROTOS = {
     "idrs_crs_pattern_grad_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     "idrs_batch_format": (c_int, [c_int, c_void_p, POINTER(c_void_p)]),
     "idrs_batch_info": (c_int, [c_void_p, POINTER(c_int), c_void_p]),
+    "idrs_idrs_batched_mixed_f64": (c_int, [c_void_p, c_void_p, c_int, POINTER(idrs_opts_t), c_double, c_int, c_void_p,
+                                            c_void_p, c_void_p]),
     "idrs_idrs_batched_f64": (c_int, [c_void_p, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, c_void_p]),
     "idrs_idrs_batched_c128": (c_int, [c_void_p, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, c_void_p]),
     "idrs_make_p_f64": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_uint64, c_void_p]),

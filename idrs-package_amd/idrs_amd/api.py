@@ -29,6 +29,7 @@ KrylovResult = namedtuple("KrylovResult", "x flag relres iterations resvec solve
 BatchResult = namedtuple("BatchResult", "x flag relres iterations solve_ms")
 # iterations: inner (single-precision) iterations; outer: FP64 residual replacements
 MixedResult = namedtuple("MixedResult", "x flag relres iterations outer solve_ms")
+BatchMixedResult = namedtuple("BatchMixedResult", "x flag relres iterations outer solve_ms")
 
 
 def _ptr(a):
@@ -688,6 +689,56 @@ def idrs_mixed(A, b, s, tolerance=None, maximum_iterations=None, variant=None, x
     check(f(A.handle, n, bp, int(s), ctypes.byref(o), float(inner_tolerance), int(max_outer), int(bool(fp32_vectors)),
             xp, ctypes.byref(st), ctypes.byref(outer)))
     return MixedResult(x, st.flag, st.relres, st.iterations, outer.value, st.solve_ms)
+
+
+def idrs_batched_mixed(A, b, s, tolerance=None, maximum_iterations=None, variant=None, p_seed=None,
+                       inner_tolerance=1e-5, max_outer=10, graph=True, out=None):
+    """x_k = A_k^-1 b_k for every system of a batch to the FP64 tolerance on
+    each system's true residual, with batched inner IDR(s) solves on the FP32
+    copy of the matrix values and per-system FP64 residual replacement
+    (idrs_idrs_batched_mixed_f64).  b: the systems' right-hand sides
+    concatenated, a float64 numpy array or CUDA tensor.  Returns
+    BatchMixedResult with per-system arrays (iterations: inner iterations;
+    outer: FP64 residual replacements)."""
+    lib = _lib.load()
+    nsys, row0 = batch_info(A.K)
+    if nsys == 0:
+        raise ValueError("A is not built on a batch matrix")
+    device = _is_torch(b)
+    n = int(b.shape[0])
+    if n != row0[-1]:
+        raise ValueError(f"b has {n} entries, the batch {row0[-1]}")
+    o = idrs_opts_t()
+    if tolerance is not None:
+        if tolerance < 0:
+            raise ValueError("Illegal value for parameter tolerance")
+        o.has_tolerance, o.tolerance = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    o.graph_mode = 0 if graph else 2
+    o.p_seed = int(p_seed) if p_seed is not None else 0
+    if device:
+        import torch
+
+        if b.dtype != torch.float64 or not b.is_contiguous():
+            raise TypeError("device b must be a contiguous float64 CUDA tensor")
+        x = out if out is not None else torch.empty_like(b)
+        o.device_pointers = 1
+        bp, xp = ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr())
+        torch.cuda.current_stream().synchronize()
+    else:
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty_like(b)
+        bp, xp = _ptr(b), _ptr(x)
+    st = (idrs_stats_t * nsys)()
+    outer = (ctypes.c_int * nsys)()
+    check(lib.idrs_idrs_batched_mixed_f64(A.handle, bp, int(s), ctypes.byref(o), float(inner_tolerance), int(max_outer),
+                                          xp, st, outer))
+    return BatchMixedResult(x, np.array([q.flag for q in st]), np.array([q.relres for q in st]),
+                            np.array([q.iterations for q in st]), np.array(list(outer)),
+                            st[0].solve_ms if nsys else 0.0)
 
 
 def ms_parameters(A, sigma):

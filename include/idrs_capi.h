@@ -363,6 +363,19 @@ int idrs_batch_info(idrs_user_matrix_t A, int *nsys, int64_t *row0);
    uses min(2 n_k, 1000).  stats: one per system (solve_ms: whole batch). */
 int idrs_idrs_batched_f64(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
                           idrs_stats_t *stats);
+/* x_k = IDRS(A_k, b_k, s, ...) for every system k of the batch matrix
+   behind A to the FP64 tolerance opts->tolerance on each system's true
+   residual, in mixed precision: r_k = b_k - A_k x_k in FP64, then batched
+   IDR(s) on A d_k = r_k/|r_k| whose SpMVs read FP32 copies of the matrix
+   values (vectors FP64) to inner_tolerance (no tighter than the tolerance
+   needs), x_k += |r_k| d_k, at most max_outer times per system.
+   opts->maximum_iterations bounds each inner solve (default min(2 n_max,
+   1000)).  Real matrix values, no seed shift; M1, U0, omega, resvec, x0 and
+   PT_raw are rejected.  stats and outer_steps: one per system
+   (stats->iterations: inner iterations; solve_ms: whole batch). */
+int idrs_idrs_batched_mixed_f64(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts,
+                                double inner_tolerance, int max_outer, double *x, idrs_stats_t *stats,
+                                int *outer_steps);
 /* complex128 form of idrs_idrs_batched_f64 */
 int idrs_idrs_batched_c128(idrs_matrix_t A, const double *b, int s, const idrs_opts_t *opts, double *x,
                            idrs_stats_t *stats);

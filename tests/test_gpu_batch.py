@@ -203,3 +203,45 @@ def test_batch_rejects(gpu):
     A, b = _batch(_systems(SPECS[:2]))
     with pytest.raises(idrs_amd.IdrsError):
         idrs_amd.idrs_batched(A, b, 0)
+
+
+@pytest.mark.parametrize("layout", ["lanes", "sliced"])
+def test_batch_mixed_precision(gpu, layout):
+    """Batched mixed precision (idrs_idrs_batched_mixed_f64), plain and with
+    diagonal scaling on the sliced copy: every system
+    meets the FP64 tolerance on its own true residual, recomputed here in
+    FP64 from the returned x by the oracle's operator; the solutions agree with the oracle's FP64
+    solves to the accuracy the tolerance allows; a zero right-hand side
+    finishes at once with x = 0."""
+    systems = _systems(SPECS, seed=7)
+    systems[2] = (systems[2][0], np.zeros(systems[2][0].n))
+    scaled = layout == "sliced"
+    A, b = _batch(systems, diag_precon=scaled)
+    if layout == "sliced":
+        A.K.set_group(1)
+    tol = 1e-8
+    r = idrs_amd.idrs_batched_mixed(A, b, 4, tolerance=tol, inner_tolerance=1e-5)
+    _, row0 = idrs_amd.batch_info(A.K)
+    assert list(r.flag) == [0] * len(SPECS)
+    for k, (c, bk) in enumerate(systems):
+        xk = r.x[row0[k]:row0[k + 1]]
+        if not bk.any():
+            assert not xk.any() and r.outer[k] == 0
+            continue
+        # the operator's residual (with scaling, A = K D^-1)
+        Ao = oracle_crs(c, diag_precon=scaled)
+        res = np.linalg.norm(bk - Ao.matvec(xk)) / np.linalg.norm(bk)
+        assert res <= tol * (1 + 1e-6), (k, res)
+        assert abs(res - r.relres[k]) <= 1e-12 + 1e-6 * res
+        assert r.outer[k] >= 1 and r.iterations[k] > 0
+        xo = orc.idrs(Ao, bk, 4, tolerance=1e-12, maximum_iterations=5000)[0]
+        assert rel(xk, xo) < 1e-4, k
+    # device vectors give the same result
+    import torch
+
+    bd = torch.from_numpy(b).to("cuda:0")
+    rd = idrs_amd.idrs_batched_mixed(A, bd, 4, tolerance=tol, inner_tolerance=1e-5)
+    assert np.array_equal(rd.x.cpu().numpy(), r.x)
+    # options the scheme does not offer
+    with pytest.raises(idrs_amd.IdrsError):
+        idrs_amd.idrs_batched_mixed(A, b, 4, tolerance=tol, inner_tolerance=2.0)
