@@ -501,6 +501,32 @@ __device__ inline T sell_row(const SpmvArgs<VT, T> &A, long row, long p0, int w)
     const T *__restrict__ x = A.x;
     const int len = A.rp[row + 1] - A.rp[row];
     T acc = T(0);
+    if constexpr (std::is_same<VT, float>::value) {
+        // FP32 copy: one 8-byte load brings an entry's value and column
+        // (4-byte loads of each would double the load instructions, which
+        // bound this product); the same chunks and sum order as below
+        if (A.pair32) {
+            const unsigned long long *pr = (const unsigned long long *)val;
+            for (int k0 = 0; k0 < w; k0 += 8) {
+                float vv[8];
+                int cc[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const bool ok = k0 + k < len;
+                    const unsigned long long e = ok ? __builtin_nontemporal_load(&pr[p0 + (long)SELL_C * (k0 + k)]) : 0ull;
+                    vv[k] = __uint_as_float((unsigned)(e & 0xffffffffull));
+                    cc[k] = (int)(e >> 32);
+                }
+                T xx[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) xx[k] = k0 + k < len ? x[cc[k]] : T(0);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (k0 + k < len) acc = fmav(vv[k], xx[k], acc);
+            }
+            return acc;
+        }
+    }
     // chunks of 8 entries (w, the slice width, is wave-uniform): all value
     // and index loads of a chunk, then its x gathers, then the in-order sum
     for (int k0 = 0; k0 < w; k0 += 8) {
@@ -710,7 +736,7 @@ template <class VT, class T>
 static bool launch_cols(const DevMatrix &K, const SpmvArgs<VT, T> &A, const ColArgs<T> &C, hipStream_t st) {
     dim3 gd(spmv_grid(K)), bd(SPMV_THREADS);
     if (K.fmt == DevFmt::CRS) {
-        if (!A.sl_ptr) return false;
+        if (!A.sl_ptr || A.pair32) return false;
         hipLaunchKernelGGL((k_spmv_sell_cols<VT, T>), gd, bd, 0, st, A, C);
     } else {
         switch (K.ndiag) {
@@ -913,6 +939,7 @@ static SpmvArgs<VT, T> base_args(const DevMatrix &K, const VT *val) {
     A.ud = INT_MAX;
     A.sl_ptr = nullptr;
     A.sl_col = nullptr;
+    A.pair32 = 0;
     return A;
 }
 
@@ -928,6 +955,7 @@ static void use_sliced(const Operator &op, SpmvArgs<VT, T> &A) {
     A.val = (const VT *)v;
     A.sl_ptr = K.sell->sl_ptr;
     A.sl_col = K.sell->sl_col;
+    A.pair32 = lo ? 1 : 0;
 }
 
 // The argument block of one product of op: its values (complex, FP32 copy
@@ -1585,6 +1613,13 @@ __global__ void k_convert(long n, const VT *__restrict__ in, OT *__restrict__ ou
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         out[i] = (OT)in[i];
 }
+// (float value, column) pairs of a sliced copy: low word the value's bits,
+// high word the column
+__global__ void k_pair32(long n, const double *__restrict__ val, const int *__restrict__ col,
+                         unsigned long long *__restrict__ out) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (unsigned long long)__float_as_uint((float)val[i]) | ((unsigned long long)(unsigned)col[i] << 32);
+}
 __global__ void k_real_to_cplx(long n, const double *__restrict__ in, c128 *__restrict__ out) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         out[i] = c128(in[i], 0.0);
@@ -1761,9 +1796,10 @@ void op_make_lowprec(Operator &op) {
         void *&dst = op.sval ? op.sl_sval32 : K.sell->sl_val32;
         const void *src = op.sval ? op.sl_sval : K.sell->sl_val;
         if (!dst && src) {
-            HIP_TRY(hipMalloc(&dst, sizeof(float) * (ne ? ne : 1)));
-            hipLaunchKernelGGL((k_convert<double, float>), dim3(gs), dim3(256), 0, st, ne, (const double *)src,
-                               (float *)dst);
+            // (value, column) pairs, 8 bytes per entry (see sell_row)
+            HIP_TRY(hipMalloc(&dst, sizeof(unsigned long long) * (ne ? ne : 1)));
+            hipLaunchKernelGGL(k_pair32, dim3(gs), dim3(256), 0, st, ne, (const double *)src, K.sell->sl_col,
+                               (unsigned long long *)dst);
             LAUNCH_CHECK();
         }
     }

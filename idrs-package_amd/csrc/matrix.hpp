@@ -62,7 +62,7 @@ struct Sell {
     int *sl_col = nullptr;         // nent (padding: 0, never read)
     int *sl_perm = nullptr;        // nent: CRS entry of each sliced entry, -1 for padding
     void *sl_val = nullptr;        // K's values (double or c128) in sliced order
-    void *sl_val32 = nullptr;      // float copy of sl_val (op_make_lowprec)
+    void *sl_val32 = nullptr;      // (float value, column) pairs of sl_val (op_make_lowprec)
     ~Sell();
 };
 
@@ -115,7 +115,7 @@ struct Operator {
     int unit_diag = -1;
     // sval in K's sliced order (K.sell), made by op_prepare
     mutable void *sl_sval = nullptr;
-    mutable void *sl_sval32 = nullptr;   // float copy of sl_sval (op_make_lowprec)
+    mutable void *sl_sval32 = nullptr;   // (float value, column) pairs of sl_sval (op_make_lowprec)
     int degree = 0;
     int poly_kind = POLY_NONE;
     std::vector<std::complex<double>> alpha;   // alpha_p(0:degree)

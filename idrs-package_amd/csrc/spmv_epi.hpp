@@ -39,6 +39,7 @@ struct SpmvArgs {
     int ud;            // DIA: index of an implicit unit diagonal (not in val), INT_MAX if none
     const long *sl_ptr;   // CRS: sliced copy (val in sliced order), nullptr: CRS order
     const int *sl_col;
+    int pair32;           // sliced FP32 copy: val holds (value, column) pairs, 8 bytes per entry
     const T *x;
     T xs;
     const T *z;
