@@ -558,6 +558,14 @@ def bench_multishift(args):
     (rq, xq), ms_qmr = timed(lambda: ms(idrs_amd.msqmridr, 8))
     print(f"# msqmridr(8): {ms_qmr:.1f} ms, {rq.iterations} its, flag {rq.flag}, max relres {rq.relres.max():.2e}",
           file=sys.stderr, flush=True)
+    # shifts still running at each MSQMRIDR iteration (an untimed rerun with
+    # the residual history): the per-iteration update reads and writes 8N
+    # (1 + active (s+4)) bytes
+    rr = idrs_amd.msqmridr(A, b, sig_solve, 8, tolerance=tol, maximum_iterations=maxit, resvec=True)
+    hist = np.asarray(rr.resvec).reshape(len(sig_solve), -1)
+    tolb = tol * float(torch.linalg.vector_norm(b))
+    active = [int((hist[:, it - 1] >= tolb).sum()) for it in range(1, rr.iterations + 1)]
+    print(f"# msqmridr(8) active shifts per iteration: {active}", file=sys.stderr, flush=True)
     # per-shift loop: IDR(4) on K - sigma_k I via the operator's seed (with the
     # polynomial rebuilt for the shifted operator, and its scale-back)
     loop_its, x = [], torch.empty_like(b)
