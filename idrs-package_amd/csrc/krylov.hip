@@ -174,25 +174,44 @@ __global__ void k_div_real(long n, T *w, double c) {
     }
 }
 
+__device__ inline double real_part(double v) { return v; }
+__device__ inline double real_part(c128 v) { return v.re; }
+
 // |w|^2 = the sum of the partials pin (reduced in every workgroup in one
 // fixed order, so all hold the same value), then w = w / sqrt(|w|^2) with
 // the square root taken on the device; workgroup 0 stores |w|^2 in nrm[0]
-// and the root in nrm[1] (the host reads them with the other step scalars)
-__device__ inline double real_part(double v) { return v; }
-__device__ inline double real_part(c128 v) { return v.re; }
-template <class T>
-__global__ __launch_bounds__(256) void k_norm_div(long n, T *w, const T *__restrict__ pin, int npin, T *nrm) {
+// and the root in nrm[1] (the host reads them with the other step scalars).
+// The same pass writes the partials of P^T w for the normalised w (prow
+// rows of P, no conjugation) to pout: the next step's P^T gn without
+// another read of gn, with k_pdotk's grid, row order and block sums.
+template <class T, int NV>
+__global__ __launch_bounds__(256) void k_norm_div_pdot(long n, T *w, const T *__restrict__ pin, int npin, T *nrm,
+                                                       const T *__restrict__ P, int prow, T *pout, int nparts) {
     __shared__ T red[1];
+    __shared__ T lds[NV * 4];
     reduce_partials(pin, npin, 1, red);
     const double c = sqrt(real_part(red[0]));
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         nrm[0] = red[0];
         nrm[1] = T(c);
     }
+    T e[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) e[i] = T(0);
     KGRID(q, n) {
-        if constexpr (Tr<T>::is_complex) w[q] = T(w[q].re / c, w[q].im / c);
-        else w[q] = w[q] / c;
+        T g;
+        if constexpr (Tr<T>::is_complex) g = T(w[q].re / c, w[q].im / c);
+        else g = w[q] / c;
+        w[q] = g;
+        T pv[NV];
+        load_prow<T, NV>(P, q, prow, pv);
+#pragma unroll
+        for (int i = 0; i < NV; i++)
+            if (i < prow) e[i] = kfma(pv[i], g, e[i]);
     }
+    block_sum<T, NV>(e, lds);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NV; i++) pout[(long)i * nparts + blockIdx.x] = e[i];
 }
 
 // w = v / d (element-wise, d the M1 diagonal)
@@ -409,10 +428,10 @@ struct Work {
         LAUNCH_CHECK();
         return q;
     }
-    unsigned post_pdot(const T *w, const T *P, int prow, int off) {
-        launch_pdotk(w, P, prow);
+    // nv sums of partials laid out [v][gv] to the result slots at off
+    unsigned post_red(const T *partials, int nv, int off) {
         const unsigned q = ++seq;
-        hipLaunchKernelGGL((k_red_host<T>), dim3(1), dim3(FIN_THREADS), 0, st, part, gv, prow, hres_d + off, hseq_d,
+        hipLaunchKernelGGL((k_red_host<T>), dim3(1), dim3(FIN_THREADS), 0, st, partials, gv, nv, hres_d + off, hseq_d,
                            q);
         LAUNCH_CHECK();
         return q;
@@ -1086,10 +1105,19 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                     } else {
                         w.dot_parts(gn, gn, pa);
                     }
-                    hipLaunchKernelGGL((k_norm_div<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, pa, w.gv, db + s + 1);
+                    // gn /= |gn| fused with the next step's P^T gn partials (in pb)
+#define NDP(NV)                                                                                                     \
+    hipLaunchKernelGGL((k_norm_div_pdot<T, NV>), dim3(w.gv), dim3(256), 0, stream, n, gn, pa, w.gv, db + s + 1, P, s, \
+                       pb, w.gv)
+                    if (s <= 1) NDP(1);
+                    else if (s <= 2) NDP(2);
+                    else if (s <= 4) NDP(4);
+                    else if (s <= 8) NDP(8);
+                    else NDP(16);
+#undef NDP
                     LAUNCH_CHECK();
                     w.post_fetch(db + i0, s + 3 - i0, 0);
-                    w.wait_seq(w.post_pdot(gn, P, s, MAX_RED));
+                    w.wait_seq(w.post_red(pb, s, MAX_RED));
                     const std::vector<H> bv = w.read(s + 3 - i0, 0);
                     for (int i = i0; i <= s; i++) h[i] = h[i] + bv[i - i0];
                     normg = reh(bv[s + 2 - i0]);
