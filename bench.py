@@ -186,7 +186,7 @@ def bench_batched(args):
 
     def mixed_step():
         return idrs_amd.idrs_batched_mixed(AB, b_all, s, tolerance=tol, maximum_iterations=maxit,
-                                           inner_tolerance=1e-5, out=x_mixed)
+                                           inner_tolerance=args.batch_inner_tol, out=x_mixed)
 
     for _ in range(max(args.warmup, 1)):
         mixed_step()
@@ -255,6 +255,7 @@ def bench_batched(args):
                                        "flags": {str(k): int(v) for k, v in enumerate(np.bincount(rmx.flag))},
                                        "worst_relres": float(rmx.relres.max()), "outer_max": int(rmx.outer.max()),
                                        "inner_iterations_median": int(np.median(rmx.iterations)),
+                                       "inner_tolerance": args.batch_inner_tol,
                                        "x_rel_diff_vs_fp64_batch": mixed_diff}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
@@ -632,6 +633,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default product layout")
     ap.add_argument("--no-loop", action="store_true", help="batched form: skip the per-system loop (profiling)")
     ap.add_argument("--inner-tol", type=float, default=1e-9, help="inner tolerance of the mixed form")
+    ap.add_argument("--batch-inner-tol", type=float, default=1e-5,
+                    help="inner tolerance of the batched form's mixed-precision solve")
     ap.add_argument("--mixed-vectors", choices=["fp64", "fp32"], default="fp64",
                     help="inner vectors of the mixed form (matrix values are FP32 either way)")
     ap.add_argument("--ms-grid", type=int, default=128, help="grid side of the multishift form")
