@@ -261,3 +261,22 @@ def test_polynomial_multishift_sequence(gpu, solver):
     g = solve(A, b, sig_p, 4, tolerance=0.0, maximum_iterations=12)
     for k in range(len(sig)):
         assert rel(g.x[:, k], o["x"][:, k]) < 1e-10
+
+
+@pytest.mark.parametrize("solver", ["msidrs", "msqmridr"])
+def test_many_shifts_coefficients_in_memory(gpu, solver):
+    """40 shifts with s = 8: the per-shift update coefficients (40 x 12)
+    exceed what a launch argument holds and go through device memory; the
+    fixed-count solve matches the oracle's, and the final residuals come
+    from five block products of 8 shifts."""
+    crs, b = _cdr(8, pe=5.0, seed=12)
+    A, Ao = gpu_crs(crs), oracle_crs(crs)
+    A.K.set_group(1)   # the sliced copy: block products for the residuals
+    sig = -5.0 * np.arange(40)
+    kw = dict(tolerance=0.0, maximum_iterations=18)
+    g = getattr(idrs_amd, solver)(A, b, sig, 8, **kw)
+    o = getattr(orc, solver)(Ao, b, sig, 8, **kw)
+    assert g.iterations == o["iterations"] and g.flag == o["flag"]
+    for k in range(40):
+        assert rel(g.x[:, k], o["x"][:, k]) < 1e-10, k
+    assert np.allclose(g.relres, o["relres"], rtol=1e-9)
