@@ -143,9 +143,14 @@ def test_msidrs_complex_helmholtz(gpu):
     assert np.all(r.relres <= 1e-9)
 
 
-def test_msqmridr_complex_helmholtz(gpu):
+@pytest.mark.parametrize("sliced", [False, True])
+def test_msqmridr_complex_helmholtz(gpu, sliced):
+    """Complex shifts; with the sliced copy the final residuals of the
+    complex solutions run as block products (real matrix, complex vectors)."""
     crs, b, sig = _helmholtz(8, 4)
     A = gpu_crs(crs)
+    if sliced:
+        A.K.set_group(1)
     Ao = orc.OracleMatrix(orc.CRS, crs.n, crs=crs, complex_vectors=True)
     g = idrs_amd.msqmridr(A, b, sig, 6, tolerance=0.0, maximum_iterations=20, resvec=True)
     o = orc.msqmridr(Ao, b, sig, 6, tolerance=0.0, maximum_iterations=20, resvec=True)
