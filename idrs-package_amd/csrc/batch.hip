@@ -435,7 +435,7 @@ void batch_idrs_solve(const Operator &A, int s, const T *b, T *x, const SolveOpt
                       std::vector<SolveStats> &stats, double &solve_ms, hipStream_t stream) {
     const Partition *part = A.K->part.get();
     if (!part) throw std::invalid_argument("the operator's matrix is not a batch (block-diagonal with a partition)");
-    if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..16");
+    if (s < 1 || s > PDOT_MAX) throw std::invalid_argument("batched solves take s in 1..16");
     if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
     if (o.has_m1 || o.U0 || o.resvec) throw std::invalid_argument("M1, U0 and resvec are not offered for batches");
     const int method = o.variant;
@@ -698,7 +698,7 @@ void batch_mixed_solve(const Operator &A, int s, const double *b, double *x, con
                        hipStream_t stream) {
     const Partition *part = A.K->part.get();
     if (!part) throw std::invalid_argument("the operator's matrix is not a batch (block-diagonal with a partition)");
-    if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..16");
+    if (s < 1 || s > PDOT_MAX) throw std::invalid_argument("batched solves take s in 1..16");
     if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
     if (!(inner_tol > 0 && inner_tol < 1)) throw std::invalid_argument("inner tolerance must be in (0, 1)");
     if (max_outer < 1) throw std::invalid_argument("at least one outer step");

@@ -201,7 +201,7 @@ static int krylov_entry(Kry kind, idrs_matrix_t A, int n, const double *b, int n
                 ko.x0 = dx0.p;
             }
             if (o.PT_raw && pt_rows_s) {
-                need(s >= 1 && s <= SMAX, "s must be in 1..16");
+                need(s >= 1 && s <= SMAX, "s must be in 1..64");
                 dP = DVec<T>((const T *)o.PT_raw, (size_t)n * s, st);
                 ko.PT_raw = dP.p;
             }
@@ -262,7 +262,7 @@ static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opt
                 so.x0 = dx0.p;
             }
             if (o.PT_raw) {
-                need(s >= 1 && s <= SMAX, "s must be in 1..16");
+                need(s >= 1 && s <= PDOT_MAX, "batched solves take s in 1..16");
                 dP = DVec<T>((const T *)o.PT_raw, (size_t)ntot * prow, st);
                 so.PT_raw = dP.p;
             }
@@ -332,7 +332,7 @@ static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idr
                        int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats, int *outer_steps) {
     return guarded([&] {
         need(A && b && x, "null argument");
-        need(s >= 1 && s <= SMAX, "s must be in 1..16");
+        need(s >= 1 && s <= SMAX, "s must be in 1..64");
         hipStream_t st = lib_stream();
         op_prepare(A->op, st);
         idrs_opts_t o{};

@@ -177,7 +177,7 @@ __device__ inline void block_sum(T (&v)[NV], T *lds) {
 // Reduce partials laid out [q * nparts + p] over p, for q < nv <= MAX_RED,
 // with one block in one pass; deterministic.  Result in out[q] (shared
 // memory), valid for all threads after return.
-constexpr int MAX_RED = 20;
+constexpr int MAX_RED = 68;   // >= SMAX + 4 (idrs.hpp)
 constexpr int FIN_THREADS = 1024;   // block size of the one-block reduction kernels
 template <class T>
 __device__ inline void reduce_partials(const T *__restrict__ partials, int nparts, int nv, T *out) {

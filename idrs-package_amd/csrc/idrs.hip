@@ -23,6 +23,7 @@
 // epilogue, fin_om, omega_update (r, x, ||r||^2 and the next P^T r), fin_cycle.
 #include "idrs.hpp"
 #include "idrs_state.hpp"
+#include "pdot.hpp"
 
 #include <cfloat>
 #include <map>
@@ -185,6 +186,28 @@ __global__ __launch_bounds__(256) void k_gs_dots(long n, int s, int j, const T *
     block_sum<T, NV>(e, lds);
     if (threadIdx.x == 0)
         for (int k = 0; k < NV; k++) partials[(long)k * nparts + blockIdx.x] = e[k];
+}
+
+// k_gs_dots for rows j above PDOT_MAX: grid row y sums the coefficients
+// k in [y*PDOT_MAX, min(j, (y+1)*PDOT_MAX)) with k_gs_dots' arithmetic
+template <class T>
+__global__ __launch_bounds__(256) void k_gs_dots_wide(long n, int s, int j, const T *__restrict__ PT, T *partials,
+                                                      int nparts) {
+    constexpr int NV = PDOT_MAX;
+    __shared__ T lds[NV * 4];
+    const int k0 = blockIdx.y * NV, cnt = min(NV, j - k0);
+    T e[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) e[k] = T(0);
+    GRID_LOOP(q, n) {
+        const T pj = PT[j + (long)s * q];
+#pragma unroll
+        for (int k = 0; k < NV; k++)
+            if (k < cnt) e[k] = fmaT(Tr<T>::conj(PT[k0 + k + (long)s * q]), pj, e[k]);
+    }
+    block_sum<T, NV>(e, lds);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < cnt; k++) partials[(long)(k0 + k) * nparts + blockIdx.x] = e[k];
 }
 
 template <class T>
@@ -452,6 +475,7 @@ __global__ __launch_bounds__(256) void k_omega_update(long n, const State<T> *st
 template <class T>
 static void launch_pdot(long n, const T *w, const T *P, int prow, int conjP, T *partials, hipStream_t st) {
     int g = vec_grid(n);
+    if (prow > PDOT_MAX) return launch_pdot_wide<T>(n, w, P, prow, conjP, partials, 0, nullptr, st);
     if (prow <= 1) hipLaunchKernelGGL((k_pdot<T, 1>), dim3(g), dim3(256), 0, st, n, w, P, prow, conjP, partials, g);
     else if (prow <= 2) hipLaunchKernelGGL((k_pdot<T, 2>), dim3(g), dim3(256), 0, st, n, w, P, prow, conjP, partials, g);
     else if (prow <= 4) hipLaunchKernelGGL((k_pdot<T, 4>), dim3(g), dim3(256), 0, st, n, w, P, prow, conjP, partials, g);
@@ -464,6 +488,13 @@ template <class T>
 static void launch_omega_update(long n, const State<T> *st, const T *v, const T *t, T *r, T *x, const T *P, int prow,
                                 int conjP, T *partials, hipStream_t stream) {
     int g = vec_grid(n);
+    if (prow > PDOT_MAX) {
+        // r, x and ||r||^2 here, P^T r from the chunked kernel (values 1..prow)
+        hipLaunchKernelGGL((k_omega_update<T, 1>), dim3(g), dim3(256), 0, stream, n, st, v, t, r, x, P, 0, conjP,
+                           partials, g);
+        LAUNCH_CHECK();
+        return launch_pdot_wide<T>(n, r, P, prow, conjP, partials, 1, &st->info, stream);
+    }
 #define OU(NV) hipLaunchKernelGGL((k_omega_update<T, NV>), dim3(g), dim3(256), 0, stream, n, st, v, t, r, x, P, prow, conjP, partials, g)
     if (prow <= 1) OU(1);
     else if (prow <= 2) OU(2);
@@ -477,6 +508,12 @@ static void launch_omega_update(long n, const State<T> *st, const T *v, const T 
 template <class T>
 static void launch_gs_dots(long n, int s, int j, const T *PT, T *partials, hipStream_t st) {
     int g = vec_grid(n);
+    if (j > PDOT_MAX) {
+        hipLaunchKernelGGL((k_gs_dots_wide<T>), dim3(g, ceil_div(j, PDOT_MAX)), dim3(256), 0, st, n, s, j, PT,
+                           partials, g);
+        LAUNCH_CHECK();
+        return;
+    }
     if (j <= 1) hipLaunchKernelGGL((k_gs_dots<T, 1>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
     else if (j <= 2) hipLaunchKernelGGL((k_gs_dots<T, 2>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
     else if (j <= 4) hipLaunchKernelGGL((k_gs_dots<T, 4>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
@@ -555,7 +592,7 @@ static T *upload_diag_as(const Operator *M1, long n, Buf &buf, hipStream_t strea
 template <class T>
 int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOpts<T> &o, SolveStats &out,
                hipStream_t stream) {
-    if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..16");
+    if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..64");
     if (n < 1) throw std::invalid_argument("empty system");
     if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
     if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
@@ -654,10 +691,18 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         hipLaunchKernelGGL((k_form_u<T>), dim3(gv), dim3(256), 0, stream, n, 0, st, r, G, U, m1);
         LAUNCH_CHECK();
         for (int k = 0; k < s; k++) {
-            op_apply<T>(A, U + (long)k * n, G + (long)k * n, w1, w2, EPI_PDOT, epd, info, stream);
-            prereduce(prow);
-            hipLaunchKernelGGL((k_fin_mu<T>), dim3(1), dim3(FIN_THREADS), 0, stream, k, st, epart, enparts);
-            LAUNCH_CHECK();
+            if (prow > PDOT_MAX) {
+                // wider than the product epilogue holds: P^T G(:,k) in chunks
+                op_apply<T>(A, U + (long)k * n, G + (long)k * n, w1, w2, EPI_NONE, none, info, stream);
+                launch_pdot_wide<T>(n, G + (long)k * n, P, prow, conjP, part, 0, info, stream);
+                hipLaunchKernelGGL((k_fin_mu<T>), dim3(1), dim3(FIN_THREADS), 0, stream, k, st, part, gv);
+                LAUNCH_CHECK();
+            } else {
+                op_apply<T>(A, U + (long)k * n, G + (long)k * n, w1, w2, EPI_PDOT, epd, info, stream);
+                prereduce(prow);
+                hipLaunchKernelGGL((k_fin_mu<T>), dim3(1), dim3(FIN_THREADS), 0, stream, k, st, epart, enparts);
+                LAUNCH_CHECK();
+            }
             hipLaunchKernelGGL((k_orth_update<T>), dim3(gv), dim3(256), 0, stream, n, k, st, G, U, r, x, m1, part);
             LAUNCH_CHECK();
             hipLaunchKernelGGL((k_fin_norm<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv, resvec);

@@ -5,9 +5,14 @@
 
 namespace idrs {
 
-constexpr int SMAX = 16;             // largest shadow-space dimension s
+constexpr int SMAX = 64;             // largest shadow-space dimension s
+// Largest number of P rows a one-pass P^T y kernel (product epilogue, the
+// update kernels) holds in registers; wider shadow spaces take their P^T y
+// in chunks of PDOT_MAX rows from a separate kernel (one grid row per chunk).
+constexpr int PDOT_MAX = 16;
 constexpr int MAX_PARTS = 32768;     // upper bound of blocks in any reduction grid
 constexpr int MAX_NV = SMAX + 3;     // values reduced per block
+static_assert(MAX_RED >= SMAX + 4, "one-workgroup reductions hold s + 4 values");
 
 struct SolveStats {
     int flag = -1;

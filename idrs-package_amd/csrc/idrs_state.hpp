@@ -58,7 +58,8 @@ __device__ inline void scalar_init(State<T> *st, int s, int method, int prow, in
     st->info = -1;
     st->iter = 0;
     st->jj = 0;
-    for (int i = 0; i < SMAX * SMAX; i++) st->M[i] = T(0);
+    for (int j = 0; j < s; j++)
+        for (int i = 0; i < s; i++) st->M[i + SMAX * j] = T(0);
 }
 
 // ---------------------------------------------------------------------------

@@ -15,6 +15,7 @@
 // reference's column shifts of G and W (G(:,1:s-1) = G(:,2:s), ...) are
 // circular windows here, so no vector is copied to age a column.
 #include "krylov.hpp"
+#include "pdot.hpp"
 
 #include <cfloat>
 #include <chrono>
@@ -44,6 +45,10 @@ inline double reh(std::complex<double> v) { return v.real(); }
 
 constexpr int MAXT = 40;   // terms of one linear-combination launch
 constexpr int MAXS = 64;   // shifts of one fused multishift update
+// per-shift coefficients (s + 4 per shift) staged in LDS by the update
+// kernels; larger sets (wide shadow spaces with many shifts) are read from
+// device memory
+constexpr int COEF_LDS = MAXS * (PDOT_MAX + 4);
 
 __device__ inline double kdiv(double a, double b) { return a / b; }
 __device__ inline c128 kdiv(c128 a, c128 b) { return a / b; }
@@ -247,18 +252,22 @@ struct CoefArg {
 //   t = v - om_s*v0*u(s+2-k) - sum_i W(:,i,s)*c_i;  W(:,k,s) = t/u(s+2);
 //   x(:,s) = x(:,s) + phi_s*W(:,k,s)
 // coef per shift: [om_s, u(s+2-k), c_1..c_s, u(s+2), phi_s]
-template <class T>
+// STAGED: the ns (s + 4) coefficients fit COEF_LDS and are staged in LDS;
+// otherwise they are read from device memory (coef).
+template <class T, bool STAGED>
 __global__ __launch_bounds__(256) void k_msidrs_update(long n, int ns, int s, int k, const T *__restrict__ v,
                                                        const T *__restrict__ v0, T *W, T *X, const T *coef,
                                                        const CoefArg<T> ca) {
-    __shared__ T sc[MAXS * (SMAX + 4)];
+    __shared__ T sc[STAGED ? COEF_LDS : 1];
     const int stride = s + 4;
-    for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef ? coef[i] : ca.c[i];
-    __syncthreads();
+    if constexpr (STAGED) {
+        for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef ? coef[i] : ca.c[i];
+        __syncthreads();
+    }
     KGRID(q, n) {
         const T vq = v[q], v0q = v0[q];
         for (int ks = 0; ks < ns; ks++) {
-            const T *c = sc + ks * stride;
+            const T *c = (STAGED ? sc : coef) + ks * stride;
             T t = vq - (c[0] * v0q) * c[1];
             T *Wk = W + (long)n * s * ks;
             for (int i = 0; i < s; i++) t = kfma(-Wk[(long)n * i + q], c[2 + i], t);
@@ -286,16 +295,18 @@ __global__ void k_ms_xaxpy(long n, int ns, const T *__restrict__ v0, T *X, const
 // W columns are a circular window: logical column j (1..s+1) is physical
 // (head + j - 1) mod (s+1).  coef per shift: [r_1..r_{s+1}, r_{s+2}, phi_s,
 // active (real part != 0)].  vt_s is vt + ks*vt_stride.
-template <class T>
+template <class T, bool STAGED>
 __global__ __launch_bounds__(256) void k_qmr_update(long n, int ns, int s, int head, const T *vt, long vt_stride,
                                                     T *W, T *X, const T *coef, const CoefArg<T> ca) {
-    __shared__ T sc[MAXS * (SMAX + 4)];
+    __shared__ T sc[STAGED ? COEF_LDS : 1];
     const int stride = s + 4, nw = s + 1;
-    for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef ? coef[i] : ca.c[i];
-    __syncthreads();
+    if constexpr (STAGED) {
+        for (int i = threadIdx.x; i < ns * stride; i += blockDim.x) sc[i] = coef ? coef[i] : ca.c[i];
+        __syncthreads();
+    }
     KGRID(q, n) {
         for (int ks = 0; ks < ns; ks++) {
-            const T *c = sc + ks * stride;
+            const T *c = (STAGED ? sc : coef) + ks * stride;
             if (Tr<T>::absv(c[s + 3]) == 0.0) continue;
             T *Ws = W + (long)n * nw * ks;
             T m = T(0);
@@ -541,6 +552,7 @@ struct Work {
         return get(d.nu, gv);
     }
     void launch_pdotk(const T *w, const T *P, int prow) {
+        if (prow > PDOT_MAX) return launch_pdot_wide<T>(n, w, P, prow, 0, part, 0, nullptr, st);
 #define PD(NV) hipLaunchKernelGGL((k_pdotk<T, NV>), dim3(gv), dim3(256), 0, st, n, w, P, prow, part, gv)
         if (prow <= 1) PD(1);
         else if (prow <= 2) PD(2);
@@ -556,6 +568,13 @@ struct Work {
     }
     std::vector<H> axpy_pdot(T *gn, const T *v, const T *t, H om, const T *P, int prow) {
 #define AP(NV) hipLaunchKernelGGL((k_axpy_pdot<T, NV>), dim3(gv), dim3(256), 0, st, n, gn, v, t, to_dev(om), P, prow, part, gv)
+        if (prow > PDOT_MAX) {
+            // gn here, P^T gn from the chunked kernel
+            hipLaunchKernelGGL((k_axpy_pdot<T, 1>), dim3(gv), dim3(256), 0, st, n, gn, v, t, to_dev(om), P, 0, part, gv);
+            LAUNCH_CHECK();
+            launch_pdot_wide<T>(n, gn, P, prow, 0, part, 0, nullptr, st);
+            return get(prow, gv);
+        }
         if (prow <= 1) AP(1);
         else if (prow <= 2) AP(2);
         else if (prow <= 4) AP(4);
@@ -697,7 +716,7 @@ inline int default_maxit(long n) { return (int)std::min<long>(2 * n, 1000); }
 
 template <class T>
 void check_common(const Operator &A, long n, int s, double tol) {
-    if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..16");
+    if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..64");
     if (n < 1) throw std::invalid_argument("empty system");
     if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
     if (tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
@@ -832,8 +851,12 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
                 }
                 CoefArg<T> ca{};
                 const T *dc = w.coefs(coef, ca);
-                hipLaunchKernelGGL((k_msidrs_update<T>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, k - 1, v, v0, W,
-                                   X, dc, ca);
+                if (ns * (s + 4) <= COEF_LDS)
+                    hipLaunchKernelGGL((k_msidrs_update<T, true>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, k - 1,
+                                       v, v0, W, X, dc, ca);
+                else
+                    hipLaunchKernelGGL((k_msidrs_update<T, false>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, k - 1,
+                                       v, v0, W, X, dc, ca);
                 LAUNCH_CHECK();
                 rv(iter);
                 double mx = 0.0;
@@ -1109,7 +1132,13 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
 #define NDP(NV)                                                                                                     \
     hipLaunchKernelGGL((k_norm_div_pdot<T, NV>), dim3(w.gv), dim3(256), 0, stream, n, gn, pa, w.gv, db + s + 1, P, s, \
                        pb, w.gv)
-                    if (s <= 1) NDP(1);
+                    if (s > PDOT_MAX) {
+                        // gn /= |gn| here, then P^T gn from the chunked kernel
+                        hipLaunchKernelGGL((k_norm_div_pdot<T, 1>), dim3(w.gv), dim3(256), 0, stream, n, gn, pa, w.gv,
+                                           db + s + 1, P, 0, pb, w.gv);
+                        LAUNCH_CHECK();
+                        launch_pdot_wide<T>(n, gn, P, s, 0, pb, 0, nullptr, stream);
+                    } else if (s <= 1) NDP(1);
                     else if (s <= 2) NDP(2);
                     else if (s <= 4) NDP(4);
                     else if (s <= 8) NDP(8);
@@ -1171,8 +1200,12 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                 const T *dc = w.coefs(coef, ca);
                 const T *vsrc = (multishift && in_it > 0) ? Vt : vtil;
                 const long vstride = (multishift && in_it > 0) ? n : 0;
-                hipLaunchKernelGGL((k_qmr_update<T>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, whead, vsrc, vstride,
-                                   W, X, dc, ca);
+                if (ns * (s + 4) <= COEF_LDS)
+                    hipLaunchKernelGGL((k_qmr_update<T, true>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, whead,
+                                       vsrc, vstride, W, X, dc, ca);
+                else
+                    hipLaunchKernelGGL((k_qmr_update<T, false>), dim3(w.gv), dim3(256), 0, stream, n, ns, s, whead,
+                                       vsrc, vstride, W, X, dc, ca);
                 LAUNCH_CHECK();
                 whead = (whead + 1) % nw;
                 if (!multishift && broke) break;                          // qmridr.inc:234-236

@@ -401,3 +401,18 @@ def grot(a, b):
 
 def uniform01(seed, index):
     return load().orc_uniform01(seed, index)
+
+
+def compute_alpha(degree, H):
+    """rcompute_alpha / ccompute_alpha (matrix_module.f90:983-1057) of the
+    Hessenberg matrix H (ldh x ncol); returns alpha(0:degree)."""
+    lib = load()
+    cplx = np.iscomplexobj(H)
+    dt = np.complex128 if cplx else np.float64
+    Hf = np.asfortranarray(H, dtype=dt)
+    a = np.empty(degree + 1, dtype=dt)
+    f = lib.orc_c_compute_alpha if cplx else lib.orc_r_compute_alpha
+    rc = f(c_int(degree), c_void_p(Hf.ctypes.data), c_int(Hf.shape[0]), c_int(Hf.shape[1]), c_void_p(a.ctypes.data))
+    if rc != 0:
+        raise ValueError("H too small for the degree")
+    return a

@@ -322,3 +322,113 @@ def test_complex_toeplitz_accuracy():
         x, flag, relres, it, _ = orc.idrs(A, b, s, tolerance=1e-12, maximum_iterations=4000)
         assert flag == 0
         assert np.linalg.norm(b - D @ x) / np.linalg.norm(b) <= 1e-12
+
+
+# ------------------------------------------------------- COMP_OM kappa ---
+
+def _omega_step_ratios(crs, b, s, cycles, cplx=False):
+    """(rho, ||r_new||/||r||) of the first omega steps of IDR(s), measured
+    from outside: x before the omega step from a run stopped there, r = b -
+    K x, t = K r (no preconditioner), rho = |(t,r)|/(||t|| ||r||); the ratio
+    from the recursive residual norms of a run one step longer."""
+    A = orc.OracleMatrix(orc.CRS, crs.n, crs=crs, complex_vectors=cplx)
+    K = crs.to_scipy()
+    out = []
+    for c in range(cycles):
+        itb = (s + 1) * c + s
+        xb, _, _, it, _ = orc.idrs(A, b, s, tolerance=0.0, maximum_iterations=itb)
+        assert it == itb
+        r = b - K @ xb
+        t = K @ r
+        rho = abs(np.vdot(t, r)) / (np.linalg.norm(t) * np.linalg.norm(r))
+        rv = orc.idrs(A, b, s, tolerance=0.0, maximum_iterations=itb + 1, resvec=True)[4]
+        out.append((rho, rv[itb + 1] / rv[itb]))
+    return out
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_comp_om_kappa_rule(cplx):
+    """The omega step's residual ratio is fixed by rho and kappa = 0.7
+    (COMP_OM, idrs_module.f90:740-748): with om = (t,r)/(t,t), boosted by
+    kappa/rho when rho < kappa, ||r - om t||/||r|| = sqrt(1 - 2 kappa rho +
+    kappa^2) if rho < kappa, else sqrt(1 - rho^2).  The problems give omega
+    steps with rho below 0.5, in (0.5, 0.7) and above 0.7, so a dropped boost,
+    a different threshold or a different boost factor fails."""
+    kappa = float(np.float32(0.7))
+    rng = np.random.default_rng(0)
+    cases = [P.convection_diffusion_3d(6, 20.0, reaction=1.0)[0],   # rho in 0.63..0.79
+             P.convection_diffusion_3d(6, 200.0)[0],                # rho ~ 0.1
+             P.random_dominant(300, 6, seed=1, factor=1.05)]        # rho > 0.7
+    seen = []
+    for crs in cases:
+        b = rng.standard_normal(crs.n)
+        if cplx:
+            b = b + 1j * rng.standard_normal(crs.n)
+        for rho, ratio in _omega_step_ratios(crs, b, 2, 6, cplx):
+            pred = np.sqrt(1 - 2 * kappa * rho + kappa ** 2) if rho < kappa else np.sqrt(1 - rho ** 2)
+            assert ratio == pytest.approx(pred, rel=1e-9), (rho, ratio, pred)
+            seen.append(rho)
+    seen = np.array(seen)
+    assert (seen < 0.5).any() and ((seen > 0.5) & (seen < kappa)).any() and (seen > kappa).any()
+
+
+# ------------------------------------------------------------ seed term ---
+
+@pytest.mark.parametrize("seed,degree", [(0.3, 0), (-1.7, 3), (0.4 - 0.9j, 0), (1.1 + 0.5j, 4)])
+def test_seeded_operator_closed_form(seed, degree):
+    """real_mv / complex_mv with a seed (matrix_module.f90:241-262):
+    w = K t - seed t with t = v/D, so A = (K - seed I) D^-1 p((K - seed I) D^-1)
+    against a dense evaluation; with a sign error in the seed term the
+    products differ by 2 seed D^-1 v."""
+    crs, _ = P.convection_diffusion_3d(4, 15.0, reaction=0.5)
+    n = crs.n
+    cplx = np.iscomplexobj(seed)
+    K = dense(crs)
+    d = np.diag(K).copy()
+    alpha = orc.chebyshev(degree, np.array([0.2, 1.8], dtype=np.complex128 if cplx else np.float64)) if degree else None
+    Ao = orc.OracleMatrix(orc.CRS, n, crs=crs, which_precon=1, degree=degree, alpha=alpha, seed=seed,
+                          complex_vectors=cplx)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(n) + (1j * rng.standard_normal(n) if cplx else 0)
+    S = (K - seed * np.eye(n)) / d[None, :]
+    pv = v if not degree else sum(alpha[i] * np.linalg.matrix_power(S, i) @ v for i in range(degree + 1))
+    ref = S @ pv
+    w = Ao.matvec(v)
+    assert np.allclose(w, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+    wrong = ((K + seed * np.eye(n)) / d[None, :]) @ pv
+    assert not np.allclose(w, wrong, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
+
+
+# -------------------------------------------------------- compute_alpha ---
+
+def hessenberg_zero_colsum(d, seed, cplx=False):
+    """(d+2) x (d+1) upper Hessenberg matrix whose columns sum to zero, the
+    form of the residual-polynomial recurrence t r_{i-1} = sum_j H(j,i) r_{j-1}
+    with r_i(0) = 1 that compute_alpha reads."""
+    rng = np.random.default_rng(seed)
+    H = np.triu(rng.standard_normal((d + 2, d + 1)) + 2 * np.eye(d + 2, d + 1), -1)
+    if cplx:
+        H = H + 1j * np.triu(rng.standard_normal((d + 2, d + 1)), -1)
+    H[d + 1, d] = 0
+    for j in range(d + 1):
+        H[j + 1, j] = -H[: j + 1, j].sum()
+    return H
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("d", [1, 3, 6])
+def test_compute_alpha_roots_are_hessenberg_eigenvalues(d, cplx):
+    """1 - t p(t) = r_{d+1}(t) is the characteristic polynomial of the square
+    part H(1:d+1, 1:d+1), scaled to r(0) = 1: its roots are that block's
+    eigenvalues (matrix_module.f90:983-1057)."""
+    H = hessenberg_zero_colsum(d, 10 + d, cplx)
+    a = orc.compute_alpha(d, H)
+    q = np.concatenate([[1.0], -a])           # 1 - t p(t), ascending powers
+    roots = np.sort_complex(np.polynomial.polynomial.polyroots(q))
+    ev = np.sort_complex(np.linalg.eigvals(H[: d + 1, : d + 1]))
+    assert np.allclose(roots, ev, rtol=1e-8, atol=1e-9)
+    # and the monic form: the polynomial equals det(tI - H)/det(-H)
+    t = np.array([0.3, -1.1, 2.5])
+    lhs = np.polynomial.polynomial.polyval(t, q)
+    rhs = [np.linalg.det(tt * np.eye(d + 1) - H[: d + 1, : d + 1]) / np.linalg.det(-H[: d + 1, : d + 1]) for tt in t]
+    assert np.allclose(lhs, rhs, rtol=1e-9, atol=1e-10)
