@@ -13,9 +13,10 @@ achieved = algorithmic bytes of the SpMV launches of one solve / their summed
 HIP event time (events on the solver's stream around each operator
 application, i.e. its d+1 back-to-back SpMV launches; an extra solve after
 the timed steps, so the events do not slow the timed ones), against the
-8 TB/s HBM peak.  cpu_baseline: the oracle (the
-project's scalar C restatement of the reference) on a bounded sample of the
-same workload, extrapolated to the GPU's iteration count.
+8 TB/s HBM peak.  cpu_baseline: the oracle (the project's C restatement of
+the reference path) built with OpenMP loops, timed on a full solve of the
+same system on the host threads this process may use, with the
+single-thread rate beside it.
 """
 import argparse
 import json
@@ -61,27 +62,47 @@ def iteration_bytes(n, nnz, s, d):
     return b_op + 8 * n * (3 * s * s + 11 * s + 3) / (s + 1)
 
 
-def cpu_baseline(crs, b, PT, s, degree, foci, sample_iters, gpu_iters):
+def cpu_baseline(dia, b, PT, s, degree, foci, sample_iters, gpu_iters):
+    """The oracle's restatement of the reference path timed on the host:
+    a full solve of the same system (DIA storage, as the headline) with the
+    OpenMP build on all the threads this process may use, and beside it the
+    single-thread rate from a sample of the same solve, scaled to the full
+    iteration count."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import orc
 
+    host_cpus = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host_cpus
     alpha = orc.chebyshev(degree, np.asarray(foci))
-    Ao = orc.OracleMatrix(orc.CRS, crs.n, crs=crs, which_precon=1, degree=degree, alpha=alpha)
+    Ao = orc.OracleMatrix(orc.DIA, dia.vals.shape[0], dia=(dia.offsets, dia.vals), which_precon=1, degree=degree,
+                          alpha=alpha)
+    lib, used = orc.load_timed(threads)
     t0 = time.perf_counter()
-    orc.idrs(Ao, b, s, tolerance=1e-8, maximum_iterations=sample_iters, PT_raw=PT)
-    dt = time.perf_counter() - t0
+    _, flag, relres, its, _ = orc.idrs(Ao, b, s, tolerance=1e-8, maximum_iterations=20000, PT_raw=PT, lib=lib)
+    full = time.perf_counter() - t0
+    print(f"# cpu baseline: {used} threads, {its} iterations, flag {flag}, relres {relres:.2e}, {full:.1f} s",
+          file=sys.stderr, flush=True)
+    lib, _ = orc.load_timed(1)
+    t0 = time.perf_counter()
+    orc.idrs(Ao, b, s, tolerance=1e-8, maximum_iterations=sample_iters, PT_raw=PT, lib=lib)
+    one = time.perf_counter() - t0
     # the sample holds setup + sample_iters iterations + the final residual;
     # one operator application costs about one iteration
-    per_iter = dt / (sample_iters + 2)
-    est_ms = per_iter * (gpu_iters + 2) * 1e3
+    one_full_ms = one / (sample_iters + 2) * (its + 2) * 1e3
+    lib.orc_threads(threads)
     return {
-        "value": round(est_ms, 1),
-        "unit": "ms (time-to-solution, extrapolated)",
-        "cores": 1,
+        "value": round(full * 1e3, 1),
+        "unit": "ms (time-to-solution)",
+        "cores": used,
         "kind": "port",
-        "sample": f"oracle (scalar C restatement) IDR({s})+Chebyshev({degree}) on the same {crs.n}-row system, "
-                  f"{sample_iters} iterations in {dt:.1f} s, scaled to the GPU's {gpu_iters} iterations",
+        "sample": f"the oracle (the C restatement of the reference path) built with OpenMP row and vector loops: "
+                  f"the full IDR({s})+Chebyshev({degree}) solve of the same {dia.vals.shape[0]}-row system in DIA "
+                  f"storage to 1e-8 on {used} threads ({its} iterations, flag {flag}; the GPU took {gpu_iters})",
+        "host_cpus": host_cpus,
+        "iterations": its,
+        "one_thread_ms": round(one_full_ms, 1),
+        "one_thread_sample": f"{sample_iters} iterations on 1 thread in {one:.1f} s, scaled to {its} iterations",
     }
 
 
@@ -115,25 +136,6 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_per_dispatch": a.numel() * 8, "dispatches": 5}))
 
 
-def batched_systems(nsys, seed=0):
-    """The batched workload: nsys 2-D 5-point convection-diffusion systems,
-    side floor(32 * 16^u) with u ~ U(0,1), Pe ~ U[1,100], convection angle
-    ~ U[0, 2 pi), b ~ U[-1,1] (all seeded)."""
-    import numpy as np
-    from idrs_amd import problems as P
-
-    rng = np.random.default_rng(seed)
-    u = rng.uniform(0.0, 1.0, nsys)
-    pe = rng.uniform(1.0, 100.0, nsys)
-    ang = rng.uniform(0.0, 2 * np.pi, nsys)
-    out = []
-    for k in range(nsys):
-        m = int(np.floor(32 * 16 ** u[k]))
-        crs, _ = P.convection_diffusion_2d(m, pe[k], angle=ang[k])
-        out.append((crs, rng.uniform(-1.0, 1.0, crs.n)))
-    return out
-
-
 def bench_batched(args):
     """One batched IDR(2) solve of all systems vs a loop of single-system
     solves on the same device-resident data."""
@@ -143,6 +145,8 @@ def bench_batched(args):
     import idrs_amd
 
     t0 = time.time()
+    from idrs_amd.problems import batched_systems
+
     systems = batched_systems(args.nsys)
     ntot = sum(c.n for c, _ in systems)
     nnz = sum(c.nnz for c, _ in systems)
@@ -201,7 +205,7 @@ def bench_batched(args):
           f"{rmx.relres.max():.2e}, outer steps max {rmx.outer.max()}, ||x_mixed - x_fp64|| / ||x_fp64|| = "
           f"{mixed_diff:.2e}", file=sys.stderr, flush=True)
     if args.no_loop:
-        return
+        return None
     # per-system loop over the same device data
     As = [idrs_amd.Matrix(K) for K in Ks]
     offs = np.cumsum([0] + [c.n for c, _ in systems])
@@ -262,7 +266,7 @@ def bench_batched(args):
                      "kernel": "whole batched solve (algorithmic bytes of each system's iterations)"},
         "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return out
 
 
 def bench_mixed(args):
@@ -322,7 +326,7 @@ def bench_mixed(args):
                    "x_rel_diff_vs_fp64": err, "inner_tolerance": args.inner_tol},
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return out
 
 
 def bench_small(args):
@@ -387,7 +391,7 @@ def bench_small(args):
                    "eager_ms": round(e_ms, 3), "graph_speedup": round(e_ms / g_ms, 2)},
         "roofline": None, "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    return out
 
 
 def bench_adjoint(args):
@@ -491,7 +495,7 @@ def bench_adjoint(args):
                    "fd_worst_rel_err": worst, "fd_solve_flags": sorted(fd_flags), "lanes_per_row": groups},
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return out
 
 
 def bench_multishift(args):
@@ -584,17 +588,22 @@ def bench_multishift(args):
     torch.cuda.synchronize()
     loop_ms = (time.perf_counter() - t) * 1e3
     A.set_seed(0j if np.iscomplexobj(sig) else 0.0)
+    # true residuals of (K - sigma_k I) x_k = b for the multishift solutions
+    # and for the per-shift loop's (recomputed on the host in FP64)
+    Kt = crs.to_scipy()
+    worst, worst_loop = 0.0, 0.0
+    for name, xx in (("msidrs", xm), ("msqmridr", xq)):
+        xh = xx.cpu().numpy()
+        for k, sk in enumerate(sig):
+            res = bh - (Kt @ xh[:, k] - sk * xh[:, k])
+            worst = max(worst, np.linalg.norm(res) / np.linalg.norm(bh))
     if poly:
-        # true residuals of (K - sigma_k I) x_k = b for the multishift solutions
-        Kt = crs.to_scipy()
-        worst = 0.0
-        for name, xx in (("msidrs", xm), ("msqmridr", xq)):
-            xh = xx.cpu().numpy()
-            for k, sk in enumerate(sig):
-                res = bh - (Kt @ xh[:, k] - sk * xh[:, k])
-                worst = max(worst, np.linalg.norm(res) / np.linalg.norm(bh))
-        print(f"# worst true residual of the scaled-back multishift solutions: {worst:.2e}", file=sys.stderr,
-              flush=True)
+        for k, sk in enumerate(sig):
+            xl = xs_loop[k].cpu().numpy()[:, 0]
+            res = bh - (Kt @ xl - sk * xl)
+            worst_loop = max(worst_loop, np.linalg.norm(res) / np.linalg.norm(bh))
+    print(f"# worst true residual: multishift solutions {worst:.2e}, per-shift loop {worst_loop:.2e}",
+          file=sys.stderr, flush=True)
     print(f"# per-shift IDR(4) loop: {loop_ms:.1f} ms, iterations {loop_its}", file=sys.stderr, flush=True)
     ok = [(t_, name) for t_, name, f_ in ((ms_idrs, "msidrs4", rm.flag), (ms_qmr, "msqmridr8", rq.flag)) if f_ == 0]
     best = min(ok)[0] if ok else min(ms_idrs, ms_qmr)
@@ -607,10 +616,64 @@ def bench_multishift(args):
                    "msidrs4_ms": round(ms_idrs, 2), "msidrs4_iterations": rm.iterations, "msidrs4_flag": rm.flag,
                    "msqmridr8_ms": round(ms_qmr, 2), "msqmridr8_iterations": rq.iterations, "msqmridr8_flag": rq.flag,
                    "per_shift_loop_ms": round(loop_ms, 2), "per_shift_iterations": loop_its,
-                   "speedup_vs_loop": round(loop_ms / best, 2), "lanes_per_row": group},
+                   "speedup_vs_loop": round(loop_ms / best, 2), "lanes_per_row": group,
+                   "worst_true_relres_multishift": worst, "worst_true_relres_loop": worst_loop},
         "roofline": None, "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return out
+
+
+def run_forms(args):
+    """The north-star forms beside the headline solve, one timed step each
+    after one warm-up: the batched solve (and its mixed-precision variant)
+    against the per-system loop, the multishift solvers against the
+    per-shift loop, the mixed-precision 200^3 solve against the FP64 one, and
+    the differentiable solve (step time and the worst finite-difference
+    error of its gradients).  A form that fails reports its error."""
+    fargs = argparse.Namespace(**vars(args))
+    fargs.steps, fargs.warmup = 1, 1
+    fargs.format = "dia"
+    summary = {}
+
+    def run(name, fn, pick):
+        t0 = time.time()
+        print(f"# form {name} ...", file=sys.stderr, flush=True)
+        try:
+            out = fn(fargs)
+            summary[name] = pick(out)
+        except Exception as e:  # report, keep the headline line
+            summary[name] = {"error": f"{type(e).__name__}: {e}"}
+        summary[name]["wall_s"] = round(time.time() - t0, 1)
+
+    def batched(o):
+        c = o["config"]
+        return {"ms": o["value"], "per_system_loop_ms": c["per_system_loop_ms"], "speedup_vs_loop": c["speedup_vs_loop"],
+                "flags": c["flags"], "mixed_ms": c["mixed_precision"]["ms"],
+                "mixed_speedup_vs_loop": c["mixed_precision"]["speedup_vs_loop"],
+                "mixed_worst_relres": c["mixed_precision"]["worst_relres"], "roofline_frac": o["roofline"]["frac"]}
+
+    def multishift(o):
+        c = o["config"]
+        return {"ms": o["value"], "per_shift_loop_ms": c["per_shift_loop_ms"], "speedup_vs_loop": c["speedup_vs_loop"],
+                "msidrs4_ms": c["msidrs4_ms"], "msqmridr8_ms": c["msqmridr8_ms"],
+                "worst_true_relres_multishift": c["worst_true_relres_multishift"],
+                "worst_true_relres_loop": c["worst_true_relres_loop"], "workload": c["workload"]}
+
+    def mixed(o):
+        c = o["config"]
+        return {"ms": o["value"], "fp64_ms": c["fp64_ms"], "speedup_vs_fp64": c["speedup_vs_fp64"],
+                "relres": c["relres"], "x_rel_diff_vs_fp64": c["x_rel_diff_vs_fp64"], "dtype": o["dtype"]}
+
+    def adjoint(o):
+        c = o["config"]
+        return {"step_ms": o["value"], "forward_ms": c["forward_ms"], "fd_directions": c["fd_directions"],
+                "fd_worst_rel_err": c["fd_worst_rel_err"], "fd_solve_flags": c["fd_solve_flags"]}
+
+    run("batched", bench_batched, batched)
+    run("multishift", bench_multishift, multishift)
+    run("mixed", bench_mixed, mixed)
+    run("adjoint", bench_adjoint, adjoint)
+    return summary
 
 
 def main():
@@ -642,8 +705,9 @@ def main():
     ap.add_argument("--ms-problem", choices=["helmholtz", "cdr", "laplacian"], default="laplacian")
     ap.add_argument("--ms-degree", type=int, default=8, help="Chebyshev degree of the laplacian multishift problem")
     ap.add_argument("--ms-maxit", type=int, default=20000)
-    ap.add_argument("--cpu-sample-iters", type=int, default=6)
+    ap.add_argument("--cpu-sample-iters", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="headline solve only (no batched/multishift/... forms)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of known size (scales rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -655,20 +719,12 @@ def main():
               "(halo exchange per product, one all-reduce per reduction point) is designed, not built or "
               "measured.", file=sys.stderr)
         sys.exit(2)
-    if args.form == "batched":
-        bench_batched(args)
-        return
-    if args.form == "mixed":
-        bench_mixed(args)
-        return
-    if args.form == "multishift":
-        bench_multishift(args)
-        return
-    if args.form == "small":
-        bench_small(args)
-        return
-    if args.form == "adjoint":
-        bench_adjoint(args)
+    forms = {"batched": bench_batched, "mixed": bench_mixed, "multishift": bench_multishift, "small": bench_small,
+             "adjoint": bench_adjoint}
+    if args.form in forms:
+        out = forms[args.form](args)
+        if out is not None:
+            print(json.dumps(out))
         return
 
     import numpy as np
@@ -773,7 +829,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline sample ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(crs, b_h, PT_h, s, d, foci, args.cpu_sample_iters, head["iterations"])
+        cpu = cpu_baseline(dia, b_h, PT_h, s, d, foci, args.cpu_sample_iters, head["iterations"])
+
+    forms = None
+    if not args.no_forms:
+        forms = run_forms(args)
 
     config = {
         "workload": f"{m}^3 convection-diffusion-reaction (Pe 50, reaction 1), FP64, IDR({s}) + "
@@ -807,6 +867,7 @@ def main():
         "config": config,
         "roofline": head["roofline"],
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(out))
 

@@ -5,6 +5,7 @@
 #include "krylov.hpp"
 #include "batch.hpp"
 #include "mixed.hpp"
+#include "mm.hpp"
 
 #include <cstdarg>
 #include <cstring>
@@ -518,6 +519,39 @@ int idrs_coo_format_c128(int nrows, int ncols, int nnz, const int *indx, const i
                          idrs_user_matrix_t *A) {
     return MAKE_UM((need(((indx && jndx) || nnz == 0) && cval, "null array"),
                     make_coo(nrows, ncols, nnz, indx, jndx, cval, ValKind::Complex)));
+}
+int idrs_mm_info(const char *path, int *rep, int *field, int *symm, int *nrows, int *ncols, int64_t *entries) {
+    return guarded([&] {
+        need(path != nullptr, "null path");
+        const MMInfo in = mm_info(path);
+        if (rep) *rep = in.rep;
+        if (field) *field = in.field;
+        if (symm) *symm = in.symm;
+        if (nrows) *nrows = in.nrows;
+        if (ncols) *ncols = in.ncols;
+        if (entries) *entries = in.entries;
+    });
+}
+int idrs_mm_matrix(const char *path, int format, idrs_user_matrix_t *A) {
+    return MAKE_UM((need(path != nullptr, "null path"),
+                    need(format == 2 || format == 3, "mm_matrix format must be 2 (crs) or 3 (coo)"), [&] {
+                        const MMCoo c = mm_coo(path);
+                        const ValKind k = c.info.field == MM_COMPLEX   ? ValKind::Complex
+                                          : c.info.field == MM_PATTERN ? ValKind::Pattern
+                                                                       : ValKind::Real;
+                        DevMatrix *M = make_coo(c.info.nrows, c.info.ncols, (long)c.indx.size(), c.indx.data(),
+                                                c.jndx.data(), c.val.data(), k);
+                        M->orig_format = format;
+                        return M;
+                    }()));
+}
+int idrs_mm_dense(const char *path, double *vals) {
+    return guarded([&] {
+        need(path && vals, "null argument");
+        MMInfo in;
+        const std::vector<double> a = mm_dense(path, in);
+        std::memcpy(vals, a.data(), sizeof(double) * a.size());
+    });
 }
 int idrs_cds_format_f64(int nrows, int ncols, int half_band, const double *rband, idrs_user_matrix_t *A) {
     return MAKE_UM(make_cds(nrows, ncols, half_band, rband, kind_of(rband, false)));

@@ -21,11 +21,14 @@
 //   fin_norm    : ||r||, iteration count, convergence test
 // and the omega step is: SpMV chain of A*r with the (r,r),(t,t),(t,r)
 // epilogue, fin_om, omega_update (r, x, ||r||^2 and the next P^T r), fin_cycle.
+// Shadow spaces wider than PDOT_MAX rows take their P^T sums from a chunked
+// kernel (pdot.hpp) instead of the product epilogue and the update kernel.
 #include "idrs.hpp"
 #include "idrs_state.hpp"
 #include "pdot.hpp"
 
 #include <cfloat>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -63,20 +66,28 @@ void pool_free(void *p, size_t bytes) {
     std::lock_guard<std::mutex> g(g_pool_mu);
     g_pool.emplace(bytes, p);
 }
+// a device buffer from the caching pool, or (with an arena) a piece of the
+// caller's region that is not freed
 struct Buf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool pooled = false;
     Buf() = default;
-    explicit Buf(size_t b) : bytes(b) { p = pool_alloc(b); }
+    explicit Buf(size_t b, Arena *ar = nullptr) : bytes(b), pooled(ar == nullptr) {
+        p = ar ? ar->take(b) : pool_alloc(b);
+    }
     Buf(const Buf &) = delete;
     Buf &operator=(const Buf &) = delete;
-    Buf(Buf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; }
+    Buf(Buf &&o) noexcept : p(o.p), bytes(o.bytes), pooled(o.pooled) { o.p = nullptr; }
     Buf &operator=(Buf &&o) noexcept {
         std::swap(p, o.p);
         std::swap(bytes, o.bytes);
+        std::swap(pooled, o.pooled);
         return *this;
     }
-    ~Buf() { pool_free(p, bytes); }
+    ~Buf() {
+        if (pooled) pool_free(p, bytes);
+    }
     template <class V> V *as() const { return (V *)p; }
 };
 }  // namespace
@@ -566,16 +577,16 @@ void make_p_device(long n, int s, int method, const T *PT_raw, const T *r, T *PT
 }
 
 template <class T>
-static T *upload_diag_as(const Operator *M1, long n, Buf &buf, hipStream_t stream);
+static T *upload_diag_as(const Operator *M1, long n, Buf &buf, hipStream_t stream, Arena *ar);
 
 template <class T> __global__ void k_diag_to_T(long n, const double *d, T *out) { GRID_LOOP(q, n) out[q] = T(d[q]); }
 __global__ void k_cdiag_to_c(long n, const c128 *d, c128 *out) { GRID_LOOP(q, n) out[q] = d[q]; }
 
 template <class T>
-static T *upload_diag_as(const Operator *M1, long n, Buf &buf, hipStream_t stream) {
+static T *upload_diag_as(const Operator *M1, long n, Buf &buf, hipStream_t stream, Arena *ar) {
     if (!M1 || M1->which_precon != 1 || !M1->pdiag) return nullptr;
     if (M1->K->nrows != n) throw std::invalid_argument("M1 size does not match the system");
-    buf = Buf(sizeof(T) * n);
+    buf = Buf(sizeof(T) * n, ar);
     int g = vec_grid(n);
     if (M1->pdiag_kind == ValKind::Complex) {
         if constexpr (std::is_same<T, c128>::value)
@@ -609,36 +620,65 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         throw std::invalid_argument("Length of vector with residual norms too small, should be maxit+1");
     const int user_omega = (o.omega && o.n_omega >= 1) ? 1 : 0;
 
+    Arena *ar = o.arena;
+    SpmvTimer &tm = spmv_timer();
+    if (ar && tm.enabled) throw std::invalid_argument("SpMV timing needs events: not in the allocation-free solve");
+    const int gv = vec_grid(n);
+    const int gs = spmv_grid(*A.K);
     const size_t vb = sizeof(T) * (size_t)n;
-    Buf bG(vb * s), bU(vb * s), br(vb), bt(vb), bw1(vb), bw2(vb), btmp(vb), bP(vb * prow);
-    Buf bpart(sizeof(T) * MAX_NV * MAX_PARTS), bst(sizeof(State<T>)), bres(sizeof(double) * (cap_it + 2));
-    Buf bpart2(sizeof(T) * MAX_NV * PRE);
+    // block partials [value][block]: every value of the widest kernel (its
+    // NV template, the chunked P^T y above PDOT_MAX rows) for the largest grid
+    const size_t part_cnt = ar ? (size_t)(std::max(s, PDOT_MAX) + 4) * (size_t)std::max(gv, gs)
+                               : (size_t)MAX_NV * MAX_PARTS;
+    const bool has_m1_diag = o.has_m1 && o.M1 && o.M1->which_precon == 1 && o.M1->pdiag;
+    if (ar) {
+        // the same pieces in the same order on a copy of the arena: a region
+        // too small is reported before anything runs
+        Arena sim{nullptr, ar->cap, ar->used};
+        for (size_t b : {vb * s, vb * s, vb, vb, vb, vb, vb, vb * prow, sizeof(T) * part_cnt, sizeof(State<T>),
+                         sizeof(double) * (cap_it + 2), sizeof(T) * MAX_NV * PRE})
+            sim.take(b);
+        if (has_m1_diag) {
+            sim.take(sizeof(T) * n);
+            sim.take(vb);
+        }
+        if (user_omega) sim.take(sizeof(T) * o.n_omega);
+    }
+    Buf bG(vb * s, ar), bU(vb * s, ar), br(vb, ar), bt(vb, ar), bw1(vb, ar), bw2(vb, ar), btmp(vb, ar), bP(vb * prow, ar);
+    Buf bpart(sizeof(T) * part_cnt, ar), bst(sizeof(State<T>), ar), bres(sizeof(double) * (cap_it + 2), ar);
+    Buf bpart2(sizeof(T) * MAX_NV * PRE, ar);
     Buf bv, bm1, bom;
     T *G = bG.as<T>(), *U = bU.as<T>(), *r = br.as<T>(), *t = bt.as<T>(), *w1 = bw1.as<T>(), *w2 = bw2.as<T>();
     T *tmp = btmp.as<T>(), *P = bP.as<T>(), *part = bpart.as<T>();
     State<T> *st = bst.as<State<T>>();
     double *resvec = bres.as<double>();
-    const T *m1 = o.has_m1 ? upload_diag_as<T>(o.M1, n, bm1, stream) : nullptr;
+    const T *m1 = o.has_m1 ? upload_diag_as<T>(o.M1, n, bm1, stream, ar) : nullptr;
     T *v = r;
     if (m1) {
-        bv = Buf(vb);
+        bv = Buf(vb, ar);
         v = bv.as<T>();
     }
     T *omega_d = nullptr;
     if (user_omega) {
-        bom = Buf(sizeof(T) * o.n_omega);
+        bom = Buf(sizeof(T) * o.n_omega, ar);
         omega_d = bom.as<T>();
         HIP_TRY(hipMemcpyAsync(omega_d, o.omega, sizeof(T) * o.n_omega, hipMemcpyHostToDevice, stream));
     }
 
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventRecord(e0, stream));
-    SpmvTimer &tm = spmv_timer();
+    // solve time: HIP events, or (allocation-free) the host clock around a
+    // synchronised solve
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::chrono::steady_clock::time_point t0;
+    if (ar) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        t0 = std::chrono::steady_clock::now();
+    } else {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        HIP_TRY(hipEventRecord(e0, stream));
+    }
     tm.reset();
 
-    const int gv = vec_grid(n);
     hipLaunchKernelGGL((k_init<T>), dim3(1), dim3(64), 0, stream, st, s, method, prow, conjP, maxit, kappa, o.tol,
                        user_omega, user_omega ? o.n_omega : 0, o.U0 ? 1 : 0, (int)(cap_it + 2));
     LAUNCH_CHECK();
@@ -674,7 +714,6 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     eom.r = r;
     eom.partials = part;
     const int *info = &st->info;
-    const int gs = spmv_grid(*A.K);
     // the product epilogues' partials (one per row slab): above 4096 slabs a
     // PRE-block first level runs before the one-workgroup scalar kernel
     T *part2 = bpart2.as<T>();
@@ -721,18 +760,33 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         LAUNCH_CHECK();
     };
 
-    // enqueue cycles, polling the device flag one cycle behind
-    int *h_info = nullptr;
-    HIP_TRY(hipHostMalloc((void **)&h_info, 2 * sizeof(int), hipHostMallocDefault));
-    hipEvent_t pe[2];
-    HIP_TRY(hipEventCreateWithFlags(&pe[0], hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&pe[1], hipEventDisableTiming));
+    // enqueue cycles, polling the device flag one cycle behind (pinned
+    // flag, events); the allocation-free solve instead reads the flag after
+    // every cycle with a blocking copy into host memory it already has
+    int h_local[2] = {-1, -1};
+    int *h_info = h_local;
+    hipEvent_t pe[2] = {nullptr, nullptr};
+    if (!ar) {
+        HIP_TRY(hipHostMalloc((void **)&h_info, 2 * sizeof(int), hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&pe[0], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&pe[1], hipEventDisableTiming));
+    }
+    auto release = [&]() {
+        if (!ar) {
+            (void)hipHostFree(h_info);
+            (void)hipEventDestroy(pe[0]);
+            (void)hipEventDestroy(pe[1]);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    };
     long max_cycles = cap_it / (s + 1) + 2;
     // Small systems are bound by the host's launch rate (~55 launches per
     // cycle): capture one cycle in a HIP graph and replay it -- the same
     // kernels with the same arguments in the same order.  Not while timing
-    // single SpMVs (events between launches).
-    const bool use_graph = !tm.enabled && (o.graph_mode == 1 || (o.graph_mode == 0 && n <= (1L << 21)));
+    // single SpMVs (events between launches), nor in the allocation-free
+    // solve (instantiating a graph allocates).
+    const bool use_graph = !ar && !tm.enabled && (o.graph_mode == 1 || (o.graph_mode == 0 && n <= (1L << 21)));
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     if (use_graph) {
@@ -743,11 +797,7 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
-            (void)hipHostFree(h_info);
-            (void)hipEventDestroy(pe[0]);
-            (void)hipEventDestroy(pe[1]);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            release();
             throw;
         }
         HIP_TRY(hipStreamEndCapture(stream, &graph));
@@ -758,6 +808,12 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         for (long c = 0; c < max_cycles && !done; c++) {
             if (use_graph) HIP_TRY(hipGraphLaunch(gexec, stream));
             else enqueue_cycle();
+            if (ar) {
+                HIP_TRY(hipMemcpyAsync(&h_info[0], info, sizeof(int), hipMemcpyDeviceToHost, stream));
+                HIP_TRY(hipStreamSynchronize(stream));
+                if (h_info[0] >= 0) done = true;
+                continue;
+            }
             HIP_TRY(hipMemcpyAsync(&h_info[c % 2], info, sizeof(int), hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipEventRecord(pe[c % 2], stream));
             if (c >= 1) {
@@ -771,16 +827,17 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
         LAUNCH_CHECK();
         hipLaunchKernelGGL((k_fin_final<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, part, gv);
         LAUNCH_CHECK();
-        HIP_TRY(hipEventRecord(e1, stream));
+        if (!ar) HIP_TRY(hipEventRecord(e1, stream));
         State<T> hs;
         HIP_TRY(hipMemcpyAsync(&hs, st, sizeof(State<T>), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
+        float ms = 0.f;
+        if (ar) ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        else HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         if (o.resvec) {
             long cnt = std::min<long>(std::min<long>(o.resvec_len, hs.iter + 1), cap_it + 2);
             HIP_TRY(hipMemcpy(o.resvec, resvec, sizeof(double) * cnt, hipMemcpyDeviceToHost));
         }
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         out.flag = hs.info < 0 ? 1 : hs.info;
         out.iterations = hs.iter;
         out.relres = hs.normr / hs.normb;
@@ -790,20 +847,12 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     } catch (...) {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
-        (void)hipHostFree(h_info);
-        (void)hipEventDestroy(pe[0]);
-        (void)hipEventDestroy(pe[1]);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        release();
         throw;
     }
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    (void)hipHostFree(h_info);
-    (void)hipEventDestroy(pe[0]);
-    (void)hipEventDestroy(pe[1]);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    release();
     return 0;
 }
 

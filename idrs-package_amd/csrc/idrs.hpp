@@ -3,6 +3,8 @@
 
 #include "matrix.hpp"
 
+#include <stdexcept>
+
 namespace idrs {
 
 constexpr int SMAX = 64;             // largest shadow-space dimension s
@@ -23,6 +25,21 @@ struct SolveStats {
     int spmv_launches = 0;
 };
 
+// Device memory carved out of one region the caller owns (the
+// allocation-free solve): 256-byte aligned pieces taken in order, never
+// freed one by one; asking for more than the region holds throws.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t bytes) {
+        const size_t at = (used + 255) & ~(size_t)255;
+        if (at + bytes > cap) throw std::invalid_argument(
+            "the solve needs more workspace than the scratch region holds (a smaller n or s fits)");
+        used = at + bytes;
+        return base + at;
+    }
+};
+
 // Solver options; pointers are DEVICE pointers unless noted.
 template <class T>
 struct SolveOpts {
@@ -41,6 +58,10 @@ struct SolveOpts {
     const T *PT_raw = nullptr;   // device, raw shadow space before orthonormalisation
     unsigned long long p_seed = 1;   // default stream seed (reference: this_image() = 1)
     int graph_mode = 0;              // 0: HIP graph per cycle when n <= 2^21, 1: always, 2: never
+    // set: every device buffer of the solve comes from this region, and the
+    // solve makes no allocation of any kind (no graph, no events, no pinned
+    // host memory: the host polls the flag with a blocking copy per cycle)
+    Arena *arena = nullptr;
 };
 
 template <class T>

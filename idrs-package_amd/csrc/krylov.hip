@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) void k_qmr_update(long n, int ns, int s, int h
 // sequence number (as k_red_host)
 template <class T>
 __global__ void k_fetch_host(const T *src, int nv, T *out, unsigned *seq_out, unsigned seq) {
-    if (threadIdx.x < nv) out[threadIdx.x] = src[threadIdx.x];
+    for (int i = threadIdx.x; i < nv; i += blockDim.x) out[i] = src[i];
     __threadfence_system();
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(seq_out, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);

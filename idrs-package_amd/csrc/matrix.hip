@@ -1488,9 +1488,14 @@ void build_sell(DevMatrix &K, hipStream_t st) {
     K.sell = std::move(S);
 }
 
-void op_prepare(const Operator &op, hipStream_t st) {
+bool op_prepared(const Operator &op) {
     const DevMatrix &K = *op.K;
-    if (K.fmt != DevFmt::CRS || !K.sell || !op.sval || op.sl_sval) return;
+    return K.fmt != DevFmt::CRS || !K.sell || !op.sval || op.sl_sval;
+}
+
+void op_prepare(const Operator &op, hipStream_t st) {
+    if (op_prepared(op)) return;
+    const DevMatrix &K = *op.K;
     op.sl_sval = sliced_copy(*K.sell, op.sval, op.values_complex(), st);
     HIP_TRY(hipStreamSynchronize(st));
 }

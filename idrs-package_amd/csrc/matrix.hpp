@@ -231,6 +231,8 @@ void build_sell(DevMatrix &K, hipStream_t st);
 // makes the operator's sliced scaled values when K has a sliced copy; call
 // before products are captured into a graph
 void op_prepare(const Operator &op, hipStream_t st);
+// true when op_prepare has nothing left to build for op
+bool op_prepared(const Operator &op);
 // block-diagonal CRS matrix of nsys square CRS systems (copies), with its
 // Partition
 DevMatrix *make_blockdiag(int nsys, const DevMatrix *const *Ks);

@@ -97,6 +97,35 @@ module idrs_capi
          type(c_ptr), intent(out) :: A
       end function
 
+      integer(c_int) function idrs_mm_info(path, rep, field, symm, nrows, ncols, entries) &
+            bind(C, name="idrs_mm_info")
+         import :: c_int, c_char, c_int64_t
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), intent(out) :: rep, field, symm, nrows, ncols
+         integer(c_int64_t), intent(out) :: entries
+      end function
+
+      integer(c_int) function idrs_mm_matrix(path, format, A) bind(C, name="idrs_mm_matrix")
+         import :: c_int, c_char, c_ptr
+         character(kind=c_char), intent(in) :: path(*)
+         integer(c_int), value :: format
+         type(c_ptr), intent(out) :: A
+      end function
+
+      integer(c_int) function idrs_user_matrix_info(A, nrows, ncols, nnz, format) &
+            bind(C, name="idrs_user_matrix_info")
+         import :: c_int, c_int64_t, c_ptr
+         type(c_ptr), value :: A
+         integer(c_int), intent(out) :: nrows, ncols, format
+         integer(c_int64_t), intent(out) :: nnz
+      end function
+
+      integer(c_int) function idrs_mm_dense(path, vals) bind(C, name="idrs_mm_dense")
+         import :: c_int, c_char, c_double
+         character(kind=c_char), intent(in) :: path(*)
+         real(c_double), intent(out) :: vals(*)
+      end function
+
       integer(c_int) function idrs_coo_format_f64(nrows, ncols, nnz, indx, jndx, rval, A) &
             bind(C, name="idrs_coo_format_f64")
          import :: c_int, c_double, c_ptr

@@ -11,9 +11,9 @@
 !   REAL_PRECON / COMPLEX_PRECON(1, K)       user_module.f90:1327-1381, 1465-1518
 !
 ! Storage formats: matrix_format 2 CRS, 3 COO, 4 CDS, 5 DIA (offset list,
-! an extension for wide sparse bands).  Dense storage and the dense/band LU
-! preconditioners (which_precon 2, 3) are not offered: they stop with a
-! message.  Device storage is released by user_matrix_free (the reference's
+! an extension for wide sparse bands); 1 dense holds host data only (dense
+! products stop with a message).  The dense/band LU preconditioners
+! (which_precon 2, 3) are not offered: they stop with a message.  Device storage is released by user_matrix_free (the reference's
 ! allocatable components are freed with the object; these handles are not).
 !
 module user_module
@@ -26,7 +26,8 @@ module user_module
 
    private
    public :: user_matrix, operator(*), operator(/), crs_format, coo_format, cds_format, dia_format, &
-             dense_format, real_diagonal, complex_diagonal, REAL_PRECON, COMPLEX_PRECON, user_matrix_free
+             dense_format, real_diagonal, complex_diagonal, REAL_PRECON, COMPLEX_PRECON, user_matrix_free, &
+             user_matrix_sizes
 
    type user_matrix
       integer                        :: matrix_format = 0
@@ -42,6 +43,11 @@ module user_module
       logical                        :: has_shift = .false.
       real(kind=rp)                  :: rshift = 0._rp
       complex(kind=cp)               :: cshift = (0._cp,0._cp)
+! Dense storage (matrix_format 1): host data such as a block of right-hand
+! sides read from a file; dense products are not offered
+      real(kind=rp), allocatable     :: rmat(:,:)
+      complex(kind=cp), allocatable  :: cmat(:,:)
+      integer, allocatable           :: imat(:,:)
 ! Device objects: h is the stored matrix, hp the diagonal preconditioner
 ! operator built from h (which_precon = 1)
       type(c_ptr)                    :: h = c_null_ptr
@@ -62,6 +68,7 @@ contains
       type(user_matrix), intent(in)     :: A
       real(kind=rp), intent(in)         :: v(:)
       real(kind=rp)                     :: w(A%nrows)
+      if ( A%matrix_format == 1 ) stop 'dense products are not offered by the GPU library, use crs_format'
       if ( .not. c_associated(A%h) ) stop 'No matrix format specified.'
       call idrs_check( idrs_user_mv_f64(A%h, v, w), 'user_mv' )
    end function ruser_mv
@@ -70,6 +77,7 @@ contains
       type(user_matrix), intent(in)     :: A
       complex(kind=cp), intent(in)      :: v(:)
       complex(kind=cp)                  :: w(A%nrows)
+      if ( A%matrix_format == 1 ) stop 'dense products are not offered by the GPU library, use crs_format'
       if ( .not. c_associated(A%h) ) stop 'Illegal value for matrix format'
       call idrs_check( idrs_user_mv_c128(A%h, v, w), 'user_mv' )
    end function cuser_mv
@@ -198,13 +206,43 @@ contains
       end if
    end function dia_format
 
+! dense_format (user_module.f90:900-930): the dense matrix as host data
+! (its values, e.g. the right-hand sides of a Matrix Market array file);
+! the products of a dense matrix are not offered by the GPU library and stop
+! with a message
    function dense_format( rmat, imat, cmat ) result(A)
       type(user_matrix)                      :: A
       real(kind=rp), optional                :: rmat(:,:)
       integer, optional                      :: imat(:,:)
       complex(kind=cp), optional             :: cmat(:,:)
-      error stop 'dense_format: dense storage is not offered by the GPU library, use crs_format'
+      A%matrix_format = 1
+      if ( present(rmat) ) then
+         A%nrows = size(rmat,1)
+         A%ncols = size(rmat,2)
+         A%rmat = rmat
+      elseif ( present(cmat) ) then
+         A%nrows = size(cmat,1)
+         A%ncols = size(cmat,2)
+         A%is_complex = .true.
+         A%cmat = cmat
+      elseif ( present(imat) ) then
+         A%nrows = size(imat,1)
+         A%ncols = size(imat,2)
+         A%imat = imat
+      end if
+      A%nnz = A%nrows*A%ncols
    end function dense_format
+
+! nrows, ncols and stored entries of a user matrix from its device object
+   subroutine user_matrix_sizes( A )
+      type(user_matrix), intent(inout)      :: A
+      integer(c_int)                        :: nr, nc, fmt
+      integer(c_int64_t)                    :: nz
+      call idrs_check( idrs_user_matrix_info(A%h, nr, nc, nz, fmt), 'user_matrix_info' )
+      A%nrows = nr
+      A%ncols = nc
+      A%nnz = int(nz)
+   end subroutine user_matrix_sizes
 
    function real_diagonal( A ) result(D)
       type(user_matrix), intent(in)         :: A

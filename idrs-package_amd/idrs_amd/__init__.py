@@ -26,6 +26,8 @@ from .api import (  # noqa: F401
     idrs_batched_mixed,
     idrs_mixed,
     make_p,
+    mm_info,
+    mm_matrix,
     ms_parameters,
     ms_scaleback,
     msidrs,

@@ -103,6 +103,10 @@ _PROTOS = {
     "idrs_dia_format_f64": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, POINTER(c_void_p)]),
     "idrs_dia_format_c128": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, POINTER(c_void_p)]),
     "idrs_user_matrix_destroy": (c_int, [c_void_p]),
+    "idrs_mm_info": (c_int, [ctypes.c_char_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                             POINTER(c_int), POINTER(c_int64)]),
+    "idrs_mm_matrix": (c_int, [ctypes.c_char_p, c_int, POINTER(c_void_p)]),
+    "idrs_mm_dense": (c_int, [ctypes.c_char_p, c_void_p]),
     "idrs_user_matrix_autotune": (c_int, [c_void_p, c_int, POINTER(c_int)]),
     "idrs_user_matrix_set_group": (c_int, [c_void_p, c_int]),
     "idrs_user_matrix_layout": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int)]),

@@ -15,6 +15,7 @@ Index arrays are 1-based like the reference's.  Vectors may be numpy arrays
 """
 from collections import namedtuple
 import ctypes
+import os
 
 import numpy as np
 
@@ -156,6 +157,39 @@ def dia_format(nrows, ncols, offsets, rvals=None, cvals=None):
         return _new_um(lib.idrs_dia_format_c128, int(nrows), int(ncols), len(off), _ptr(off), _ptr(v), is_complex=True)
     v = None if rvals is None else np.asfortranarray(rvals, dtype=np.float64)
     return _new_um(lib.idrs_dia_format_f64, int(nrows), int(ncols), len(off), _ptr(off), _ptr(v))
+
+
+MM_REP = {1: "coordinate", 2: "array"}
+MM_FIELD = {1: "real", 2: "complex", 3: "integer", 4: "pattern"}
+MM_SYMM = {1: "general", 2: "symmetric", 3: "skew-symmetric", 4: "hermitian"}
+
+
+def mm_info(path):
+    """Header of a Matrix Market file: dict(rep, field, symm, nrows, ncols,
+    entries) (mminfo, EXAMPLES/MATRIX_MARKET/mmio.f)."""
+    r, f, sy, m, n = (ctypes.c_int() for _ in range(5))
+    e = ctypes.c_int64()
+    check(_lib.load().idrs_mm_info(os.fsencode(path), ctypes.byref(r), ctypes.byref(f), ctypes.byref(sy),
+                                   ctypes.byref(m), ctypes.byref(n), ctypes.byref(e)))
+    return dict(rep=MM_REP[r.value], field=MM_FIELD[f.value], symm=MM_SYMM[sy.value], nrows=m.value,
+                ncols=n.value, entries=e.value)
+
+
+def mm_matrix(path, which_format="coo"):
+    """A = mm_matrix(mm_file, which_format) (EXAMPLES/MATRIX_MARKET/
+    mm_module.f90:48-150): a coordinate file as a user matrix on the device
+    ('coo' or 'crs'); an array file as a numpy array (the reference's dense
+    user matrix, here host data such as a block of right-hand sides)."""
+    info = mm_info(path)
+    if info["rep"] == "array":
+        cplx = info["field"] == "complex"
+        a = np.empty((info["nrows"], info["ncols"]), dtype=np.complex128 if cplx else np.float64, order="F")
+        check(_lib.load().idrs_mm_dense(os.fsencode(path), _ptr(a)))
+        return a
+    fmt = {"coo": 3, "crs": 2}[which_format]
+    h = ctypes.c_void_p()
+    check(_lib.load().idrs_mm_matrix(os.fsencode(path), fmt, ctypes.byref(h)))
+    return UserMatrix(h.value, info["field"] == "complex")
 
 
 def real_diagonal(K):

@@ -8,7 +8,8 @@ The operators restate the reference's examples:
   - 3-D Helmholtz: h^2-scaled 7-point -Laplacian (laplace_matrix,
     EXAMPLES/STENCIL/user_module.f90:569-620) with shifts;
   - 1-D convection-diffusion Toeplitz matrix (EXAMPLES/TOEPLITZ/toeplitz.f90:111-128);
-  - random diagonally dominant sparse matrix.
+  - random diagonally dominant sparse matrix;
+  - the batch of uneven 2-D systems of the batched workload.
 CRS arrays are 1-based (the reference's crs_format convention) with columns
 ascending within each row.
 """
@@ -174,6 +175,23 @@ def random_dominant(n, k=16, seed=0, factor=1.2):
     allv = np.take_along_axis(allv, order, axis=1)
     row_ptr = (1 + np.arange(n + 1, dtype=np.int64) * (k + 1)).astype(np.int32)
     return Crs(n, row_ptr, (allc.ravel() + 1).astype(np.int32), np.ascontiguousarray(allv.ravel()))
+
+
+def batched_systems(nsys, seed=0):
+    """The batched workload: nsys 2-D 5-point convection-diffusion systems,
+    side floor(32 * 16^u) with u ~ U(0,1) (log-uniform in [32, 512]),
+    Pe ~ U[1,100], convection angle ~ U[0, 2 pi), b ~ U[-1,1] (all seeded).
+    Returns a list of (Crs, b)."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0.0, 1.0, nsys)
+    pe = rng.uniform(1.0, 100.0, nsys)
+    ang = rng.uniform(0.0, 2 * np.pi, nsys)
+    out = []
+    for k in range(nsys):
+        m = int(np.floor(32 * 16 ** u[k]))
+        crs, _ = convection_diffusion_2d(m, pe[k], angle=ang[k])
+        out.append((crs, rng.uniform(-1.0, 1.0, crs.n)))
+    return out
 
 
 def gaussian_shadow(n, s, seed=0, complex=False):

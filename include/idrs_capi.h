@@ -74,6 +74,24 @@ int idrs_dia_format_f64(int nrows, int ncols, int ndiag, const int *offsets, con
 /* idrs_dia_format_f64 with complex values */
 int idrs_dia_format_c128(int nrows, int ncols, int ndiag, const int *offsets, const double *vals,
                          idrs_user_matrix_t *A);
+/* Matrix Market files (EXAMPLES/MATRIX_MARKET/mm_module.f90 over the NIST
+   reader mmio.f): header of a file -- rep 1 coordinate / 2 array; field
+   1 real / 2 complex / 3 integer / 4 pattern; symm 1 general / 2 symmetric
+   / 3 skew-symmetric / 4 hermitian; entries: stored entries (coordinate)
+   or rows*cols (array).  Replaces mminfo as mm_matrix calls it
+   (mm_module.f90:85). */
+int idrs_mm_info(const char *path, int *rep, int *field, int *symm, int *nrows, int *ncols, int64_t *entries);
+/* A = mm_matrix(path, which_format) for a coordinate file,
+   mm_module.f90:48-150 with mm_coo (:542-738) / mm_crs (:312-540): format
+   3 'coo' or 2 'crs' (the same device CRS either way: the symmetric,
+   skew-symmetric and hermitian halves are mirrored right after each
+   off-diagonal entry, in file order); complex values pass through single
+   precision as the reference's reader stores them (mm_module.f90:61). */
+int idrs_mm_matrix(const char *path, int format, idrs_user_matrix_t *A);
+/* the values of an array file (a right-hand-side block), column-major
+   nrows x ncols into host vals ((re, im) pairs for complex files), the
+   symmetric kinds expanded: mm_dense, mm_module.f90:152-310 */
+int idrs_mm_dense(const char *path, double *vals);
 /* Frees the device storage of a user matrix (the reference's allocatable
    components are released when the Fortran object goes out of scope). */
 int idrs_user_matrix_destroy(idrs_user_matrix_t A);

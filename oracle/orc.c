@@ -23,6 +23,35 @@ double orc_uniform01(uint64_t seed, uint64_t index)
     return (double)(z >> 11) * 0x1.0p-53;
 }
 
+/* Row and vector loops run in parallel in the OpenMP build (liborc_omp.so,
+   the timed CPU baseline); the checker build (liborc.so) keeps every loop
+   sequential.  Parallel sums (norms, dots) then add partial sums per thread,
+   a different rounding order than the sequential loops. */
+#ifdef _OPENMP
+#define ORC_FOR _Pragma("omp parallel for schedule(static)")
+#define ORC_FOR_SUM _Pragma("omp parallel for schedule(static) reduction(+:acc)")
+#else
+#define ORC_FOR
+#define ORC_FOR_SUM
+#endif
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the parallel loops: sets them when n > 0, returns the count in
+   use (1 in the sequential checker build). */
+int orc_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 /* ---- real instantiation ---- */
 #define T double
 #define FN(name) orc_r_##name

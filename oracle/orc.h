@@ -1,8 +1,10 @@
 /*
- * Oracle: a plain-C, single-threaded restatement of the reference IDRS
+ * Oracle: a plain-C restatement of the reference IDRS
  * package's hot path (IDR(s) solvers, operator, polynomial preconditioner,
  * sparse formats).  TEST INFRASTRUCTURE ONLY: it is the checker that the
- * HIP library is compared against, and the CPU baseline timed by bench.py.
+ * HIP library is compared against (liborc.so, single-threaded) and, built
+ * with OpenMP row and vector loops (liborc_omp.so), the CPU baseline timed
+ * by bench.py.
  * Nothing in the product links, loads or calls it.
  *
  * Every routine names the reference file:line it restates (paths relative to
@@ -24,6 +26,9 @@
 #include <stdint.h>
 
 typedef double _Complex orc_cplx;
+
+/* threads of the parallel loops (OpenMP build); returns the count in use */
+int orc_threads(int n);
 
 /* user_matrix formats, PP_MATRIX/user_module.f90:134-167 */
 enum { ORC_CRS = 2, ORC_COO = 3, ORC_CDS = 4, ORC_DIA = 5 };

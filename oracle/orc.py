@@ -13,6 +13,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
+# the same restatement with OpenMP loops: timed as the CPU baseline, never
+# used as the checker (its parallel sums round in another order)
+LIB_OMP = os.path.join(HERE, "liborc_omp.so")
 
 CRS, COO, CDS, DIA = 2, 3, 4, 5
 
@@ -62,10 +65,22 @@ class Stats(ctypes.Structure):
 
 
 _lib = None
+_lib_omp = None
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE, "liborc.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "liborc.so", "liborc_omp.so"], check=True)
+
+
+def load_timed(threads=0):
+    """liborc_omp.so with `threads` OpenMP threads (0: OMP_NUM_THREADS or the
+    runtime's default); returns (library, threads in use)."""
+    global _lib_omp
+    if _lib_omp is None:
+        if not os.path.exists(LIB_OMP):
+            build()
+        _lib_omp = _setup(ctypes.CDLL(LIB_OMP))
+    return _lib_omp, _lib_omp.orc_threads(int(threads))
 
 
 def load():
@@ -73,19 +88,25 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        _lib = ctypes.CDLL(LIB)
-        _lib.orc_uniform01.restype = c_double
-        _lib.orc_uniform01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
-        _lib.orc_r_norm.restype = c_double
-        _lib.orc_c_norm.restype = c_double
-        for p in ("r", "c"):
-            getattr(_lib, f"orc_{p}_idrs").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, POINTER(Opts), c_void_p, POINTER(Stats)]
-            getattr(_lib, f"orc_{p}_qmridr").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, POINTER(Opts), c_void_p, POINTER(Stats)]
-            for name in ("msidrs", "msqmridr"):
-                getattr(_lib, f"orc_{p}_{name}").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, c_void_p, c_int,
-                                                             POINTER(Opts), c_void_p, POINTER(Stats)]
-            getattr(_lib, f"orc_{p}_compute_alpha").restype = c_int
+        _lib = _setup(ctypes.CDLL(LIB))
     return _lib
+
+
+def _setup(lib):
+    lib.orc_uniform01.restype = c_double
+    lib.orc_uniform01.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    lib.orc_r_norm.restype = c_double
+    lib.orc_c_norm.restype = c_double
+    for p in ("r", "c"):
+        getattr(lib, f"orc_{p}_idrs").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, POINTER(Opts), c_void_p, POINTER(Stats)]
+        getattr(lib, f"orc_{p}_qmridr").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, POINTER(Opts), c_void_p, POINTER(Stats)]
+        for name in ("msidrs", "msqmridr"):
+            getattr(lib, f"orc_{p}_{name}").argtypes = [POINTER(Matrix), c_int, c_void_p, c_int, c_void_p, c_int,
+                                                         POINTER(Opts), c_void_p, POINTER(Stats)]
+        getattr(lib, f"orc_{p}_compute_alpha").restype = c_int
+    lib.orc_threads.restype = c_int
+    lib.orc_threads.argtypes = [c_int]
+    return lib
 
 
 def _p(a):
@@ -223,10 +244,11 @@ def make_p(n, s, method, PT_raw=None, r=None, complex=False):
 
 
 def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, U0=None, omega=None,
-         resvec=False, PT_raw=None):
+         resvec=False, PT_raw=None, lib=None):
     """Oracle IDRS with the reference's optional arguments; returns
-    (x, flag, relres, iterations, resvec)."""
-    lib = load()
+    (x, flag, relres, iterations, resvec).  lib: load_timed()'s library to
+    run the OpenMP build instead of the checker."""
+    lib = lib or load()
     cplx = A.complex
     dt = np.complex128 if cplx else np.float64
     n = len(b)

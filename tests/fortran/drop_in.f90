@@ -7,18 +7,56 @@ program drop_in
 
    use precision_module
    use user_module
+   use mm_module
    use matrix_module
    use idrs_module
 
    implicit none
+
+   character(len=512) :: data_dir
 
    call toeplitz_real()
    call cdr_preconditioned()
    call toeplitz_complex()
    call krylov_family()
    call ms_pp_sequence()
+! Matrix Market files: the directory is the first command-line argument
+   if ( command_argument_count() >= 1 ) then
+      call get_command_argument( 1, data_dir )
+      call stommel_mm( trim(data_dir) )
+   end if
 
 contains
+
+! The stommel example's chain (EXAMPLES/MATRIX_MARKET/stommel.f90): the
+! system matrix and the twelve monthly right-hand sides read by mm_matrix,
+! diagonal scaling, here IDR(4) with the default Chebyshev(8) polynomial of
+! real_preconditioner, for the first right-hand side; err is the true
+! relative residual of the scaled-back solution.
+   subroutine stommel_mm( dir )
+      character(len=*), intent(in)  :: dir
+      type(user_matrix)             :: K, RHS
+      type(matrix)                  :: A, M1
+      real(kind=rp), allocatable    :: x(:), b(:), y(:)
+      real(kind=rp)                 :: relres
+      integer                       :: flag, iter
+
+      K = mm_matrix( dir//'/stommel6.mtx', 'coo' )
+      RHS = mm_matrix( dir//'/stommel6_b.mtx' )
+      allocate( b(K%nrows), x(K%nrows), y(K%nrows) )
+      b = RHS%rmat(:,1)
+      A%K = K
+      A%P = real_precon( 1, K )
+      call real_preconditioner( A, 2, 8 )
+      y = idrs( A, b, 4, M1, 1.e-8_rp, 4000, 1, flag, relres, iter )
+      x = scaleback( A, b, y )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'stommel6_mm', ' iter', iter, ' flag', flag, &
+         ' relres', relres, ' err', norm( b - K*x )/norm( b )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'stommel6_mm_sizes', ' iter', K%nnz, &
+         ' flag', size(RHS%rmat,2), ' relres', real(K%nrows, rp), ' err', 0._rp
+      call user_matrix_free( A%P )
+      call user_matrix_free( K )
+   end subroutine stommel_mm
 
 ! 1-D convection-diffusion, N=60 (EXAMPLES/TOEPLITZ/toeplitz.f90:111-165),
 ! stored in CRS; positional call as in the example.
@@ -118,11 +156,12 @@ contains
 
 ! Complex Toeplitz accuracy test (toeplitz.f90:41-70) in offset storage.
    subroutine toeplitz_complex()
-      integer, parameter            :: N = 200
+      integer, parameter            :: N = 200, maxit = 4000
       type(matrix)                  :: A, M
       complex(kind=cp)              :: x(N), b(N), vals(N,4), cv(4)
-      integer                       :: offs(4), flag, iter, i, d
-      real(kind=rp)                 :: relres
+      integer                       :: offs(4), flag, iter, i, d, s, variant
+      real(kind=rp)                 :: relres, tol
+      character(len=32)             :: name
 
       offs = [-1, 0, 2, 3]
       cv = [ (0._cp,3.6_cp), (4._cp,0._cp), (1._cp,0._cp), (0.7_cp,0._cp) ]
@@ -137,9 +176,27 @@ contains
       end do
       A%K = dia_format( N, N, offs, cvals=vals )
       b = (0._cp,1._cp)
-      x = idrs( A, b, 4, M, 1.e-12_rp, 4000, 1, flag, relres, iter )
-      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'toeplitz_complex_s4', ' iter', iter, ' flag', flag, &
-         ' relres', relres, ' err', norm( b - A%K*x )/norm( b )
+      ! the accuracy test's loops of toeplitz.f90:55-89: IDRS and QMRIDR
+      ! with s = 1, 2, 4, ..., 64, BiCGSTAB between them
+      tol = 1e-12_rp
+      variant = 1
+      do i = 1, 7
+         s = 2**(i-1)
+         x = idrs( A, b, s, M, tol, maxit, variant, flag, relres, iter )
+         write(name,'(a,i0)') 'toeplitz_complex_idrs_s', s
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', relres, ' err', norm( b - A%K*x )/norm( b )
+      end do
+      x = idrs( A, b, 1, M, tol, maxit, 2, flag, relres, iter )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'toeplitz_complex_bicgstab', ' iter', iter, &
+         ' flag', flag, ' relres', relres, ' err', norm( b - A%K*x )/norm( b )
+      do i = 1, 7
+         s = 2**(i-1)
+         x = qmridr( A, b, s, M, tol, maxit, flag, relres, iter )
+         write(name,'(a,i0)') 'toeplitz_complex_qmridr_s', s
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', relres, ' err', norm( b - A%K*x )/norm( b )
+      end do
    end subroutine toeplitz_complex
 
 ! QMRIDR, MSIDRS and MSQMRIDR on the real Toeplitz system of toeplitz_real

