@@ -9,6 +9,7 @@
 
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -141,6 +142,84 @@ static int idrs_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs
             stats->solve_ms = ss.solve_ms;
             stats->spmv_ms = ss.spmv_ms;
             stats->spmv_launches = ss.spmv_launches;
+            stats->reserved = 0;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// allocation-free IDRS: every workspace buffer of the solve is carved from
+// `scratch`, a region of NOALLOC_SCRATCH_BYTES in the code object's device
+// data, so the call makes no hipMalloc/hipFree, no pinned host allocation,
+// no event and no graph.  Each GPU holds its own copy of the array (until
+// the process ends); its address on the current device is looked up once
+// per device with hipGetSymbolAddress and kept, and only that address ever
+// reaches a kernel or a copy.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device (looked up on the
+// first call on that device)
+static void *scratch_on_current_device() {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    need(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+template <class T>
+static int idrs_noalloc_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                              idrs_stats_t *stats) {
+    return guarded([&] {
+        need(A && b && x, "null argument");
+        idrs_opts_t o{};
+        if (opts) o = *opts;
+        need(!o.time_spmv, "SpMV timing is not offered by the allocation-free solve");
+        need(op_prepared(A->op),
+             "the operator's sliced copy is not built yet: run one ordinary product or solve on it first");
+        hipStream_t st = lib_stream();
+        Arena ar;
+        ar.base = (char *)scratch_on_current_device();
+        ar.cap = NOALLOC_SCRATCH_BYTES;
+        SolveOpts<T> so;
+        so.arena = &ar;
+        so.has_m1 = o.has_M1 != 0;
+        so.M1 = (o.has_M1 && o.M1) ? &o.M1->op : nullptr;
+        if (o.has_tolerance) so.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            so.has_maxit = true;
+            so.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) so.variant = o.variant;
+        so.omega = (const T *)o.omega;
+        so.n_omega = o.omega ? o.n_omega : 0;
+        so.resvec = o.resvec;
+        so.resvec_len = o.resvec_len;
+        so.p_seed = o.p_seed ? o.p_seed : 1;
+        so.graph_mode = 2;
+        so.x0 = (const T *)o.x0;
+        so.U0 = (const T *)o.U0;
+        so.PT_raw = (const T *)o.PT_raw;
+        SolveStats ss;
+        idrs_solve<T>(A->op, n, s, (const T *)b, (T *)x, so, ss, st);
+        if (stats) {
+            stats->flag = ss.flag;
+            stats->iterations = ss.iterations;
+            stats->relres = ss.relres;
+            stats->solve_ms = ss.solve_ms;
+            stats->spmv_ms = 0.0;
+            stats->spmv_launches = 0;
             stats->reserved = 0;
         }
     });
@@ -828,6 +907,10 @@ int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opt
 int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                    idrs_stats_t *stats) {
     return idrs_entry<c128>(A, n, b, s, opts, x, stats);
+}
+int idrs_idrs_f64_noalloc(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                          idrs_stats_t *stats) {
+    return idrs_noalloc_entry<double>(A, n, b, s, opts, x, stats);
 }
 
 int idrs_qmridr_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_krylov_opts_t *opts, double *x,

@@ -140,6 +140,8 @@ _PROTOS = {
     "idrs_scaleback_c128": (c_int, [c_void_p, c_void_p, c_void_p]),
     "idrs_idrs_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, POINTER(idrs_stats_t)]),
     "idrs_idrs_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, POINTER(idrs_stats_t)]),
+    "idrs_idrs_f64_noalloc": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p,
+                                      POINTER(idrs_stats_t)]),
     "idrs_qmridr_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t), c_void_p,
                                 POINTER(idrs_stats_t)]),
     "idrs_qmridr_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t), c_void_p,

@@ -261,6 +261,24 @@ int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opt
 /* CIDRS, idrs_module.f90:107-178 */
 int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                    idrs_stats_t *stats);
+/* idrs_idrs_f64 that allocates nothing.  No counterpart in the reference:
+   for callers that cannot allocate device or pinned memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   b, x and opts' x0, U0, PT_raw are device pointers (device_pointers is
+   implied); every workspace buffer comes from a fixed 64 MiB region in the
+   library's device data (one per GPU, held until the process ends), so a
+   system whose workspace is larger -- about (3s + 7) n doubles plus the
+   reduction partials, e.g. n up to ~450 000 rows at s = 4 -- is refused
+   with IDRS_ERR_ARG before anything runs.  The solve runs eagerly (no
+   graph), reads the convergence flag after every cycle with a blocking
+   copy (no events, no pinned memory) and reports host-clock solve_ms; SpMV
+   timing (time_spmv) is rejected.  An operator whose K uses the sliced copy
+   must have had one ordinary product or solve first (that builds its
+   sliced values).  Results are bit-identical to idrs_idrs_f64's.  All
+   calls on one GPU share the region: calls that overlap on one GPU (other
+   host threads, other streams) race. */
+int idrs_idrs_f64_noalloc(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                          idrs_stats_t *stats);
 
 /* Optional arguments of QMRIDR, MSIDRS and MSQMRIDR
    (MODULES/IDRS/idrs_module.f90:253-713).  has_* flags are present(). */
