@@ -6,6 +6,7 @@
 #include "batch.hpp"
 #include "mixed.hpp"
 #include "mm.hpp"
+#include "ritz.hpp"
 
 #include <cstdarg>
 #include <cstring>
@@ -107,6 +108,8 @@ static int idrs_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs
         so.p_seed = o.p_seed ? o.p_seed : 1;
         need(o.graph_mode >= 0 && o.graph_mode <= 2, "graph_mode must be 0, 1 or 2");
         so.graph_mode = o.graph_mode;
+        so.H = (T *)o.H;
+        so.nritz = o.H ? o.H_nritz : 0;
         const int prow = (so.variant == 1) ? s : 1;
         SolveStats ss;
         SpmvTimer &tm = spmv_timer();
@@ -142,6 +145,87 @@ static int idrs_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs
             stats->solve_ms = ss.solve_ms;
             stats->spmv_ms = ss.spmv_ms;
             stats->spmv_launches = ss.spmv_launches;
+            stats->reserved = 0;
+        }
+    });
+}
+
+// IDRS_RITZ: the solve with its Hessenberg matrix, then omega, U0 and H
+// (ritz_module.f90:225-556).  opts->has_M1/M1 is the M argument; the
+// shift-and-invert start block continues the shadow space's stream.
+template <class T>
+static int ritz_entry(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                      idrs_stats_t *stats, double *omega, int n_omega, double *U0, int dim_u0, double *H, int h_rows,
+                      int h_cols, int *nritz_out, int *converged) {
+    return guarded([&] {
+        need(A && b && x, "null argument");
+        need(s >= 1 && s <= SMAX, "s must be in 1..64");
+        need(n_omega >= 0 && dim_u0 >= 0 && h_rows >= 0 && h_cols >= 0, "negative output size");
+        need((n_omega == 0 || omega) && (dim_u0 == 0 || U0) && ((h_rows == 0 && h_cols == 0) || H),
+             "output array missing");
+        hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
+        idrs_opts_t o{};
+        if (opts) o = *opts;
+        need(!o.U0 && !o.omega && !o.H, "IDRS_RITZ returns U0, omega and H; it takes none of them");
+        need(!o.time_spmv, "SpMV timing is not offered by IDRS_RITZ");
+        SolveOpts<T> so;
+        if (o.has_tolerance) so.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            so.has_maxit = true;
+            so.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) so.variant = o.variant;
+        so.resvec = o.resvec;
+        so.resvec_len = o.resvec_len;
+        so.p_seed = o.p_seed ? o.p_seed : 1;
+        need(o.graph_mode >= 0 && o.graph_mode <= 2, "graph_mode must be 0, 1 or 2");
+        so.graph_mode = o.graph_mode;
+        const int prow = (so.variant == 1) ? s : 1;
+        RitzRequest<T> rq;
+        if (o.has_M1) {
+            rq.has_M = true;
+            rq.M = o.M1 ? &o.M1->op : nullptr;
+        }
+        rq.omega = (T *)omega;
+        rq.n_omega = n_omega;
+        rq.dimU0 = dim_u0;
+        rq.H = (T *)H;
+        rq.h_rows = h_rows;
+        rq.h_cols = h_cols;
+        rq.seed = so.p_seed;
+        rq.offset = (unsigned long long)prow * (unsigned long long)n;
+        SolveStats ss;
+        int nritz = 0;
+        if (o.device_pointers) {
+            so.x0 = (const T *)o.x0;
+            so.PT_raw = (const T *)o.PT_raw;
+            rq.U0 = (T *)U0;
+            nritz = idrs_ritz_solve<T>(A->op, n, s, (const T *)b, (T *)x, so, rq, ss, st);
+        } else {
+            DVec<T> db((const T *)b, n, st), dx(n), dx0, dP, dU((size_t)n * dim_u0);
+            if (o.x0) {
+                dx0 = DVec<T>((const T *)o.x0, n, st);
+                so.x0 = dx0.p;
+            }
+            if (o.PT_raw) {
+                dP = DVec<T>((const T *)o.PT_raw, (size_t)n * prow, st);
+                so.PT_raw = dP.p;
+            }
+            rq.U0 = dim_u0 ? dU.p : nullptr;
+            nritz = idrs_ritz_solve<T>(A->op, n, s, db.p, dx.p, so, rq, ss, st);
+            dx.download((T *)x, st);
+            if (dim_u0) dU.download((T *)U0, st);
+        }
+        if (nritz_out) *nritz_out = nritz;
+        if (converged) *converged = rq.converged;
+        if (stats) {
+            stats->flag = ss.flag;
+            stats->iterations = ss.iterations;
+            stats->relres = ss.relres;
+            stats->solve_ms = ss.solve_ms;
+            stats->spmv_ms = 0.0;
+            stats->spmv_launches = 0;
             stats->reserved = 0;
         }
     });
@@ -186,6 +270,7 @@ static int idrs_noalloc_entry(idrs_matrix_t A, int n, const double *b, int s, co
         idrs_opts_t o{};
         if (opts) o = *opts;
         need(!o.time_spmv, "SpMV timing is not offered by the allocation-free solve");
+        need(!o.H, "the Hessenberg output is not offered by the allocation-free solve");
         need(op_prepared(A->op),
              "the operator's sliced copy is not built yet: run one ordinary product or solve on it first");
         hipStream_t st = lib_stream();
@@ -313,7 +398,7 @@ static int batched_entry(idrs_matrix_t A, const double *b, int s, const idrs_opt
         op_prepare(A->op, st);
         idrs_opts_t o{};
         if (opts) o = *opts;
-        need(!o.has_M1 && !o.U0 && !o.resvec, "M1, U0 and resvec are not offered for batches");
+        need(!o.has_M1 && !o.U0 && !o.resvec && !o.H, "M1, U0, resvec and H are not offered for batches");
         SolveOpts<T> so;
         if (o.has_tolerance) so.tol = o.tolerance;
         if (o.has_maximum_iterations) {
@@ -371,8 +456,8 @@ static int batched_mixed_entry(idrs_matrix_t A, const double *b, int s, const id
         op_prepare(A->op, st);
         idrs_opts_t o{};
         if (opts) o = *opts;
-        need(!o.has_M1 && !o.U0 && !o.resvec && !o.omega && !o.x0 && !o.PT_raw,
-             "M1, U0, omega, resvec, x0 and PT_raw are not offered here");
+        need(!o.has_M1 && !o.U0 && !o.resvec && !o.omega && !o.x0 && !o.PT_raw && !o.H,
+             "M1, U0, omega, resvec, x0, PT_raw and H are not offered here");
         need(o.graph_mode >= 0 && o.graph_mode <= 2, "graph_mode must be 0, 1 or 2");
         SolveOpts<double> so;
         if (o.has_tolerance) so.tol = o.tolerance;
@@ -417,7 +502,7 @@ static int mixed_entry(idrs_matrix_t A, int n, const double *b, int s, const idr
         op_prepare(A->op, st);
         idrs_opts_t o{};
         if (opts) o = *opts;
-        need(!o.has_M1 && !o.U0 && !o.resvec && !o.omega, "M1, U0, omega and resvec are not offered here");
+        need(!o.has_M1 && !o.U0 && !o.resvec && !o.omega && !o.H, "M1, U0, omega, resvec and H are not offered here");
         SolveOpts<T> so;
         if (o.has_tolerance) so.tol = o.tolerance;
         if (o.has_maximum_iterations) {
@@ -908,6 +993,103 @@ int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_op
                    idrs_stats_t *stats) {
     return idrs_entry<c128>(A, n, b, s, opts, x, stats);
 }
+int idrs_idrs_ritz_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                       idrs_stats_t *stats, double *omega, int n_omega, double *U0, int dim_u0, double *H, int h_rows,
+                       int h_cols, int *nritz, int *converged) {
+    return ritz_entry<double>(A, n, b, s, opts, x, stats, omega, n_omega, U0, dim_u0, H, h_rows, h_cols, nritz,
+                              converged);
+}
+int idrs_idrs_ritz_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                        idrs_stats_t *stats, double *omega, int n_omega, double *U0, int dim_u0, double *H,
+                        int h_rows, int h_cols, int *nritz, int *converged) {
+    return ritz_entry<c128>(A, n, b, s, opts, x, stats, omega, n_omega, U0, dim_u0, H, h_rows, h_cols, nritz,
+                            converged);
+}
+
+// spectral routines on small dense matrices (host); complex values as
+// (re, im) pairs
+static std::vector<std::complex<double>> pairs_in(const double *v, size_t cnt) {
+    std::vector<std::complex<double>> out(cnt);
+    for (size_t q = 0; q < cnt; q++) out[q] = {v[2 * q], v[2 * q + 1]};
+    return out;
+}
+static void pairs_out(const std::vector<std::complex<double>> &v, double *out) {
+    for (size_t q = 0; q < v.size(); q++) {
+        out[2 * q] = v[q].real();
+        out[2 * q + 1] = v[q].imag();
+    }
+}
+static std::vector<double> reals_in(const double *v, size_t cnt) {
+    std::vector<double> out(cnt);
+    for (size_t q = 0; q < cnt; q++) out[q] = v[2 * q];
+    return out;
+}
+
+int idrs_qr_eig(int n, const double *H, int ldh, int is_complex, double *eig, double *U, int *converged) {
+    return guarded([&] {
+        need(n >= 1 && ldh >= n && H && eig, "bad arguments");
+        std::vector<std::complex<double>> e, Uv;
+        if (is_complex) {
+            const auto h = pairs_in(H, (size_t)ldh * n);
+            e = ritz::qr_eig(n, h.data(), ldh, U ? &Uv : nullptr, converged);
+        } else {
+            const auto h = reals_in(H, (size_t)ldh * n);
+            e = ritz::qr_eig(n, h.data(), ldh, U ? &Uv : nullptr, converged);
+        }
+        pairs_out(e, eig);
+        if (U) pairs_out(Uv, U);
+    });
+}
+
+int idrs_filter_ritzval(int n, const double *ritzval, int *outlier) {
+    return guarded([&] {
+        need(n >= 1 && ritzval && outlier, "bad arguments");
+        const auto o = ritz::filter_ritzval(pairs_in(ritzval, n));
+        for (int i = 0; i < n; i++) outlier[i] = o[i] ? 1 : 0;
+    });
+}
+
+int idrs_analyse_spectrum(int n, const double *H, int ldh, int is_complex, double *foci, double *center,
+                          double *shift) {
+    return guarded([&] {
+        need(n >= 1 && ldh >= n && H, "bad arguments");
+        if (is_complex) {
+            const auto h = pairs_in(H, (size_t)ldh * n);
+            std::complex<double> f[2], c, sh;
+            ritz::analyse_spectrum(n, h.data(), ldh, f, &c, &sh);
+            if (foci) pairs_out({f[0], f[1]}, foci);
+            if (center) pairs_out({c}, center);
+            if (shift) pairs_out({sh}, shift);
+        } else {
+            const auto h = reals_in(H, (size_t)ldh * n);
+            double f[2], c, sh;
+            ritz::analyse_spectrum(n, h.data(), ldh, f, &c, &sh);
+            if (foci) pairs_out({f[0], f[1]}, foci);
+            if (center) pairs_out({c}, center);
+            if (shift) pairs_out({sh}, shift);
+        }
+    });
+}
+
+int idrs_si_power(int n, const double *H, int ldh, int is_complex, int s, const double *shift, uint64_t seed,
+                  uint64_t offset, double *schurvec) {
+    return guarded([&] {
+        need(n >= 1 && ldh >= n && s >= 1 && s <= n && H && schurvec, "bad arguments");
+        if (is_complex) {
+            const auto h = pairs_in(H, (size_t)ldh * n);
+            const std::complex<double> sh = shift ? std::complex<double>(shift[0], shift[1]) : 0.0;
+            pairs_out(ritz::si_power(n, h.data(), ldh, s, sh, seed, offset), schurvec);
+        } else {
+            const auto h = reals_in(H, (size_t)ldh * n);
+            const auto v = ritz::si_power(n, h.data(), ldh, s, shift ? shift[0] : 0.0, seed, offset);
+            for (size_t q = 0; q < v.size(); q++) {
+                schurvec[2 * q] = v[q];
+                schurvec[2 * q + 1] = 0.0;
+            }
+        }
+    });
+}
+
 int idrs_idrs_f64_noalloc(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                           idrs_stats_t *stats) {
     return idrs_noalloc_entry<double>(A, n, b, s, opts, x, stats);

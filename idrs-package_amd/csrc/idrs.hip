@@ -99,9 +99,12 @@ struct Buf {
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void k_init(State<T> *st, int s, int method, int prow, int conjP, int maxit, double kappa, double tol,
-                       int user_omega, int n_omega, int inispace, int rv_cap) {
-    if (threadIdx.x == 0 && blockIdx.x == 0)
+                       int user_omega, int n_omega, int inispace, int rv_cap, T *H, int nritz) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
         scalar_init(st, s, method, prow, conjP, maxit, kappa, tol, user_omega, n_omega, inispace, rv_cap);
+        st->H = H;
+        st->nritz = nritz;
+    }
 }
 
 // partials[block] = sum |v|^2 over the block's rows
@@ -619,8 +622,11 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     if (o.resvec && maxit + 1 > o.resvec_len)
         throw std::invalid_argument("Length of vector with residual norms too small, should be maxit+1");
     const int user_omega = (o.omega && o.n_omega >= 1) ? 1 : 0;
+    // H(nritz+1, nritz) wanted only for nritz >= 1 (idrs.inc:82-84)
+    const bool out_h = o.H != nullptr && o.nritz > 0;
 
     Arena *ar = o.arena;
+    if (ar && out_h) throw std::invalid_argument("the Hessenberg output is not offered by the allocation-free solve");
     SpmvTimer &tm = spmv_timer();
     if (ar && tm.enabled) throw std::invalid_argument("SpMV timing needs events: not in the allocation-free solve");
     const int gv = vec_grid(n);
@@ -647,7 +653,7 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     Buf bG(vb * s, ar), bU(vb * s, ar), br(vb, ar), bt(vb, ar), bw1(vb, ar), bw2(vb, ar), btmp(vb, ar), bP(vb * prow, ar);
     Buf bpart(sizeof(T) * part_cnt, ar), bst(sizeof(State<T>), ar), bres(sizeof(double) * (cap_it + 2), ar);
     Buf bpart2(sizeof(T) * MAX_NV * PRE, ar);
-    Buf bv, bm1, bom;
+    Buf bv, bm1, bom, bh;
     T *G = bG.as<T>(), *U = bU.as<T>(), *r = br.as<T>(), *t = bt.as<T>(), *w1 = bw1.as<T>(), *w2 = bw2.as<T>();
     T *tmp = btmp.as<T>(), *P = bP.as<T>(), *part = bpart.as<T>();
     State<T> *st = bst.as<State<T>>();
@@ -679,8 +685,15 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     }
     tm.reset();
 
+    const size_t h_bytes = out_h ? sizeof(T) * (size_t)(o.nritz + 1) * o.nritz : 0;
+    T *hd = nullptr;
+    if (out_h) {
+        bh = Buf(h_bytes, ar);
+        hd = bh.as<T>();
+        HIP_TRY(hipMemsetAsync(hd, 0, h_bytes, stream));   // H = 0 (idrs.inc:89)
+    }
     hipLaunchKernelGGL((k_init<T>), dim3(1), dim3(64), 0, stream, st, s, method, prow, conjP, maxit, kappa, o.tol,
-                       user_omega, user_omega ? o.n_omega : 0, o.U0 ? 1 : 0, (int)(cap_it + 2));
+                       user_omega, user_omega ? o.n_omega : 0, o.U0 ? 1 : 0, (int)(cap_it + 2), hd, o.nritz);
     LAUNCH_CHECK();
     // x = 0 or x0; U = 0 or U0 (idrs.inc:52-65)
     if (o.x0) HIP_TRY(hipMemcpyAsync(x, o.x0, vb, hipMemcpyDeviceToDevice, stream));
@@ -838,6 +851,7 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
             long cnt = std::min<long>(std::min<long>(o.resvec_len, hs.iter + 1), cap_it + 2);
             HIP_TRY(hipMemcpy(o.resvec, resvec, sizeof(double) * cnt, hipMemcpyDeviceToHost));
         }
+        if (out_h) HIP_TRY(hipMemcpy(o.H, hd, h_bytes, hipMemcpyDeviceToHost));
         out.flag = hs.info < 0 ? 1 : hs.info;
         out.iterations = hs.iter;
         out.relres = hs.normr / hs.normb;

@@ -58,6 +58,10 @@ struct SolveOpts {
     const T *PT_raw = nullptr;   // device, raw shadow space before orthonormalisation
     unsigned long long p_seed = 1;   // default stream seed (reference: this_image() = 1)
     int graph_mode = 0;              // 0: HIP graph per cycle when n <= 2^21, 1: always, 2: never
+    // Hessenberg matrix H(nritz+1, nritz) of the inner steps (idrs.inc:81-90),
+    // HOST output, column-major; null: not wanted
+    T *H = nullptr;
+    int nritz = 0;
     // set: every device buffer of the solve comes from this region, and the
     // solve makes no allocation of any kind (no graph, no events, no pinned
     // host memory: the host polls the flag with a blocking copy per cycle)
@@ -67,6 +71,28 @@ struct SolveOpts {
 template <class T>
 int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOpts<T> &o, SolveStats &st,
                hipStream_t stream);
+
+// The outputs of IDRS_RITZ (ritz_module.f90:225-556) beside the solution.
+template <class T>
+struct RitzRequest {
+    bool has_M = false;            // present(M): IDRS's preconditioner and v/M in compute_U0
+    const Operator *M = nullptr;   // its operator; null: v/M = v (a default type(matrix))
+    T *omega = nullptr;            // HOST output omega(n_omega)
+    int n_omega = 0;
+    T *U0 = nullptr;               // DEVICE output U0(n, dimU0), orthonormal columns
+    int dimU0 = 0;
+    T *H = nullptr;                // HOST output H(h_rows, h_cols) = Hf(1:h_rows, 1:h_cols)
+    int h_rows = 0, h_cols = 0;
+    unsigned long long seed = 1;   // stream of the shift-and-invert power method's start block,
+    unsigned long long offset = 0; // from this counter on
+    int converged = 0;             // output: Ritz values converged for the omegas
+};
+
+// x = IDRS_RITZ(A, b, s, ...): the solve with H = Hf(maxit+1, maxit), then
+// the requested outputs; returns nritz.  o.H / o.nritz are set here.
+template <class T>
+int idrs_ritz_solve(const Operator &A, long n, int s, const T *b, T *x, SolveOpts<T> o, RitzRequest<T> &rq,
+                    SolveStats &st, hipStream_t stream);
 
 // Standalone P generation (MAKE_P) for tests: PT(s,n) orthonormalised, device.
 template <class T>

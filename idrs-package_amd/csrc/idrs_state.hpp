@@ -21,6 +21,8 @@ struct State {
     double normb, tolb, normr, kappa, rho, tol;
     int info, iter, jj, maxit, s, method, prow, conjP, user_omega, n_omega, inispace;
     int rv_cap;                // resvec entries available on the device
+    T *H;                      // Hessenberg matrix H(nritz+1, nritz), column-major, or null
+    int nritz;
 };
 
 __device__ inline double sdiv(double a, double b) { return a / b; }
@@ -55,6 +57,8 @@ __device__ inline void scalar_init(State<T> *st, int s, int method, int prow, in
     st->n_omega = n_omega;
     st->inispace = inispace;
     st->om = T(1.0);
+    st->H = nullptr;
+    st->nritz = 0;
     st->info = -1;
     st->iter = 0;
     st->jj = 0;
@@ -81,11 +85,23 @@ __device__ inline void scalar_gamma(State<T> *st, int k) {
     }
 }
 
-// after G(:,k) = A*U(:,k) and mu = P^T G(:,k)  (idrs.inc:198-229)
+// H(row, col), 1-based, of the Hessenberg output
+template <class T>
+__device__ inline T &hess_at(State<T> *st, int row, int col) {
+    return st->H[(row - 1) + (long)(st->nritz + 1) * (col - 1)];
+}
+
+// after G(:,k) = A*U(:,k) and mu = P^T G(:,k)  (idrs.inc:198-229), with the
+// Hessenberg columns of the inner step ii = jj s + k + 1 (idrs.inc:179-181,
+// 211-213, 231-237)
 template <class T>
 __device__ inline void scalar_mu(State<T> *st, int k, const T *red) {
 #pragma clang fp contract(off)
     const int s = st->s, prow = st->prow;
+    const int ii = st->jj * s + k + 1;
+    const bool out_h = st->H != nullptr && ii <= st->nritz;
+    if (out_h && st->jj > 0)                                               // idrs.inc:180-181
+        for (int i = k; i < s; i++) hess_at(st, ii - s + i - k, ii) = -sdiv(st->gamma[i], st->beta[i]);
     T mu[SMAX];
     for (int i = 0; i < s; i++) mu[i] = i < prow ? red[i] : T(0);
     for (int i = 0; i < k; i++) {
@@ -96,6 +112,8 @@ __device__ inline void scalar_mu(State<T> *st, int k, const T *red) {
         for (int q = k; q < s; q++) mu[q] = mu[q] - IDRS_MS(q, i) * a;
     }
     for (int q = k; q < s; q++) IDRS_MS(q, k) = mu[q];
+    if (out_h)                                                             // idrs.inc:212-213
+        for (int i = 0; i < k; i++) hess_at(st, ii - k + i, ii) = sdiv(st->alpha[i], st->beta[i]);
     if (sabs(IDRS_MS(k, k)) <= DBL_MIN) {                                  // idrs.inc:216
         st->info = 3;
         return;
@@ -103,6 +121,12 @@ __device__ inline void scalar_mu(State<T> *st, int k, const T *red) {
     T beta = sdiv(st->f[k], IDRS_MS(k, k));                                // idrs.inc:222
     st->beta[k] = beta;
     for (int q = k + 1; q < s; q++) st->f[q] = st->f[q] - beta * IDRS_MS(q, k);   // idrs.inc:228
+    if (out_h) {                                                           // idrs.inc:232-237
+        hess_at(st, ii, ii) = sdiv(T(1.0), beta);
+        const int l = ii - s > 1 ? ii - s : 1;
+        for (int i = ii + 1; i >= l + 1; i--) hess_at(st, i, ii) = hess_at(st, i, ii) - hess_at(st, i - 1, ii);
+        for (int i = l; i <= ii + 1; i++) hess_at(st, i, ii) = sdiv(hess_at(st, i, ii), st->om);
+    }
     if (k + 1 < s && st->jj > 0) scalar_gamma(st, k + 1);
 }
 

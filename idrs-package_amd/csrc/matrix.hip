@@ -26,6 +26,7 @@
 //   dots, norms) is fused into the product as an epilogue.
 #include "matrix.hpp"
 #include "spmv_epi.hpp"
+#include "ritz.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -1881,18 +1882,32 @@ void neumann(int degree, std::complex<double> center, std::complex<double> *alph
 }
 
 // coefficient computation behind real_preconditioner / complex_preconditioner
-// (matrix_module.f90:551-722, 851-1057), in real arithmetic unless complex_poly
+// (matrix_module.f90:551-722, 851-1057), in real arithmetic unless complex_poly.
+// Without foci (Chebyshev) or center (Neumann) but with H, they come from
+// analyse_spectrum(H) (matrix_module.f90:600-619, 687-706), whose Ritz values
+// are those of the leading ncol x ncol block.
 void poly_coeffs(int kind, int degree, const std::complex<double> *foci, const std::complex<double> *center,
                  const std::complex<double> *H, int ldh, int ncol, bool complex_poly, std::complex<double> *out) {
+    const bool from_h = H && ((kind == POLY_NEUMANN && !center) || (kind == POLY_CHEBYSHEV && !foci));
+    if (from_h && ldh < ncol) throw std::invalid_argument("H has fewer rows than columns");
     if (!complex_poly) {
-        std::vector<double> a((size_t)degree + 1);
+        std::vector<double> a((size_t)degree + 1), Hr;
+        if (from_h) {
+            Hr.resize((size_t)ldh * ncol);
+            for (size_t q = 0; q < Hr.size(); q++) Hr[q] = H[q].real();
+        }
         if (kind == POLY_NEUMANN) {
-            neumann_t<double>(degree, center ? center->real() : 1.0, a.data());
+            double c = 1.0;
+            if (center) c = center->real();
+            else if (from_h) ritz::analyse_spectrum(ncol, Hr.data(), ldh, nullptr, &c, nullptr);
+            neumann_t<double>(degree, c, a.data());
         } else if (kind == POLY_CHEBYSHEV) {
             double f[2] = {0.25, 1.75};
             if (foci) {
                 f[0] = foci[0].real();
                 f[1] = foci[1].real();
+            } else if (from_h) {
+                ritz::analyse_spectrum(ncol, Hr.data(), ldh, f, nullptr, nullptr);
             }
             chebyshev_t<double>(degree, f, a.data());
         } else if (kind == POLY_HESSENBERG) {
@@ -1906,12 +1921,17 @@ void poly_coeffs(int kind, int degree, const std::complex<double> *foci, const s
         for (int i = 0; i <= degree; i++) out[i] = a[i];
     } else {
         if (kind == POLY_NEUMANN) {
-            neumann_t<std::complex<double>>(degree, center ? *center : std::complex<double>(1.0), out);
+            std::complex<double> c = 1.0;
+            if (center) c = *center;
+            else if (from_h) ritz::analyse_spectrum(ncol, H, ldh, nullptr, &c, nullptr);
+            neumann_t<std::complex<double>>(degree, c, out);
         } else if (kind == POLY_CHEBYSHEV) {
             std::complex<double> f[2] = {0.25, 1.75};
             if (foci) {
                 f[0] = foci[0];
                 f[1] = foci[1];
+            } else if (from_h) {
+                ritz::analyse_spectrum(ncol, H, ldh, f, nullptr, nullptr);
             }
             chebyshev_t<std::complex<double>>(degree, f, out);
         } else if (kind == POLY_HESSENBERG) {

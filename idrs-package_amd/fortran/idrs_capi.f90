@@ -32,7 +32,8 @@ module idrs_capi
       integer(c_int32_t) :: device_pointers = 0
       integer(c_int32_t) :: time_spmv = 0
       integer(c_int32_t) :: graph_mode = 0
-      integer(c_int32_t) :: reserved = 0
+      integer(c_int32_t) :: H_nritz = 0
+      type(c_ptr)        :: H = c_null_ptr
    end type idrs_opts_t
 
 ! Optional arguments of QMRIDR, MSIDRS and MSQMRIDR (idrs_krylov_opts_t)
@@ -357,6 +358,68 @@ module idrs_capi
          type(idrs_opts_t), intent(in) :: opts
          complex(c_double_complex), intent(out) :: x(*)
          type(idrs_stats_t), intent(out) :: stats
+      end function
+
+      integer(c_int) function idrs_idrs_ritz_f64(A, n, b, s, opts, x, stats, omega, n_omega, U0, dim_u0, &
+                                                 H, h_rows, h_cols, nritz, converged) &
+            bind(C, name="idrs_idrs_ritz_f64")
+         import :: c_int, c_double, c_ptr, idrs_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, s, n_omega, dim_u0, h_rows, h_cols
+         real(c_double), intent(in) :: b(*)
+         type(idrs_opts_t), intent(in) :: opts
+         real(c_double), intent(out) :: x(*)
+         type(idrs_stats_t), intent(out) :: stats
+         type(c_ptr), value :: omega, U0, H
+         integer(c_int), intent(out) :: nritz, converged
+      end function
+
+      integer(c_int) function idrs_idrs_ritz_c128(A, n, b, s, opts, x, stats, omega, n_omega, U0, dim_u0, &
+                                                  H, h_rows, h_cols, nritz, converged) &
+            bind(C, name="idrs_idrs_ritz_c128")
+         import :: c_int, c_double_complex, c_ptr, idrs_opts_t, idrs_stats_t
+         type(c_ptr), value :: A
+         integer(c_int), value :: n, s, n_omega, dim_u0, h_rows, h_cols
+         complex(c_double_complex), intent(in) :: b(*)
+         type(idrs_opts_t), intent(in) :: opts
+         complex(c_double_complex), intent(out) :: x(*)
+         type(idrs_stats_t), intent(out) :: stats
+         type(c_ptr), value :: omega, U0, H
+         integer(c_int), intent(out) :: nritz, converged
+      end function
+
+! Spectral routines on small dense matrices (complex values, pairs)
+      integer(c_int) function idrs_qr_eig(n, H, ldh, is_complex, eig, U, converged) bind(C, name="idrs_qr_eig")
+         import :: c_int, c_double_complex, c_ptr
+         integer(c_int), value :: n, ldh, is_complex
+         complex(c_double_complex), intent(in) :: H(*)
+         complex(c_double_complex), intent(out) :: eig(*)
+         type(c_ptr), value :: U
+         integer(c_int), intent(out) :: converged
+      end function
+
+      integer(c_int) function idrs_filter_ritzval(n, ritzval, outlier) bind(C, name="idrs_filter_ritzval")
+         import :: c_int, c_double_complex
+         integer(c_int), value :: n
+         complex(c_double_complex), intent(in) :: ritzval(*)
+         integer(c_int), intent(out) :: outlier(*)
+      end function
+
+      integer(c_int) function idrs_analyse_spectrum(n, H, ldh, is_complex, foci, center, shift) &
+            bind(C, name="idrs_analyse_spectrum")
+         import :: c_int, c_double_complex
+         integer(c_int), value :: n, ldh, is_complex
+         complex(c_double_complex), intent(in) :: H(*)
+         complex(c_double_complex), intent(out) :: foci(2), center, shift
+      end function
+
+      integer(c_int) function idrs_si_power(n, H, ldh, is_complex, s, shift, seed, offset, schurvec) &
+            bind(C, name="idrs_si_power")
+         import :: c_int, c_int64_t, c_double_complex
+         integer(c_int), value :: n, ldh, is_complex, s
+         complex(c_double_complex), intent(in) :: H(*), shift
+         integer(c_int64_t), value :: seed, offset
+         complex(c_double_complex), intent(out) :: schurvec(*)
       end function
 
       integer(c_int) function idrs_qmridr_f64(A, n, b, s, opts, x, stats) bind(C, name="idrs_qmridr_f64")

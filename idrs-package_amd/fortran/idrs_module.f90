@@ -10,7 +10,8 @@
 !   MSIDRS   (RMSIDRS, CMSIDRS)      idrs_module.f90:572-713, msidrs.inc:1-334
 !   NORM, INPROD               idrs_module.f90:787-837 (host arrays)
 !
-! The optional output H (Hessenberg matrix for IDRS_RITZ) is not offered.
+! The optional output H of IDRS (Hessenberg matrix, idrs.inc:81-90) is
+! filled on the device; IDRS_RITZ is in ritz_module.
 !
 module idrs_module
 
@@ -168,13 +169,20 @@ contains
 
       type(idrs_opts_t)                      :: o
       type(idrs_stats_t)                     :: st
-      real(kind=rp), allocatable, target     :: bx0(:), bU0(:,:), bom(:), brv(:)
+      real(kind=rp), allocatable, target     :: bx0(:), bU0(:,:), bom(:), brv(:), bH(:,:)
       real(kind=rp), allocatable             :: bb(:)
-      integer                                :: n, maxit, rvlen
+      integer                                :: n, maxit, rvlen, nritz
 
       n = size(b)
+      nritz = 0
       if ( present(H) ) then
-         if ( size(H,1) > 1 ) error stop 'IDRS: the H output (for IDRS_RITZ) is not offered by the GPU library'
+         nritz = size(H,1) - 1
+         if ( nritz > 0 ) then
+            if ( size(H,2) /= nritz ) stop "Second dimension of H incompatible, with first"
+            allocate( bH(nritz+1, nritz) )
+            o%H = c_loc(bH)
+            o%H_nritz = nritz
+         end if
       end if
       rvlen = -1
       if ( present(resvec) ) rvlen = size(resvec)
@@ -210,6 +218,7 @@ contains
       if ( present(relres) )     relres = st%relres
       if ( present(iterations) ) iterations = st%iterations
       if ( present(resvec) )     resvec = brv
+      if ( nritz > 0 )           H = bH
    end function RIDRS
 
    function CIDRS( A, b, s, &                                     ! required
@@ -236,14 +245,21 @@ contains
 
       type(idrs_opts_t)                      :: o
       type(idrs_stats_t)                     :: st
-      complex(kind=cp), allocatable, target  :: bx0(:), bU0(:,:), bom(:)
+      complex(kind=cp), allocatable, target  :: bx0(:), bU0(:,:), bom(:), bH(:,:)
       real(kind=rp), allocatable, target     :: brv(:)
       complex(kind=cp), allocatable          :: bb(:)
-      integer                                :: n, maxit, rvlen
+      integer                                :: n, maxit, rvlen, nritz
 
       n = size(b)
+      nritz = 0
       if ( present(H) ) then
-         if ( size(H,1) > 1 ) error stop 'IDRS: the H output (for IDRS_RITZ) is not offered by the GPU library'
+         nritz = size(H,1) - 1
+         if ( nritz > 0 ) then
+            if ( size(H,2) /= nritz ) stop "Second dimension of H incompatible, with first"
+            allocate( bH(nritz+1, nritz) )
+            o%H = c_loc(bH)
+            o%H_nritz = nritz
+         end if
       end if
       rvlen = -1
       if ( present(resvec) ) rvlen = size(resvec)
@@ -279,6 +295,7 @@ contains
       if ( present(relres) )     relres = st%relres
       if ( present(iterations) ) iterations = st%iterations
       if ( present(resvec) )     resvec = brv
+      if ( nritz > 0 )           H = bH
    end function CIDRS
 
    function RQMRIDR( A, b, s, &                                    ! required

@@ -15,10 +15,11 @@
 !                 ms_rscaleback / ms_cscaleback               :724-806
 !   MS_PARAMETERS real_ / complex_ms_parameters               :402-549
 !   Pmul          identity without deflation matrix_module.f90:1384-1431
+!   filter_ritzval, ANALYSE_SPECTRUM         matrix_module.f90:1059-1282
+!                 (host routines of the library)
 !
 ! Not offered (stop with a message): deflation, mass/damping matrices,
-! block systems, shift-and-invert, spectrum analysis from a Hessenberg
-! matrix (analyse_spectrum).
+! block systems, shift-and-invert.
 !
 module matrix_module
 
@@ -31,7 +32,8 @@ module matrix_module
 
    private
    public :: matrix, operator(*), operator(/), CHEBYSHEV, NEUMANN, SCALEBACK, COMPUTE_ALPHA, Pmul, &
-             real_preconditioner, complex_preconditioner, matrix_handle, MS_PARAMETERS
+             real_preconditioner, complex_preconditioner, matrix_handle, MS_PARAMETERS, &
+             filter_ritzval, ANALYSE_SPECTRUM
 
    type :: matrix
       type(user_matrix)              :: K, M, C, P
@@ -83,6 +85,10 @@ module matrix_module
 
    interface MS_PARAMETERS
       module procedure COMPLEX_MS_PARAMETERS, REAL_MS_PARAMETERS
+   end interface
+
+   interface ANALYSE_SPECTRUM
+      module procedure RANALYSE_SPECTRUM, CANALYSE_SPECTRUM
    end interface
 
 ! Cache of device operators built from type(matrix) states
@@ -263,7 +269,7 @@ contains
             if ( present(center) ) then
                c = center - A%rseed
             elseif ( present(H) ) then
-               error stop 'real_preconditioner: spectrum analysis (analyse_spectrum) is not offered; pass center'
+               call analyse_spectrum( H, center = c )
             else
                c = 1._rp
             end if
@@ -273,7 +279,7 @@ contains
             if ( present(foci) ) then
                f = foci - A%rseed
             elseif ( present(H) ) then
-               error stop 'real_preconditioner: spectrum analysis (analyse_spectrum) is not offered; pass foci'
+               call analyse_spectrum( H, foci = f )
             else
                f(1) = 0.25
                f(2) = 1.75
@@ -304,7 +310,7 @@ contains
             if ( present(center) ) then
                c = center - A%cseed
             elseif ( present(H) ) then
-               error stop 'complex_preconditioner: spectrum analysis (analyse_spectrum) is not offered; pass center'
+               call analyse_spectrum( H, center = c )
             else
                c = 1._cp
             end if
@@ -314,7 +320,7 @@ contains
             if ( present(foci) ) then
                f = foci - A%cseed
             elseif ( present(H) ) then
-               error stop 'complex_preconditioner: spectrum analysis (analyse_spectrum) is not offered; pass foci'
+               call analyse_spectrum( H, foci = f )
             else
                f(1) = 0.25_cp
                f(2) = 1.75_cp
@@ -326,6 +332,39 @@ contains
             A%calpha_p = compute_alpha( A%degree, H )
       end select
    end subroutine complex_preconditioner
+
+! filter_ritzval (matrix_module.f90:1059-1077)
+   function filter_ritzval( ritzval ) result(outlier)
+      complex(kind=cp), intent(in)      :: ritzval(:)
+      logical                           :: outlier(size(ritzval))
+      integer(c_int)                    :: o(size(ritzval))
+      call idrs_check( idrs_filter_ritzval(size(ritzval), ritzval, o), 'filter_ritzval' )
+      outlier = o /= 0
+   end function filter_ritzval
+
+! ranalyse_spectrum (matrix_module.f90:1079-1181)
+   subroutine ranalyse_spectrum( H, foci, center, shift )
+      real(kind=rp), intent(in)            :: H(:,:)
+      real(kind=rp), intent(out), optional :: foci(2), center, shift
+      complex(kind=cp), allocatable        :: Hc(:,:)
+      complex(kind=cp)                     :: f(2), c, sh
+      Hc = cmplx(H, 0._rp, kind=cp)
+      call idrs_check( idrs_analyse_spectrum(size(H,2), Hc, size(H,1), 0, f, c, sh), 'analyse_spectrum' )
+      if ( present(foci) )   foci = real(f, kind=rp)
+      if ( present(center) ) center = real(c, kind=rp)
+      if ( present(shift) )  shift = real(sh, kind=rp)
+   end subroutine ranalyse_spectrum
+
+! canalyse_spectrum (matrix_module.f90:1183-1282)
+   subroutine canalyse_spectrum( H, foci, center, shift )
+      complex(kind=cp), intent(in)            :: H(:,:)
+      complex(kind=cp), intent(out), optional :: foci(2), center, shift
+      complex(kind=cp)                        :: f(2), c, sh
+      call idrs_check( idrs_analyse_spectrum(size(H,2), H, size(H,1), 1, f, c, sh), 'analyse_spectrum' )
+      if ( present(foci) )   foci = f
+      if ( present(center) ) center = c
+      if ( present(shift) )  shift = sh
+   end subroutine canalyse_spectrum
 
    function rchebyshev( degree, foci ) result(alpha)
       integer, intent(in)                      :: degree

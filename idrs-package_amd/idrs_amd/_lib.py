@@ -45,7 +45,8 @@ class idrs_opts_t(ctypes.Structure):
         ("device_pointers", c_int32),
         ("time_spmv", c_int32),
         ("graph_mode", c_int32),
-        ("reserved", c_int32),
+        ("H_nritz", c_int32),
+        ("H", c_void_p),
     ]
 
 
@@ -142,6 +143,16 @@ _PROTOS = {
     "idrs_idrs_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p, POINTER(idrs_stats_t)]),
     "idrs_idrs_f64_noalloc": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p,
                                       POINTER(idrs_stats_t)]),
+    "idrs_idrs_ritz_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p,
+                                   POINTER(idrs_stats_t), c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
+                                   POINTER(c_int), POINTER(c_int)]),
+    "idrs_idrs_ritz_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_void_p,
+                                    POINTER(idrs_stats_t), c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
+                                    POINTER(c_int), POINTER(c_int)]),
+    "idrs_qr_eig": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, POINTER(c_int)]),
+    "idrs_filter_ritzval": (c_int, [c_int, c_void_p, c_void_p]),
+    "idrs_analyse_spectrum": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "idrs_si_power": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_uint64, c_uint64, c_void_p]),
     "idrs_qmridr_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t), c_void_p,
                                 POINTER(idrs_stats_t)]),
     "idrs_qmridr_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t), c_void_p,

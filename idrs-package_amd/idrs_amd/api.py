@@ -22,7 +22,12 @@ import numpy as np
 from . import _lib
 from ._lib import check, idrs_krylov_opts_t, idrs_opts_t, idrs_stats_t
 
-IdrsResult = namedtuple("IdrsResult", "x flag relres iterations resvec solve_ms spmv_ms spmv_launches")
+# H: the Hessenberg matrix H(nritz+1, nritz) when asked for, else None
+IdrsResult = namedtuple("IdrsResult", "x flag relres iterations resvec solve_ms spmv_ms spmv_launches H",
+                        defaults=(None,))
+# IDRS_RITZ: nritz Ritz values available; omega, U0, H when asked for;
+# converged: Ritz values the QR iteration converged for the omegas
+RitzResult = namedtuple("RitzResult", "x flag relres iterations resvec nritz omega U0 H converged solve_ms")
 # relres: per-shift relative residuals (array) for the multishift solvers;
 # colfac: MSIDRS collinearity factors when asked for, else None
 KrylovResult = namedtuple("KrylovResult", "x flag relres iterations resvec solve_ms colfac")
@@ -357,9 +362,12 @@ def make_p(n, s, method=1, PT_raw=None, r=None, seed=1, complex=False):
 
 
 def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, U0=None,
-         omega=None, resvec=False, PT_raw=None, p_seed=None, time_spmv=False, graph=None, out=None):
+         omega=None, resvec=False, PT_raw=None, p_seed=None, time_spmv=False, graph=None, out=None, H=None):
     """x = IDRS(A, b, s, M1, tolerance, maximum_iterations, variant, flag,
-    relres, iterations, x0, U0, omega, resvec)  (idrs_module.f90:107-251).
+    relres, iterations, x0, U0, omega, resvec, H)  (idrs_module.f90:107-251).
+
+    H: nritz (an int) asks for the Hessenberg matrix H(nritz+1, nritz) of
+    the inner steps (idrs.inc:81-90, 179-181, 211-213, 231-237).
 
     Returns IdrsResult(x, flag, relres, iterations, resvec, solve_ms, ...).
     b may be a numpy array (host) or a torch CUDA tensor (device-resident
@@ -433,12 +441,133 @@ def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None
     o.p_seed = int(p_seed) if p_seed is not None else 0
     o.time_spmv = 1 if time_spmv else 0
     o.graph_mode = 0 if graph is None else (1 if graph else 2)
+    Hm = None
+    if H is not None:
+        Hm = np.zeros((int(H) + 1, int(H)), dtype=npdt, order="F")
+        o.H, o.H_nritz = Hm.ctypes.data, int(H)
     st = idrs_stats_t()
     f = lib.idrs_idrs_c128 if cplx else lib.idrs_idrs_f64
     check(f(A.handle, n, bp, int(s), ctypes.byref(o), xp, ctypes.byref(st)))
     if rv is not None:
         rv = rv[: st.iterations + 1]
-    return IdrsResult(x, st.flag, st.relres, st.iterations, rv, st.solve_ms, st.spmv_ms, st.spmv_launches)
+    return IdrsResult(x, st.flag, st.relres, st.iterations, rv, st.solve_ms, st.spmv_ms, st.spmv_launches, Hm)
+
+
+def idrs_ritz(A, b, s, M=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, n_omega=0,
+              dim_u0=0, H_shape=None, resvec=False, PT_raw=None, p_seed=None):
+    """x = IDRS_RITZ(A, b, s, M, tolerance, maximum_iterations, variant,
+    flag, relres, iterations, x0, U0, omega, resvec, H)
+    (ritz_module.f90:225-556): the IDRS solve, then n_omega suggested
+    omegas, the recycle space U0(n, dim_u0) and H = Hf(H_shape) of its
+    Hessenberg matrix.  Host arrays (numpy)."""
+    lib = _lib.load()
+    n = int(len(b))
+    cplx = A.is_complex or np.iscomplexobj(b)
+    npdt = np.complex128 if cplx else np.float64
+    b = np.ascontiguousarray(b, dtype=npdt)
+    x = np.empty_like(b)
+    o = idrs_opts_t()
+    keep = []
+    if M is not None:
+        o.has_M1, o.M1 = 1, M.handle
+    if tolerance is not None:
+        if tolerance < 0:
+            raise ValueError("Illegal value for parameter tolerance")
+        o.has_tolerance, o.tolerance = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    maxit = int(maximum_iterations) if maximum_iterations is not None else min(2 * n, 1000)
+    if x0 is not None:
+        a = np.ascontiguousarray(x0, dtype=npdt)
+        keep.append(a)
+        o.x0 = a.ctypes.data
+    if PT_raw is not None:
+        a = np.ascontiguousarray(PT_raw, dtype=npdt)
+        keep.append(a)
+        o.PT_raw = a.ctypes.data
+    rv = None
+    if resvec:
+        rv = np.zeros(maxit + 1)
+        o.resvec, o.resvec_len = rv.ctypes.data, len(rv)
+    o.p_seed = int(p_seed) if p_seed is not None else 0
+    om = np.zeros(max(n_omega, 1), dtype=npdt)
+    U0 = np.zeros((n, max(dim_u0, 1)), dtype=npdt, order="F")
+    hr, hc = H_shape if H_shape is not None else (0, 0)
+    Hm = np.zeros((max(hr, 1), max(hc, 1)), dtype=npdt, order="F")
+    nritz, conv = ctypes.c_int(0), ctypes.c_int(0)
+    st = idrs_stats_t()
+    f = lib.idrs_idrs_ritz_c128 if cplx else lib.idrs_idrs_ritz_f64
+    check(f(A.handle, n, _ptr(b), int(s), ctypes.byref(o), _ptr(x), ctypes.byref(st), _ptr(om), int(n_omega),
+            _ptr(U0), int(dim_u0), _ptr(Hm), int(hr), int(hc), ctypes.byref(nritz), ctypes.byref(conv)))
+    if rv is not None:
+        rv = rv[: st.iterations + 1]
+    return RitzResult(x, st.flag, st.relres, st.iterations, rv, nritz.value, om[:n_omega] if n_omega else None,
+                      U0 if dim_u0 else None, Hm if H_shape is not None else None,
+                      conv.value if n_omega else None, st.solve_ms)
+
+
+def _pairs(a, shape_cnt):
+    a = np.asarray(a)
+    p = np.empty(2 * shape_cnt)
+    v = np.asarray(a, dtype=np.complex128).ravel(order="F")
+    p[0::2], p[1::2] = v.real, v.imag
+    return p
+
+
+def qr_eig(H, want_U=False):
+    """eig = QR_eig(H, U, converged) (dense_la_module.f90:192-403) of
+    H(1:n,1:n), n = number of columns; RQR_eig for real H, CQR_eig for
+    complex H.  Returns (eig, U or None, converged).  Host routine."""
+    H = np.asarray(H)
+    n, ldh = H.shape[1], H.shape[0]
+    hp = _pairs(H, H.size)
+    eig = np.empty(2 * n)
+    U = np.empty(2 * n * n) if want_U else None
+    conv = ctypes.c_int(0)
+    check(_lib.load().idrs_qr_eig(n, _ptr(hp), ldh, int(np.iscomplexobj(H)), _ptr(eig),
+                                  _ptr(U) if want_U else None, ctypes.byref(conv)))
+    e = eig[0::2] + 1j * eig[1::2]
+    Um = (U[0::2] + 1j * U[1::2]).reshape((n, n), order="F") if want_U else None
+    return e, Um, conv.value
+
+
+def filter_ritzval(ritzval):
+    """outlier = filter_ritzval(ritzval) (matrix_module.f90:1059-1077)."""
+    rv = np.asarray(ritzval, dtype=np.complex128)
+    out = np.zeros(len(rv), dtype=np.int32)
+    check(_lib.load().idrs_filter_ritzval(len(rv), _ptr(_pairs(rv, len(rv))), _ptr(out)))
+    return out.astype(bool)
+
+
+def analyse_spectrum(H):
+    """(foci, center, shift) = analyse_spectrum(H) (matrix_module.f90:
+    1079-1282); real values for real H."""
+    H = np.asarray(H)
+    cplx = np.iscomplexobj(H)
+    f, c, sh = np.empty(4), np.empty(2), np.empty(2)
+    check(_lib.load().idrs_analyse_spectrum(H.shape[1], _ptr(_pairs(H, H.size)), H.shape[0], int(cplx), _ptr(f),
+                                            _ptr(c), _ptr(sh)))
+    foci = f[0::2] + 1j * f[1::2]
+    center, shift = complex(c[0], c[1]), complex(sh[0], sh[1])
+    if not cplx:
+        return foci.real, center.real, shift.real
+    return foci, center, shift
+
+
+def si_power(H, s, shift=0.0, seed=1, offset=0):
+    """schurvec = SI_power(H, s, shift) (dense_la_module.f90:455-603),
+    started from the library's uniform stream (seed) at counter offset."""
+    H = np.asarray(H)
+    cplx = np.iscomplexobj(H)
+    n = H.shape[1]
+    out = np.empty(2 * n * s)
+    sh = np.array([np.real(shift), np.imag(shift)])
+    check(_lib.load().idrs_si_power(n, _ptr(_pairs(H, H.size)), H.shape[0], int(cplx), int(s), _ptr(sh),
+                                    int(seed), int(offset), _ptr(out)))
+    v = (out[0::2] + 1j * out[1::2]).reshape((n, s), order="F")
+    return v if cplx else v.real.copy()
 
 
 def _krylov(kind, A, b, s, sigma=None, M1=None, tolerance=None, maximum_iterations=None, variant=None,

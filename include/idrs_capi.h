@@ -239,7 +239,10 @@ typedef struct {
                                    batched solve always); 1: always;
                                    2: never.  No reference counterpart; the
                                    results are identical either way        */
-    int32_t reserved;
+    int32_t H_nritz;            /* H(H_nritz+1, H_nritz): size(H,1)-1      */
+    void *H;                    /* IDRS only: Hessenberg matrix, host,
+                                   column-major, output (idrs.inc:81-90,
+                                   179-181, 211-213, 231-237); NULL: absent */
 } idrs_opts_t;
 
 typedef struct {
@@ -253,14 +256,54 @@ typedef struct {
 } idrs_stats_t;
 
 /* x = IDRS(A, b, s, [M1, tolerance, maximum_iterations, variant, flag,
-   relres, iterations, x0, U0, omega, resvec]): RIDRS,
-   MODULES/IDRS/idrs_module.f90:180-251 with MODULES/IDRS/idrs.inc:1-311.
-   (The H output, used only by the out-of-scope IDRS_RITZ, is not offered.) */
+   relres, iterations, x0, U0, omega, resvec, H]): RIDRS,
+   MODULES/IDRS/idrs_module.f90:180-251 with MODULES/IDRS/idrs.inc:1-311. */
 int idrs_idrs_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                   idrs_stats_t *stats);
 /* CIDRS, idrs_module.f90:107-178 */
 int idrs_idrs_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
                    idrs_stats_t *stats);
+/* x = IDRS_RITZ(A, b, s, [M, tolerance, maximum_iterations, variant,
+   flag, relres, iterations, x0, U0, omega, resvec, H]): RIDRS_RITZ,
+   MODULES/IDRS/ritz_module.f90:427-556.  opts as for idrs_idrs_f64, with
+   has_M1/M1 the argument M (IDRS's preconditioner and v/M in compute_U0);
+   opts' U0, omega and H must be absent (they are outputs here).  Outputs
+   (NULL / 0: absent): omega(n_omega), host; U0(n, dim_u0) -- a device
+   pointer when opts->device_pointers, else host; H(h_rows, h_cols) =
+   Hf(1:h_rows, 1:h_cols) of the solve's Hf(maxit+1, maxit), host;
+   *nritz = (iter/(s+1)) s + mod(iter, s+1); *converged = Ritz values the
+   QR iteration converged for the omegas.  The reference's stop statements
+   ("Too many parameters omega requested, reduce omega.", "Recycle space
+   is too large, reduce dimension of U0.", "Dimensions of Hessenberg
+   matrix too large") return IDRS_ERR_ARG with that message.  The
+   shift-and-invert power method starts from the library's uniform stream
+   (seed p_seed) after the shadow space's values; the reference continues
+   its compiler's random_number stream instead. */
+int idrs_idrs_ritz_f64(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                       idrs_stats_t *stats, double *omega, int n_omega, double *U0, int dim_u0, double *H, int h_rows,
+                       int h_cols, int *nritz, int *converged);
+/* CIDRS_RITZ, ritz_module.f90:225-425 (complex arrays as (re, im) pairs) */
+int idrs_idrs_ritz_c128(idrs_matrix_t A, int n, const double *b, int s, const idrs_opts_t *opts, double *x,
+                        idrs_stats_t *stats, double *omega, int n_omega, double *U0, int dim_u0, double *H,
+                        int h_rows, int h_cols, int *nritz, int *converged);
+/* Spectral routines on small dense matrices (host, no GPU).  Matrices
+   column-major with leading dimension ldh >= n; complex values as (re, im)
+   pairs, H too (imaginary parts ignored unless is_complex).
+   eig(n) = QR_eig(H(1:n,1:n)) (dense_la_module.f90:192-403: CQR_eig for
+   is_complex, else RQR_eig), U(n,n) its Schur vectors when non-NULL,
+   *converged when non-NULL. */
+int idrs_qr_eig(int n, const double *H, int ldh, int is_complex, double *eig, double *U, int *converged);
+/* outlier(n) = filter_ritzval(ritzval(n)) (matrix_module.f90:1059-1077), 0/1 */
+int idrs_filter_ritzval(int n, const double *ritzval, int *outlier);
+/* analyse_spectrum(H(1:n,1:n), foci, center, shift)
+   (matrix_module.f90:1079-1282); each output (2, 1, 1 values) may be NULL */
+int idrs_analyse_spectrum(int n, const double *H, int ldh, int is_complex, double *foci, double *center,
+                          double *shift);
+/* schurvec(n, s) = SI_power(H(1:n,1:n), s, shift) (dense_la_module.f90:
+   455-603); shift NULL: 0; start block from the uniform stream (seed) at
+   counter offset */
+int idrs_si_power(int n, const double *H, int ldh, int is_complex, int s, const double *shift, uint64_t seed,
+                  uint64_t offset, double *schurvec);
 /* idrs_idrs_f64 that allocates nothing.  No counterpart in the reference:
    for callers that cannot allocate device or pinned memory (code under HIP
    graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -90,6 +90,9 @@ typedef struct {
        resvec is (resvec_len, n_sigma) column-major for the multishift solvers */
     double *relres_vec;
     void *colfac;
+    /* IDRS: Hessenberg matrix H(H_nritz+1, H_nritz), T column-major, output
+       (idrs.inc:81-90, 179-181, 211-213, 231-237), or NULL */
+    void *H; int H_nritz;
 } orc_idrs_opts;
 
 typedef struct {

@@ -57,6 +57,7 @@ class Opts(ctypes.Structure):
         ("has_inner_it", c_int), ("inner_it", c_int),
         ("relres_vec", c_void_p),
         ("colfac", c_void_p),
+        ("H", c_void_p), ("H_nritz", c_int),
     ]
 
 
@@ -186,6 +187,15 @@ class OracleMatrix:
         f(ctypes.byref(self.m), c_void_p(v.ctypes.data), c_void_p(w.ctypes.data))
         return w
 
+    def precon(self, v):
+        """v / A (rprecon): divides by the preconditioner diagonal, or copies."""
+        dt = np.complex128 if self.complex else np.float64
+        v = np.ascontiguousarray(v, dtype=dt)
+        w = np.empty_like(v)
+        f = load().orc_c_precon if self.complex else load().orc_r_precon
+        f(ctypes.byref(self.m), c_void_p(v.ctypes.data), c_void_p(w.ctypes.data))
+        return w
+
     def user_mv(self, v):
         dt = np.complex128 if self.complex else np.float64
         v = np.ascontiguousarray(v, dtype=dt)
@@ -244,10 +254,11 @@ def make_p(n, s, method, PT_raw=None, r=None, complex=False):
 
 
 def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None, x0=None, U0=None, omega=None,
-         resvec=False, PT_raw=None, lib=None):
+         resvec=False, PT_raw=None, lib=None, H_nritz=None):
     """Oracle IDRS with the reference's optional arguments; returns
-    (x, flag, relres, iterations, resvec).  lib: load_timed()'s library to
-    run the OpenMP build instead of the checker."""
+    (x, flag, relres, iterations, resvec), and H(H_nritz+1, H_nritz) as a
+    sixth item when H_nritz is given.  lib: load_timed()'s library to run
+    the OpenMP build instead of the checker."""
     lib = lib or load()
     cplx = A.complex
     dt = np.complex128 if cplx else np.float64
@@ -284,6 +295,10 @@ def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None
         o.resvec, o.resvec_len = rv.ctypes.data, len(rv)
     if PT_raw is not None:
         o.PT_raw = hold(PT_raw)
+    H = None
+    if H_nritz is not None:
+        H = np.zeros((H_nritz + 1, H_nritz), dtype=dt, order="F")
+        o.H, o.H_nritz = H.ctypes.data, int(H_nritz)
     st = Stats()
     f = lib.orc_c_idrs if cplx else lib.orc_r_idrs
     rc = f(ctypes.byref(A.m), n, c_void_p(b.ctypes.data), s, ctypes.byref(o), c_void_p(x.ctypes.data), ctypes.byref(st))
@@ -291,6 +306,8 @@ def idrs(A, b, s, M1=None, tolerance=None, maximum_iterations=None, variant=None
         raise ValueError("oracle idrs rejected its arguments")
     if rv is not None:
         rv = rv[: st.iterations + 1]
+    if H is not None:
+        return x, st.flag, st.relres, st.iterations, rv, H
     return x, st.flag, st.relres, st.iterations, rv
 
 

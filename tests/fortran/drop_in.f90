@@ -10,6 +10,7 @@ program drop_in
    use mm_module
    use matrix_module
    use idrs_module
+   use ritz_module
 
    implicit none
 
@@ -20,6 +21,7 @@ program drop_in
    call toeplitz_complex()
    call krylov_family()
    call ms_pp_sequence()
+   call cdr_ritz_chain()
 ! Matrix Market files: the directory is the first command-line argument
    if ( command_argument_count() >= 1 ) then
       call get_command_argument( 1, data_dir )
@@ -99,19 +101,18 @@ contains
          ' relres', relres, ' err', maxval(abs(x-1._rp))
    end subroutine toeplitz_real
 
-! pp_idrs call sequence (PP_IDRS/real_matrix.inc, real_preconditioner.inc,
-! run_idrs.inc): diagonal scaling + Chebyshev(8), IDR(4), scale back.
-   subroutine cdr_preconditioned()
-      integer, parameter            :: m = 16, n = m*m*m
-      type(matrix)                  :: A, M1
+! 16^3 convection-diffusion-reaction matrix (Pe 5, reaction 1), 7-point
+! stencil, CRS
+   function cdr_matrix( m ) result(K)
+      integer, intent(in)           :: m
       type(user_matrix)             :: K
-      real(kind=rp)                 :: x(n), b(n), x0(n), relres, h, e2, c, foci(2)
       real(kind=rp), allocatable    :: val(:)
       integer, allocatable          :: row_ptr(:), col_ind(:)
-      integer                       :: flag, iter, i, j, l, q, nz
-      real(kind=rp)                 :: cw(7)
+      integer                       :: i, j, l, q, nz, n
+      real(kind=rp)                 :: cw(7), h, e2, c
       integer                       :: off(7, 3)
 
+      n = m*m*m
       h = 1._rp/(m+1)
       e2 = 1._rp/h**2
       c = 6._rp*e2 + 1._rp
@@ -139,6 +140,18 @@ contains
          end do
       end do
       K = crs_format( n, n, row_ptr, col_ind(1:nz), val(1:nz) )
+   end function cdr_matrix
+
+! pp_idrs call sequence (PP_IDRS/real_matrix.inc, real_preconditioner.inc,
+! run_idrs.inc): diagonal scaling + Chebyshev(8), IDR(4), scale back.
+   subroutine cdr_preconditioned()
+      integer, parameter            :: m = 16, n = m*m*m
+      type(matrix)                  :: A, M1
+      type(user_matrix)             :: K
+      real(kind=rp)                 :: x(n), b(n), x0(n), relres, foci(2)
+      integer                       :: flag, iter
+
+      K = cdr_matrix( m )
       x = 1._rp
       b = K*x
       A%K = K
@@ -153,6 +166,37 @@ contains
       call user_matrix_free( A%P )
       call user_matrix_free( K )
    end subroutine cdr_preconditioned
+
+! pp_idrs without foci (PP_IDRS/real_preconditioner.inc:12-34, run_idrs.inc):
+! initial IDRS_RITZ iterations (n_ritz = 32, ini_iter = n_ritz + n_ritz/s)
+! for the omegas and H, real_preconditioner(A, 2, 8, H = H(1:n_ritz,1:n_ritz))
+! (foci from analyse_spectrum), then IDR(4) from the initial solution with
+! those omegas, scale back.
+   subroutine cdr_ritz_chain()
+      integer, parameter            :: m = 16, n = m*m*m, s = 4, n_ritz = 32, ini = n_ritz + n_ritz/s
+      type(matrix)                  :: A, M1
+      type(user_matrix)             :: K
+      real(kind=rp)                 :: x(n), b(n), x0(n), y(n), relres, omega(s), H(ini+1, ini)
+      integer                       :: flag, iter, nr
+
+      K = cdr_matrix( m )
+      x = 1._rp
+      b = K*x
+      A%K = K
+      A%P = real_precon( 1, K )
+      x0 = idrs_ritz( A, b, s, M1, 1.e-8_rp, ini, 1, flag, relres, iter, omega = omega, H = H )
+      nr = (iter/(s+1))*s + mod(iter,s+1)
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'cdr_ritz_init', ' iter', iter, ' flag', nr, &
+         ' relres', relres, ' err', omega(1)
+      call real_preconditioner( A, 2, 8, H = H(1:nr,1:nr) )
+      y = idrs( A, Pmul(A,b), s, M1, 1.e-8_rp, 2000, 1, flag, relres, iter, x0, omega = omega )
+      x = scaleback( A, b, y )
+      relres = norm( b - K*x )/norm( b )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'cdr_ritz', ' iter', iter, ' flag', flag, &
+         ' relres', relres, ' err', maxval(abs(x-1._rp))
+      call user_matrix_free( A%P )
+      call user_matrix_free( K )
+   end subroutine cdr_ritz_chain
 
 ! Complex Toeplitz accuracy test (toeplitz.f90:41-70) in offset storage.
    subroutine toeplitz_complex()
