@@ -155,3 +155,23 @@ def test_preconditioner_from_ritz_values(gpu, kind):
     ro = orc.idrs(Dp, b, s, M1=M1o, tolerance=1e-8, maximum_iterations=2000, omega=o["omega"])
     assert r.flag == ro[1] == 0
     assert abs(r.iterations - ro[3]) <= max(2, 0.05 * ro[3])
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_recycle_space_solve(gpu, cplx):
+    """PP_IDRS/recycle.inc: IDRS_RITZ returns U0(n, s); the next solve starts
+    its first cycle from that search space (idrs.inc:55-65, 183-190).  The
+    GPU's U0 drives both the GPU and the oracle solve (equal at a fixed
+    count), and the recycled solve converges."""
+    crs = _cd3(m=10, cplx=cplx)
+    n, s = crs.n, 4
+    A, Ao = gpu_crs(crs), oracle_crs(crs)
+    g = idrs_amd.idrs_ritz(A, _rhs(n, cplx, seed=4), s, tolerance=1e-8, maximum_iterations=400, dim_u0=s)
+    U0 = g.U0
+    b2 = _rhs(n, cplx, seed=5)
+    r = idrs_amd.idrs(A, b2, s, tolerance=0.0, maximum_iterations=25, U0=U0)
+    ro = orc.idrs(Ao, b2, s, tolerance=0.0, maximum_iterations=25, U0=U0)
+    assert rel(r.x, ro[0]) < 1e-10
+    r = idrs_amd.idrs(A, b2, s, tolerance=1e-8, maximum_iterations=1000, U0=U0)
+    ro = orc.idrs(Ao, b2, s, tolerance=1e-8, maximum_iterations=1000, U0=U0)
+    assert r.flag == ro[1] == 0 and abs(r.iterations - ro[3]) <= 2
