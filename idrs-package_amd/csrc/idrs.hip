@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_pdot(long n, const T *__restrict__ w, c
 #pragma unroll
         for (int i = 0; i < NV; i++)
             if (i < prow) {
-                T p = P[q * prow + i];
+                T p = ldnt(&P[q * prow + i]);
                 if (conjP) p = Tr<T>::conj(p);
                 e[i] = fmaT(p, wq, e[i]);
             }
@@ -379,9 +379,9 @@ __global__ __launch_bounds__(256) void k_form_u(long n, int k, const State<T> *s
     if (jj > 0) {
         GRID_LOOP(q, n) {
             T v = r[q];
-            for (int i = k; i < s; i++) v = fmaT(-sg[i], G[(long)i * n + q], v);
+            for (int i = k; i < s; i++) v = fmaT(-sg[i], ldnt(&G[(long)i * n + q]), v);
             T t = om * precon_div(v, m1, q);
-            for (int i = k; i < s; i++) t = fmaT(sg[i], U[(long)i * n + q], t);
+            for (int i = k; i < s; i++) t = fmaT(sg[i], ldnt(&U[(long)i * n + q]), t);
             U[(long)k * n + q] = t;
         }
     } else if (!st->inispace) {
@@ -409,8 +409,8 @@ __global__ __launch_bounds__(256) void k_orth_update(long n, int k, const State<
     GRID_LOOP(q, n) {
         T gk = G[(long)k * n + q], uk = U[(long)k * n + q];
         for (int i = 0; i < k; i++) {
-            gk = fmaT(-sa[i], G[(long)i * n + q], gk);
-            uk = fmaT(-sa[i], U[(long)i * n + q], uk);
+            gk = fmaT(-sa[i], ldnt(&G[(long)i * n + q]), gk);
+            uk = fmaT(-sa[i], ldnt(&U[(long)i * n + q]), uk);
         }
         G[(long)k * n + q] = gk;
         U[(long)k * n + q] = uk;
@@ -421,9 +421,9 @@ __global__ __launch_bounds__(256) void k_orth_update(long n, int k, const State<
         if (next) {
             if (jj > 0) {
                 T v = rq;
-                for (int i = k + 1; i < s; i++) v = fmaT(-sg[i], G[(long)i * n + q], v);
+                for (int i = k + 1; i < s; i++) v = fmaT(-sg[i], ldnt(&G[(long)i * n + q]), v);
                 T t = om * precon_div(v, m1, q);
-                for (int i = k + 1; i < s; i++) t = fmaT(sg[i], U[(long)i * n + q], t);
+                for (int i = k + 1; i < s; i++) t = fmaT(sg[i], ldnt(&U[(long)i * n + q]), t);
                 U[(long)(k + 1) * n + q] = t;
             } else if (!ini) {
                 U[(long)(k + 1) * n + q] = precon_div(rq, m1, q);
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(256) void k_omega_update(long n, const State<T> *st
             if (prow == NV) {   // whole, even-length P row: 16-byte loads
 #pragma unroll
                 for (int i = 0; i < NV; i += 2) {
-                    const double2 p2 = *(const double2 *)(P + q * NV + i);
+                    const dbl2_nt p2 = __builtin_nontemporal_load((const dbl2_nt *)(P + q * NV + i));
                     e[1 + i] = fmaT(p2.x, rq, e[1 + i]);
                     e[2 + i] = fmaT(p2.y, rq, e[2 + i]);
                 }
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(256) void k_omega_update(long n, const State<T> *st
 #pragma unroll
         for (int i = 0; i < NV; i++)
             if (i < prow) {
-                T p = P[q * prow + i];
+                T p = ldnt(&P[q * prow + i]);
                 if (conjP) p = Tr<T>::conj(p);
                 e[1 + i] = fmaT(p, rq, e[1 + i]);
             }

@@ -90,7 +90,7 @@ __device__ inline void epi_load(T (&pe)[PC], const EpiArgs<T> &E, long row, int 
             if (step == 1 && c == 0 && E.prow == PC) {
 #pragma unroll
                 for (int k = 0; k < PC; k += 2) {
-                    const double2 v = *(const double2 *)(p + k);
+                    const dbl2_nt v = __builtin_nontemporal_load((const dbl2_nt *)(p + k));
                     pe[k] = v.x;
                     pe[k + 1] = v.y;
                 }
@@ -99,7 +99,7 @@ __device__ inline void epi_load(T (&pe)[PC], const EpiArgs<T> &E, long row, int 
         }
         const int last = E.prow - 1;
 #pragma unroll
-        for (int k = 0; k < PC; k++) pe[k] = p[min(c + k * step, last)];
+        for (int k = 0; k < PC; k++) pe[k] = ldnt(p + min(c + k * step, last));
     } else if constexpr (EPI == EPI_OMEGA) {
         pe[0] = E.r[row];
     }
@@ -162,16 +162,6 @@ __device__ inline void epi_store_group(T (&e)[PC], const EpiArgs<T> &E, T *lds) 
         for (int v = 1; v < NW; v++) sum = sum + lds[threadIdx.x * NW + v];
         E.partials[(long)threadIdx.x * E.nparts + blockIdx.x] = sum;
     }
-}
-
-// Non-temporal loads for the matrix streams (read once per product): they
-// should not push the gathered x entries out of L2.
-template <class V>
-__device__ inline V ldnt(const V *p) { return __builtin_nontemporal_load(p); }
-template <class R>
-__device__ inline cplx<R> ldnt(const cplx<R> *p) {
-    const R *q = reinterpret_cast<const R *>(p);
-    return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
 }
 
 }  // namespace idrs
