@@ -7,6 +7,7 @@
 #include "mixed.hpp"
 #include "mm.hpp"
 #include "ritz.hpp"
+#include "general.hpp"
 
 #include <cstdarg>
 #include <cstring>
@@ -549,7 +550,7 @@ static int matvec_entry(idrs_matrix_t A, const double *v, double *w, int what) {
         need(A && v && w, "null argument");
         hipStream_t st = lib_stream();
         op_prepare(A->op, st);
-        long n = A->op.K->nrows;
+        long n = A->op.rows();
         DVec<T> dv((const T *)v, n, st), dw(n), w1(n), w2(n), w3(n);
         EpiArgs<T> none{};
         if (what == 0) op_apply<T>(A->op, dv.p, dw.p, w1.p, w2.p, EPI_NONE, none, nullptr, st);
@@ -565,11 +566,12 @@ static int matvec_block_entry(idrs_matrix_t A, int ncols, const double *V, doubl
         need(A && V && W, "null argument");
         need(ncols >= 1, "ncols must be at least 1");
         need(A->op.K->nrows == A->op.K->ncols, "block products need a square matrix");
+        const long rows = A->op.rows();
         hipStream_t st = lib_stream();
         op_prepare(A->op, st);
-        const size_t cnt = (size_t)A->op.K->nrows * ncols;
+        const size_t cnt = (size_t)rows * ncols;
         DVec<T> dv((const T *)V, cnt, st), dw(cnt), w1(cnt), w2(cnt);
-        op_apply_cols<T>(A->op, ncols, A->op.K->nrows, dv.p, dw.p, w1.p, w2.p, st);
+        op_apply_cols<T>(A->op, ncols, rows, dv.p, dw.p, w1.p, w2.p, st);
         dw.download((T *)W, st);
     });
 }
@@ -582,7 +584,7 @@ static int ms_scaleback_entry(idrs_matrix_t A, int ns, int degree, const double 
         need(ns >= 1 && degree >= 0, "bad arguments");
         hipStream_t st = lib_stream();
         op_prepare(A->op, st);
-        const long n = A->op.K->nrows;
+        const long n = A->op.rows();
         // blocks of up to 8 shifts share each product's read of the matrix
         const int nb = std::min(ns, SPMV_COLS);
         DVec<T> dy(device ? 0 : n * nb), dx(device ? 0 : n * nb), w1(n * nb), w2(n * nb);
@@ -971,6 +973,197 @@ int idrs_matvec_block_f64(idrs_matrix_t A, int ncols, const double *V, double *W
 int idrs_matvec_block_c128(idrs_matrix_t A, int ncols, const double *V, double *W) {
     return matvec_block_entry<c128>(A, ncols, V, W);
 }
+// ---------------------------------------------------------------------------
+// generalized operators: mass and damping matrices, block systems,
+// deflation (matrix_module.f90:169-264, 306-400, 1284-1623)
+// ---------------------------------------------------------------------------
+int idrs_matrix_set_mass(idrs_matrix_t A, idrs_user_matrix_t M) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        if (M) need(M->M->nrows == A->op.K->nrows && M->M->ncols == A->op.K->ncols, "M must have K's shape");
+        A->op.Mm = M ? M->M : nullptr;
+    });
+}
+int idrs_matrix_set_damping(idrs_matrix_t A, idrs_user_matrix_t C) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        if (C) need(C->M->nrows == A->op.K->nrows && C->M->ncols == A->op.K->ncols, "C must have K's shape");
+        A->op.Cm = C ? C->M : nullptr;
+    });
+}
+int idrs_matrix_set_flags(idrs_matrix_t A, int block_system, int multishift) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        if (block_system) need(A->op.ndefl == 0, "set the block flag before make_deflation");
+        A->op.block = block_system != 0;
+        A->op.multishift = multishift != 0;
+    });
+}
+int idrs_matrix_rows(idrs_matrix_t A, int *rows) {
+    return guarded([&] {
+        need(A && rows, "null argument");
+        *rows = (int)A->op.rows();
+    });
+}
+int idrs_matrix_set_precon_diag(idrs_matrix_t A, idrs_user_matrix_t PK, idrs_user_matrix_t PM, idrs_user_matrix_t PC,
+                                const double *shift, int is_complex) {
+    return guarded([&] {
+        need(A && PK, "null argument");
+        const long n = A->op.K->nrows;
+        for (idrs_user_matrix_t U : {PK, PM, PC})
+            if (U) need(U->M->nrows == n, "preconditioner matrices must have K's rows");
+        hipStream_t st = lib_stream();
+        // real_diagonal / complex_diagonal of each matrix (zeros -> 1), host
+        auto diag = [&](const DevMatrix &D) {
+            std::vector<std::complex<double>> h(n);
+            if (D.kind == ValKind::Complex) {
+                std::vector<c128> c(n);
+                HIP_TRY(hipMemcpyAsync(c.data(), D.diag, sizeof(c128) * n, hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                for (long i = 0; i < n; i++) h[i] = {c[i].re, c[i].im};
+            } else {
+                std::vector<double> r(n);
+                HIP_TRY(hipMemcpyAsync(r.data(), D.diag, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+                HIP_TRY(hipStreamSynchronize(st));
+                for (long i = 0; i < n; i++) h[i] = r[i];
+            }
+            return h;
+        };
+        std::vector<std::complex<double>> d = diag(*PK->M);
+        bool cdiag = is_complex != 0 || PK->M->kind == ValKind::Complex;
+        if (shift) {
+            const std::complex<double> s = is_complex ? std::complex<double>(shift[0], shift[1]) : shift[0];
+            if (s.imag() != 0.0) cdiag = true;
+            const std::complex<double> s2 = s * s;
+            if (PM && PC) {                       // user_module.f90:1361-1364
+                const auto dc = diag(*PC->M), dm = diag(*PM->M);
+                for (long i = 0; i < n; i++) d[i] = (d[i] + s * dc[i]) + s2 * dm[i];
+            } else if (PC) {                      // :1365-1369
+                const auto dc = diag(*PC->M);
+                for (long i = 0; i < n; i++) d[i] = d[i] + s * dc[i];
+                for (long i = 0; i < n; i++) d[i] = d[i] + s2;
+            } else if (PM) {                      // :1370-1372
+                const auto dm = diag(*PM->M);
+                for (long i = 0; i < n; i++) d[i] = d[i] - s * dm[i];
+            } else {                              // :1373-1376
+                for (long i = 0; i < n; i++) d[i] = d[i] - s;
+            }
+            for (idrs_user_matrix_t U : {PM, PC})
+                if (U && U->M->kind == ValKind::Complex) cdiag = true;
+        }
+        if (!cdiag)
+            for (long i = 0; i < n; i++) d[i] = d[i].real();
+        op_set_diag_values(A->op, d, cdiag);
+    });
+}
+
+extern "C++" {
+template <class T>
+static int make_deflation_entry(idrs_matrix_t A, int nd, const double *V) {
+    return guarded([&] {
+        need(A && V, "null argument");
+        need(nd >= 1 && nd <= MAX_DEFL, "deflation takes 1..16 vectors");
+        hipStream_t st = lib_stream();
+        const long n = A->op.rows();
+        DVec<T> dv((const T *)V, (size_t)n * nd, st);
+        gen_make_deflation<T>(A->op, dv.p, nd, st);
+    });
+}
+}  // extern "C++"
+int idrs_make_deflation_f64(idrs_matrix_t A, int nd, const double *V) { return make_deflation_entry<double>(A, nd, V); }
+int idrs_make_deflation_c128(idrs_matrix_t A, int nd, const double *V) { return make_deflation_entry<c128>(A, nd, V); }
+
+// what: 0 Pmul, 1 real_mv, 2 add_deflation(b, y), 3 scaleback(b, y)
+extern "C++" {
+template <class T>
+static int general_entry(idrs_matrix_t A, int what, const double *b, const double *v, double *w) {
+    return guarded([&] {
+        need(A && v && w && (what < 2 || b), "null argument");
+        hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
+        gen_prepare(A->op);
+        const long n = A->op.rows();
+        DVec<T> dv((const T *)v, n, st), dw(n), db, t(n), u(n);
+        if (b) db = DVec<T>((const T *)b, n, st);
+        const bool gen = A->op.general();
+        if (what == 0) {
+            HIP_TRY(hipMemcpyAsync(dw.p, dv.p, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+            if (gen) gen_pmul<T>(A->op, dw.p, st);
+        } else if (what == 1) {
+            if (gen) {
+                gen_mv<T>(A->op, dv.p, dw.p, t.p, u.p, st);
+            } else {
+                EpiArgs<T> none{};
+                op_spmv<T>(A->op, dv.p, T(1), nullptr, T(0), dw.p, EPI_NONE, none, nullptr, st);
+            }
+        } else if (what == 2) {
+            if (gen) gen_add_deflation<T>(A->op, db.p, dv.p, dw.p, st);
+            else HIP_TRY(hipMemcpyAsync(dw.p, dv.p, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+        } else {
+            if (gen) gen_scaleback<T>(A->op, db.p, dv.p, dw.p, st);
+            else op_scaleback<T>(A->op, dv.p, dw.p, t.p, u.p, nullptr, st);
+        }
+        dw.download((T *)w, st);
+    });
+}
+}  // extern "C++"
+int idrs_pmul_f64(idrs_matrix_t A, const double *v, double *w) { return general_entry<double>(A, 0, nullptr, v, w); }
+int idrs_pmul_c128(idrs_matrix_t A, const double *v, double *w) { return general_entry<c128>(A, 0, nullptr, v, w); }
+int idrs_system_mv_f64(idrs_matrix_t A, const double *v, double *w) { return general_entry<double>(A, 1, nullptr, v, w); }
+int idrs_system_mv_c128(idrs_matrix_t A, const double *v, double *w) { return general_entry<c128>(A, 1, nullptr, v, w); }
+int idrs_add_deflation_f64(idrs_matrix_t A, const double *b, const double *y, double *x) {
+    return general_entry<double>(A, 2, b, y, x);
+}
+int idrs_add_deflation_c128(idrs_matrix_t A, const double *b, const double *y, double *x) {
+    return general_entry<c128>(A, 2, b, y, x);
+}
+int idrs_scaleback_b_f64(idrs_matrix_t A, const double *b, const double *y, double *x) {
+    return general_entry<double>(A, 3, b, y, x);
+}
+int idrs_scaleback_b_c128(idrs_matrix_t A, const double *b, const double *y, double *x) {
+    return general_entry<c128>(A, 3, b, y, x);
+}
+
+// ms scale-back with the deflation step (matrix_module.f90:724-806): per
+// shift k, x = p_k(A) y_k, ms_add_deflation for sigma_k, x/A
+extern "C++" {
+template <class T>
+static int ms_scaleback_b_entry(idrs_matrix_t A, int ns, const double *sigma, int degree, const double *alpha_s,
+                                const double *b, const double *Y, double *X) {
+    return guarded([&] {
+        need(A && sigma && alpha_s && b && Y && X, "null argument");
+        need(ns >= 1 && degree >= 0, "bad arguments");
+        hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
+        gen_prepare(A->op);
+        const long n = A->op.rows();
+        DVec<T> db((const T *)b, n, st), dy(n), dx(n), w1(n), w2(n);
+        for (int k = 0; k < ns; k++) {
+            std::vector<std::complex<double>> a(degree + 1);
+            for (int i = 0; i <= degree; i++) {
+                const long q = 2 * ((long)k * (degree + 1) + i);
+                a[i] = {alpha_s[q], alpha_s[q + 1]};
+            }
+            const std::complex<double> sg = Tr<T>::is_complex ? std::complex<double>(sigma[2 * k], sigma[2 * k + 1])
+                                                              : std::complex<double>(sigma[k], 0.0);
+            HIP_TRY(hipMemcpyAsync(dy.p, (const T *)Y + (long)k * n, sizeof(T) * n, hipMemcpyHostToDevice, st));
+            if (A->op.general()) gen_ms_scaleback_col<T>(A->op, a, sg, db.p, dy.p, dx.p, st);
+            else op_scaleback_with<T>(A->op, a, dy.p, dx.p, w1.p, w2.p, st);
+            HIP_TRY(hipMemcpyAsync((T *)X + (long)k * n, dx.p, sizeof(T) * n, hipMemcpyDeviceToHost, st));
+        }
+        HIP_TRY(hipStreamSynchronize(st));
+    });
+}
+}  // extern "C++"
+int idrs_ms_scaleback_b_f64(idrs_matrix_t A, int n_sigma, const double *sigma, int degree, const double *alpha_s,
+                            const double *b, const double *Y, double *X) {
+    return ms_scaleback_b_entry<double>(A, n_sigma, sigma, degree, alpha_s, b, Y, X);
+}
+int idrs_ms_scaleback_b_c128(idrs_matrix_t A, int n_sigma, const double *sigma, int degree, const double *alpha_s,
+                             const double *b, const double *Y, double *X) {
+    return ms_scaleback_b_entry<c128>(A, n_sigma, sigma, degree, alpha_s, b, Y, X);
+}
+
 int idrs_precon_f64(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<double>(A, v, w, 1); }
 int idrs_precon_c128(idrs_matrix_t A, const double *v, double *w) { return matvec_entry<c128>(A, v, w, 1); }
 int idrs_scaleback_f64(idrs_matrix_t A, const double *y, double *x) { return matvec_entry<double>(A, y, x, 2); }

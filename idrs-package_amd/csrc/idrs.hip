@@ -608,7 +608,10 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
                hipStream_t stream) {
     if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..64");
     if (n < 1) throw std::invalid_argument("empty system");
-    if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
+    if (A.rows() != n || A.K->nrows != A.K->ncols) throw std::invalid_argument("matrix is not n x n");
+    if (o.arena && A.general()) throw std::invalid_argument("the allocation-free solve takes standard operators only");
+    if (o.has_m1 && o.M1 && o.M1->general())
+        throw std::invalid_argument("M1 with mass, damping, block or deflation is not offered (v/M1 divides by a diagonal)");
     if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
     const int method = o.variant;
     if (method < 1 || method > 4) throw std::invalid_argument("variant must be 1..4");

@@ -718,7 +718,7 @@ template <class T>
 void check_common(const Operator &A, long n, int s, double tol) {
     if (s < 1 || s > SMAX) throw std::invalid_argument("s must be in 1..64");
     if (n < 1) throw std::invalid_argument("empty system");
-    if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
+    if (A.rows() != n || A.K->nrows != A.K->ncols) throw std::invalid_argument("matrix is not n x n");
     if (tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
 }
 

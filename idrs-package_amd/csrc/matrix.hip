@@ -25,6 +25,7 @@
 // - The operator's follow-up work (Horner term, shadow-space dots, omega
 //   dots, norms) is fused into the product as an epilogue.
 #include "matrix.hpp"
+#include "general.hpp"
 #include "spmv_epi.hpp"
 #include "ritz.hpp"
 
@@ -77,6 +78,10 @@ Operator::~Operator() {
     (void)hipFree(sval32);
     (void)hipFree(sl_sval);
     (void)hipFree(sl_sval32);
+    (void)hipFree(dV);
+    (void)hipFree(dAV);
+    (void)hipFree(dVtAV);
+    (void)hipFree(gws);
 }
 
 SpmvTimer &spmv_timer() {
@@ -1055,9 +1060,21 @@ static T coef(const Operator &op, int i) {
     return to_T<T>(op.alpha[i]);
 }
 
+// double and complex<double> vectors: the general path's types
+template <class T>
+constexpr bool gen_type() {
+    return std::is_same<T, double>::value || std::is_same<T, c128>::value;
+}
+
 template <class T>
 void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const EpiArgs<T> &E,
               const int *info, hipStream_t st, const TileCtx *tc) {
+    if (op.general()) {
+        if (tc) throw std::invalid_argument("batched solves take standard operators only");
+        if constexpr (gen_type<T>()) gen_apply<T>(op, v, w, epi, E, info, st);
+        else throw std::invalid_argument("mass, damping, block and deflated operators run in double precision");
+        return;
+    }
     const int d = op.degree;
     EpiArgs<T> none{};
     timer_mark(st);
@@ -1079,6 +1096,12 @@ void op_apply(const Operator &op, const T *v, T *w, T *w1, T *w2, int epi, const
 
 template <class T>
 void op_apply_cols(const Operator &op, int nc, long ld, const T *V, T *W, T *W1, T *W2, hipStream_t st) {
+    if (op.general()) {
+        EpiArgs<T> none{};
+        for (int j = 0; j < nc; j++)
+            op_apply<T>(op, V + (long)j * ld, W + (long)j * ld, W1, W2, EPI_NONE, none, nullptr, st);
+        return;
+    }
     const int d = op.degree;
     if (d == 0) {
         op_spmv_cols<T>(op, nc, ld, V, T(1), nullptr, nullptr, W, st);
@@ -1333,6 +1356,11 @@ __global__ void k_scale_copy(long n, T c, const T *__restrict__ v, T *__restrict
 
 template <class T>
 void op_precon(const Operator &op, const T *v, T *w, hipStream_t st) {
+    if (op.general()) {
+        if constexpr (gen_type<T>()) gen_precon<T>(op, v, w, st);
+        else throw std::invalid_argument("mass, damping, block and deflated operators run in double precision");
+        return;
+    }
     long n = op.K->nrows;
     int g = vec_grid(n);
     if (op.which_precon == 1) {
@@ -1366,6 +1394,13 @@ void op_precon(const Operator &op, const T *v, T *w, hipStream_t st) {
 template <class T>
 void op_scaleback_with(const Operator &op, const std::vector<std::complex<double>> &alpha, const T *y, T *x, T *w1,
                        T *w2, hipStream_t st) {
+    if (op.general()) {
+        // without deflation (its step needs b and the shift: gen_ms_scaleback_col)
+        if (op.ndefl > 0) throw std::invalid_argument("the scale-back of a deflated operator needs b");
+        if constexpr (gen_type<T>()) gen_ms_scaleback_col<T>(op, alpha, op.seed, nullptr, y, x, st);
+        else throw std::invalid_argument("mass, damping, block and deflated operators run in double precision");
+        return;
+    }
     const int d = (int)alpha.size() - 1;
     long n = op.K->nrows;
     EpiArgs<T> none{};
@@ -1389,6 +1424,10 @@ template <class T>
 void op_scaleback_cols(const Operator &op, int nc, long ld, const std::vector<std::vector<std::complex<double>>> &alpha,
                        const T *Y, T *X, T *W1, T *W2, hipStream_t st) {
     if ((int)alpha.size() < nc) throw std::invalid_argument("one coefficient set per column");
+    if (op.general()) {
+        for (int j = 0; j < nc; j++) op_scaleback_with<T>(op, alpha[j], Y + (long)j * ld, X + (long)j * ld, W1, W2, st);
+        return;
+    }
     const int d = nc > 0 ? (int)alpha[0].size() - 1 : 0;
     for (int j = 0; j < nc; j++)
         if ((int)alpha[j].size() != d + 1) throw std::invalid_argument("coefficient sets of unequal degree");
@@ -1491,13 +1530,16 @@ void build_sell(DevMatrix &K, hipStream_t st) {
 
 bool op_prepared(const Operator &op) {
     const DevMatrix &K = *op.K;
-    return K.fmt != DevFmt::CRS || !K.sell || !op.sval || op.sl_sval;
+    const bool sliced = K.fmt != DevFmt::CRS || !K.sell || !op.sval || op.sl_sval;
+    return sliced && (!op.general() || (op.gws && op.gws_rows == op.rows()));
 }
 
 void op_prepare(const Operator &op, hipStream_t st) {
     if (op_prepared(op)) return;
     const DevMatrix &K = *op.K;
-    op.sl_sval = sliced_copy(*K.sell, op.sval, op.values_complex(), st);
+    if (!(K.fmt != DevFmt::CRS || !K.sell || !op.sval || op.sl_sval))
+        op.sl_sval = sliced_copy(*K.sell, op.sval, op.values_complex(), st);
+    gen_prepare(op);
     HIP_TRY(hipStreamSynchronize(st));
 }
 
@@ -1701,6 +1743,8 @@ long Operator::stored_values() const {
 // D = diag(K) [- shift] with zeros already replaced by 1 in diag(K).  The
 // values of K D^-1 are precomputed so that real_mv's t = v/D is folded into
 // the product.
+static void op_scale_values(Operator &op, bool cdiag, hipStream_t st);
+
 void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shift) {
     hipStream_t st = lib_stream();
     DevMatrix &K = *op.K;
@@ -1740,7 +1784,12 @@ void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shi
             LAUNCH_CHECK();
         }
     }
-    // scaled values
+    op_scale_values(op, cdiag, st);
+}
+
+// K D^-1 (column-scaled values) for the fast path, from op.pdiag
+static void op_scale_values(Operator &op, bool cdiag, hipStream_t st) {
+    DevMatrix &K = *op.K;
     long cnt = K.fmt == DevFmt::CRS ? K.nnz : (long)K.ndiag * K.nrows;
     int gv = vec_grid(cnt);
     bool cvals = (K.kind == ValKind::Complex) || cdiag;
@@ -1773,6 +1822,36 @@ void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shi
     (void)hipFree(tmpc);
     if (K.fmt != DevFmt::CRS) drop_unit_diagonal(op, cvals, st);
     else op_prepare(op, st);
+}
+
+// A%P given by its diagonal (REAL_PRECON / COMPLEX_PRECON(1, K, M, C,
+// shift) of other matrices, user_module.f90:1327-1381, 1465-1530)
+void op_set_diag_values(Operator &op, const std::vector<std::complex<double>> &d, bool cdiag) {
+    hipStream_t st = lib_stream();
+    const long n = op.K->nrows;
+    if ((long)d.size() != n) throw std::invalid_argument("preconditioner size does not match the system");
+    (void)hipFree(op.pdiag);
+    (void)hipFree(op.sval);
+    (void)hipFree(op.sval32);
+    (void)hipFree(op.sl_sval);
+    (void)hipFree(op.sl_sval32);
+    op.pdiag = op.sval = op.sval32 = nullptr;
+    op.sl_sval = op.sl_sval32 = nullptr;
+    op.unit_diag = -1;
+    op.which_precon = 1;
+    op.pdiag_kind = cdiag ? ValKind::Complex : ValKind::Real;
+    if (cdiag) {
+        std::vector<c128> h(n);
+        for (long i = 0; i < n; i++) h[i] = c128(d[i].real(), d[i].imag());
+        HIP_TRY(hipMalloc(&op.pdiag, sizeof(c128) * (n ? n : 1)));
+        HIP_TRY(hipMemcpy(op.pdiag, h.data(), sizeof(c128) * n, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> h(n);
+        for (long i = 0; i < n; i++) h[i] = d[i].real();
+        HIP_TRY(hipMalloc(&op.pdiag, sizeof(double) * (n ? n : 1)));
+        HIP_TRY(hipMemcpy(op.pdiag, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    op_scale_values(op, cdiag, st);
 }
 
 void op_make_lowprec(Operator &op) {

@@ -124,6 +124,19 @@ struct Operator {
     // products of a double-vector solve read the FP32 copy of the (scaled)
     // values (set by the mixed-precision driver around its inner solves)
     bool values32 = false;
+    // the generalized operator (matrix_module.f90:169-264, 306-400,
+    // 1284-1623): any of these sends products through general.hip
+    const DevMatrix *Mm = nullptr;   // A%M (mass_matrix), not owned
+    const DevMatrix *Cm = nullptr;   // A%C (damping_matrix), not owned
+    bool block = false;              // A%block_system: vectors of 2 K%nrows
+    bool multishift = false;         // A%multishift: deflation projector I - V V^H
+    int ndefl = 0;                   // A%n_deflation
+    bool defl_complex = false;       // dV, dAV, dVtAV hold c128 (else double)
+    void *dV = nullptr, *dAV = nullptr, *dVtAV = nullptr;   // rows x nd, rows x nd, nd x nd (device)
+    mutable void *gws = nullptr;     // workspace of the general path (gen_prepare)
+    mutable long gws_rows = 0;
+    bool general() const { return Mm || Cm || block || ndefl > 0; }
+    long rows() const { return block ? 2L * K->nrows : K->nrows; }
     ~Operator();
     const void *values() const { return sval ? sval : K->val; }
     // number of value entries the SpMV reads (sval, sval32 or K's)
@@ -250,6 +263,8 @@ void pattern_grad(const DevMatrix &K, const T *lambda, const T *x, T *grad, hipS
 
 // operator construction
 void op_set_diag_precon(Operator &op, int which, const std::complex<double> *shift);
+// A%P = a diagonal given on the host (which_precon 1)
+void op_set_diag_values(Operator &op, const std::vector<std::complex<double>> &d, bool cdiag);
 void op_set_poly(Operator &op, int kind, int degree, const std::complex<double> *foci,
                  const std::complex<double> *center, const std::complex<double> *H, int ldh, int ncol,
                  bool complex_poly);

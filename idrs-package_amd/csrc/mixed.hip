@@ -112,6 +112,7 @@ void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const S
     using T32 = typename Lo<T64>::type;
     if (n < 1) throw std::invalid_argument("empty system");
     if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
+    if (A.general()) throw std::invalid_argument("mixed precision takes standard operators only");
     if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
     if (!(inner_tol > 0 && inner_tol < 1)) throw std::invalid_argument("inner tolerance must be in (0, 1)");
     if (max_outer < 1) throw std::invalid_argument("at least one outer step");

@@ -150,6 +150,23 @@ _PROTOS = {
                                     POINTER(idrs_stats_t), c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int,
                                     POINTER(c_int), POINTER(c_int)]),
     "idrs_qr_eig": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, POINTER(c_int)]),
+    "idrs_matrix_set_mass": (c_int, [c_void_p, c_void_p]),
+    "idrs_matrix_set_damping": (c_int, [c_void_p, c_void_p]),
+    "idrs_matrix_set_flags": (c_int, [c_void_p, c_int, c_int]),
+    "idrs_matrix_rows": (c_int, [c_void_p, POINTER(c_int)]),
+    "idrs_matrix_set_precon_diag": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "idrs_make_deflation_f64": (c_int, [c_void_p, c_int, c_void_p]),
+    "idrs_make_deflation_c128": (c_int, [c_void_p, c_int, c_void_p]),
+    "idrs_pmul_f64": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "idrs_pmul_c128": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "idrs_system_mv_f64": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "idrs_system_mv_c128": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "idrs_add_deflation_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "idrs_add_deflation_c128": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "idrs_scaleback_b_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "idrs_scaleback_b_c128": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "idrs_ms_scaleback_b_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "idrs_ms_scaleback_b_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "idrs_filter_ritzval": (c_int, [c_int, c_void_p, c_void_p]),
     "idrs_analyse_spectrum": (c_int, [c_int, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "idrs_si_power": (c_int, [c_int, c_void_p, c_int, c_int, c_int, c_void_p, c_uint64, c_uint64, c_void_p]),

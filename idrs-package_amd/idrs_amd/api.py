@@ -304,6 +304,77 @@ class Matrix:
         check(f(self._h, _ptr(v), _ptr(w)))
         return w
 
+    # --- generalized operators (PP_IDRS/real_matrix.inc, complex_matrix.inc) ---
+    def set_mass(self, M):
+        """A%M = M (A%mass_matrix): products K - seed M, or with C,
+        K + seed C + seed^2 M (matrix_module.f90:203-214)."""
+        self._M = M
+        check(_lib.load().idrs_matrix_set_mass(self._h, M.handle if M is not None else None))
+
+    def set_damping(self, C):
+        """A%C = C (A%damping_matrix): K + seed C + seed^2 (M or I)."""
+        self._C = C
+        check(_lib.load().idrs_matrix_set_damping(self._h, C.handle if C is not None else None))
+
+    def set_flags(self, block_system=False, multishift=False):
+        """A%block_system (vectors of 2 K%nrows: [-C -K; I 0] - seed [M 0; 0 I])
+        and A%multishift (deflation projector I - V V^H)."""
+        check(_lib.load().idrs_matrix_set_flags(self._h, int(bool(block_system)), int(bool(multishift))))
+
+    @property
+    def rows(self):
+        r = ctypes.c_int()
+        check(_lib.load().idrs_matrix_rows(self._h, ctypes.byref(r)))
+        return r.value
+
+    def set_precon_diag(self, PK, PM=None, PC=None, shift=None):
+        """A%P = REAL_PRECON / COMPLEX_PRECON(1, PK, PM, PC, shift)
+        (user_module.f90:1327-1381): e.g. PK = M, a lumped mass matrix."""
+        self._P = (PK, PM, PC)
+        cplx = self.is_complex or np.iscomplexobj(shift)
+        s = None
+        if shift is not None:
+            s = np.array([np.real(shift), np.imag(shift)]) if cplx else np.array([float(shift)])
+        check(_lib.load().idrs_matrix_set_precon_diag(self._h, PK.handle, PM.handle if PM is not None else None,
+                                                      PC.handle if PC is not None else None, _ptr(s), int(cplx)))
+
+    def _cpl(self, *arrs):
+        return self.is_complex or any(np.iscomplexobj(a) for a in arrs)
+
+    def make_deflation(self, V):
+        """call make_deflation(A, V) (matrix_module.f90:1284-1352)."""
+        cplx = self._cpl(V)
+        V = np.asfortranarray(V, dtype=np.complex128 if cplx else np.float64)
+        if V.ndim == 1:
+            V = V.reshape(-1, 1, order="F")
+        f = _lib.load().idrs_make_deflation_c128 if cplx else _lib.load().idrs_make_deflation_f64
+        check(f(self._h, V.shape[1], _ptr(V)))
+
+    def _gen(self, name, v, b=None):
+        cplx = self._cpl(v, b if b is not None else 0.0)
+        dt = np.complex128 if cplx else np.float64
+        v = np.ascontiguousarray(v, dtype=dt)
+        w = np.empty_like(v)
+        f = getattr(_lib.load(), f"idrs_{name}_{'c128' if cplx else 'f64'}")
+        if b is None:
+            check(f(self._h, _ptr(v), _ptr(w)))
+        else:
+            bb = np.ascontiguousarray(b, dtype=dt)
+            check(f(self._h, _ptr(bb), _ptr(v), _ptr(w)))
+        return w
+
+    def pmul(self, v):
+        """Pmul(A, v) (matrix_module.f90:1369-1431)."""
+        return self._gen("pmul", v)
+
+    def system_mv(self, v):
+        """real_mv(A, v) / complex_mv(A, v) alone (no polynomial, no Pmul)."""
+        return self._gen("system_mv", v)
+
+    def add_deflation(self, b, y):
+        """add_deflation(A, b, y) (matrix_module.f90:1433-1506)."""
+        return self._gen("add_deflation", y, b)
+
     def __rtruediv__(self, v):
         """w = v/A (rprecon / cprecon, matrix_module.f90:306-400)."""
         cplx = self.is_complex or np.iscomplexobj(v)
@@ -338,14 +409,41 @@ def complex_preconditioner(A, preconditioner=0, degree=0, foci=None, center=None
 
 
 def scaleback(A, b, y):
-    """x = scaleback(A, b, y) = polpre(y) / D (matrix_module.f90:808-849)."""
-    cplx = A.is_complex or np.iscomplexobj(y)
+    """x = scaleback(A, b, y) = add_deflation(A, b, polpre(y)) / A
+    (matrix_module.f90:808-849)."""
+    cplx = A.is_complex or np.iscomplexobj(y) or np.iscomplexobj(b)
     dt = np.complex128 if cplx else np.float64
     y = np.ascontiguousarray(y, dtype=dt)
+    b = np.ascontiguousarray(b, dtype=dt)
     x = np.empty_like(y)
-    f = _lib.load().idrs_scaleback_c128 if cplx else _lib.load().idrs_scaleback_f64
-    check(f(A.handle, _ptr(y), _ptr(x)))
+    f = _lib.load().idrs_scaleback_b_c128 if cplx else _lib.load().idrs_scaleback_b_f64
+    check(f(A.handle, _ptr(b), _ptr(y), _ptr(x)))
     return x
+
+
+def ms_scaleback_b(A, sigma, alpha_s, b, Y):
+    """X = scaleback(A, sigma, b, Y) (ms_rscaleback / ms_cscaleback,
+    matrix_module.f90:724-806, without shift-and-invert): per shift the
+    polynomial alpha_s(:, k) (from ms_parameters), the deflation step for
+    sigma_k, then /A."""
+    sig = np.atleast_1d(sigma)
+    cplx = A.is_complex or np.iscomplexobj(sig) or np.iscomplexobj(Y) or np.iscomplexobj(b)
+    dt = np.complex128 if cplx else np.float64
+    Y = np.asfortranarray(Y, dtype=dt)
+    X = np.empty_like(Y, order="F")
+    b = np.ascontiguousarray(b, dtype=dt)
+    a = np.asfortranarray(alpha_s, dtype=np.complex128)      # (degree+1, n_sigma)
+    deg = a.shape[0] - 1
+    ap = np.empty(2 * a.size)
+    ap[0::2], ap[1::2] = a.ravel(order="F").real, a.ravel(order="F").imag
+    if cplx:
+        sp = np.empty(2 * len(sig))
+        sp[0::2], sp[1::2] = np.real(sig), np.imag(sig)
+    else:
+        sp = np.ascontiguousarray(sig, dtype=np.float64)
+    f = _lib.load().idrs_ms_scaleback_b_c128 if cplx else _lib.load().idrs_ms_scaleback_b_f64
+    check(f(A.handle, len(sig), _ptr(sp), deg, _ptr(ap), _ptr(b), _ptr(Y), _ptr(X)))
+    return X
 
 
 def make_p(n, s, method=1, PT_raw=None, r=None, seed=1, complex=False):

@@ -177,6 +177,55 @@ int idrs_neumann_coeffs(int degree, const double *center, int is_complex, double
 int idrs_compute_alpha_coeffs(int degree, const double *H, int ldh, int ncol, int is_complex, double *alpha);
 /* A%ralpha_p / A%calpha_p(0:degree) as (re, im) pairs; *degree out */
 int idrs_matrix_get_alpha(idrs_matrix_t A, double *alpha, int maxlen, int *degree);
+/* Generalized operators (PP_IDRS/real_matrix.inc, complex_matrix.inc set
+   these components of type(matrix)).  Any of them sends A's products
+   through the reference's general statements (matrix_module.f90:169-264):
+   t = v/P, then K t with the seed terms K + seed C + seed^2 M (damping and
+   mass), K + seed C + seed^2 I (damping), K - seed M (mass), or the block
+   system [-C -K; I 0] - seed [M 0; 0 I] on vectors of 2 K%nrows.  Such
+   operators run in double precision; M and C stay owned by the caller. */
+/* A%M = M, A%mass_matrix = .true. (M NULL: none) */
+int idrs_matrix_set_mass(idrs_matrix_t A, idrs_user_matrix_t M);
+/* A%C = C, A%damping_matrix = .true. (C NULL: none) */
+int idrs_matrix_set_damping(idrs_matrix_t A, idrs_user_matrix_t C);
+/* A%block_system, A%multishift (the latter selects the deflation
+   projector I - V V^H, matrix_module.f90:1376-1383) */
+int idrs_matrix_set_flags(idrs_matrix_t A, int block_system, int multishift);
+/* rows of A's vectors: 2 K%nrows for a block system, else K%nrows */
+int idrs_matrix_rows(idrs_matrix_t A, int *rows);
+/* A%P = REAL_PRECON / COMPLEX_PRECON(1, PK, PM, PC, shift)
+   (user_module.f90:1327-1381, 1465-1530): v/A divides by
+   diag(PK) [+ shift diag(PC) + shift^2 diag(PM) | + shift diag(PC) + shift^2
+   | - shift diag(PM) | - shift]; PM, PC, shift may be NULL; shift as one
+   value or an (re, im) pair when is_complex.  E.g. a lumped mass matrix:
+   PK = M. */
+int idrs_matrix_set_precon_diag(idrs_matrix_t A, idrs_user_matrix_t PK, idrs_user_matrix_t PM, idrs_user_matrix_t PC,
+                                const double *shift, int is_complex);
+/* make_deflation(A, V), matrix_module.f90:1284-1352: V(rows, nd) host,
+   nd <= 16; AV = real_mv(A, V) and V^H AV are formed on the device */
+int idrs_make_deflation_f64(idrs_matrix_t A, int nd, const double *V);
+int idrs_make_deflation_c128(idrs_matrix_t A, int nd, const double *V);
+/* w = Pmul(A, v), matrix_module.f90:1369-1431 (the real version subtracts
+   matmul(A%rV, t), the complex one matmul(A%cAV, t), as written) */
+int idrs_pmul_f64(idrs_matrix_t A, const double *v, double *w);
+int idrs_pmul_c128(idrs_matrix_t A, const double *v, double *w);
+/* w = real_mv(A, v) / complex_mv(A, v) alone (matrix_module.f90:169-264) */
+int idrs_system_mv_f64(idrs_matrix_t A, const double *v, double *w);
+int idrs_system_mv_c128(idrs_matrix_t A, const double *v, double *w);
+/* x = add_deflation(A, b, y), matrix_module.f90:1433-1506 */
+int idrs_add_deflation_f64(idrs_matrix_t A, const double *b, const double *y, double *x);
+int idrs_add_deflation_c128(idrs_matrix_t A, const double *b, const double *y, double *x);
+/* x = scaleback(A, b, y) with its deflation step, matrix_module.f90:808-849 */
+int idrs_scaleback_b_f64(idrs_matrix_t A, const double *b, const double *y, double *x);
+int idrs_scaleback_b_c128(idrs_matrix_t A, const double *b, const double *y, double *x);
+/* X = scaleback(A, sigma, b, Y), matrix_module.f90:724-806 without
+   shift-and-invert: per shift k, p_k(A) y_k (alpha_s as for
+   idrs_ms_scaleback_*), ms_add_deflation for sigma_k (:1508-1623), x/A;
+   sigma(n_sigma) real, or (re, im) pairs for c128; host arrays */
+int idrs_ms_scaleback_b_f64(idrs_matrix_t A, int n_sigma, const double *sigma, int degree, const double *alpha_s,
+                            const double *b, const double *Y, double *X);
+int idrs_ms_scaleback_b_c128(idrs_matrix_t A, int n_sigma, const double *sigma, int degree, const double *alpha_s,
+                             const double *b, const double *Y, double *X);
 /* w = A*v: rmatvec, matrix_module.f90:152-167 */
 int idrs_matvec_f64(idrs_matrix_t A, const double *v, double *w);
 /* w = A*v: cmatvec, matrix_module.f90:135-150 */

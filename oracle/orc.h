@@ -53,9 +53,9 @@ typedef struct {
     const int *offsets;
 } orc_user_matrix;
 
-/* type(matrix), PP_MATRIX/matrix_module.f90:38-81, restricted to what the hot
-   path uses: K, diagonal preconditioner P (which_precon 0/1), seed shift,
-   polynomial coefficients. */
+/* type(matrix), PP_MATRIX/matrix_module.f90:38-81: K, the diagonal
+   preconditioner P (which_precon 0/1), seed shift, polynomial coefficients,
+   mass and damping matrices, block systems and deflation. */
 typedef struct {
     orc_user_matrix K;
     int which_precon;          /* 0 identity, 1 diagonal scaling            */
@@ -66,6 +66,14 @@ typedef struct {
     int degree;                /* A%degree                                  */
     const double *ralpha_p;    /* A%ralpha_p(0:degree)                      */
     const orc_cplx *calpha_p;  /* A%calpha_p(0:degree)                      */
+    const orc_user_matrix *M;  /* A%M when A%mass_matrix, else NULL         */
+    const orc_user_matrix *C;  /* A%C when A%damping_matrix, else NULL      */
+    int block_system;          /* A%block_system: vectors of 2 K%nrows      */
+    int multishift;            /* A%multishift (deflation projector I - VV^H) */
+    int n_deflation;           /* A%n_deflation                             */
+    void *V, *AV, *VtAV;       /* A%rV/cV (n,nd), A%rAV/cAV (n,nd), A%rVtAV/cVtAV
+                                  (nd,nd), T column-major; AV and VtAV are
+                                  filled by make_deflation                  */
 } orc_matrix;
 
 /* optional arguments of IDRS (idrs_module.f90:180-251).  has_* flags mirror
@@ -117,6 +125,30 @@ void orc_r_precon(const orc_matrix *A, const double *v, double *w);
 void orc_c_precon(const orc_matrix *A, const orc_cplx *v, orc_cplx *w);
 void orc_r_scaleback(const orc_matrix *A, const double *y, double *x);
 void orc_c_scaleback(const orc_matrix *A, const orc_cplx *y, orc_cplx *x);
+/* rows of the operator's vectors (2 K%nrows for a block system) */
+int  orc_r_rows(const orc_matrix *A);
+int  orc_c_rows(const orc_matrix *A);
+/* real_mv / complex_mv alone (matrix_module.f90:169-264) */
+void orc_r_system_mv(const orc_matrix *A, const double *v, double *w);
+void orc_c_system_mv(const orc_matrix *A, const orc_cplx *v, orc_cplx *w);
+/* Pmul in place (matrix_module.f90:1369-1431) */
+void orc_r_pmul(const orc_matrix *A, double *w);
+void orc_c_pmul(const orc_matrix *A, orc_cplx *w);
+/* make_deflation (matrix_module.f90:1284-1352): fills A->AV, A->VtAV from A->V */
+void orc_r_make_deflation(orc_matrix *A);
+void orc_c_make_deflation(orc_matrix *A);
+/* add_deflation (matrix_module.f90:1433-1506) */
+void orc_r_add_deflation(const orc_matrix *A, const double *b, const double *y, double *x);
+void orc_c_add_deflation(const orc_matrix *A, const orc_cplx *b, const orc_cplx *y, orc_cplx *x);
+/* rms_add_deflation / cms_add_deflation (matrix_module.f90:1508-1623) for
+   shifts sigma (n_sigma), Y and X (rows, n_sigma) column-major */
+void orc_r_ms_add_deflation(const orc_matrix *A, int n_sigma, const double *sigma, const double *b,
+                            const double *Y, double *X);
+void orc_c_ms_add_deflation(const orc_matrix *A, int n_sigma, const orc_cplx *sigma, const orc_cplx *b,
+                            const orc_cplx *Y, orc_cplx *X);
+/* rscaleback / cscaleback with the deflation step (matrix_module.f90:808-849) */
+void orc_r_scaleback_b(const orc_matrix *A, const double *b, const double *y, double *x);
+void orc_c_scaleback_b(const orc_matrix *A, const orc_cplx *b, const orc_cplx *y, orc_cplx *x);
 int  orc_r_compute_alpha(int degree, const double *H, int ldh, int ncol, double *alpha);
 int  orc_c_compute_alpha(int degree, const orc_cplx *H, int ldh, int ncol, orc_cplx *alpha);
 void orc_r_chebyshev(int degree, const double foci[2], double *alpha);

@@ -39,6 +39,10 @@ class Matrix(ctypes.Structure):
         ("rseed", c_double), ("cseed", c_double * 2),
         ("degree", c_int),
         ("ralpha_p", c_void_p), ("calpha_p", c_void_p),
+        ("M", c_void_p), ("C", c_void_p),
+        ("block_system", c_int), ("multishift", c_int),
+        ("n_deflation", c_int),
+        ("V", c_void_p), ("AV", c_void_p), ("VtAV", c_void_p),
     ]
 
 
@@ -118,7 +122,11 @@ class OracleMatrix:
     """Holds numpy arrays alive behind an orc_matrix."""
 
     def __init__(self, fmt, n, ncols=None, *, crs=None, coo=None, cds=None, dia=None, which_precon=0,
-                 degree=0, alpha=None, seed=0.0, complex_vectors=False):
+                 degree=0, alpha=None, seed=0.0, complex_vectors=False, mass=None, damping=None, block=False,
+                 multishift=False, precon_diag=None):
+        """mass, damping: OracleMatrix objects whose K is A%M / A%C;
+        precon_diag: P%rband(:,0) given directly (e.g. a lumped mass
+        matrix's diagonal, real_precon(1, M)) instead of K's diagonal."""
         self.keep = []
         self.complex = complex_vectors
         um = UserMatrix()
@@ -157,7 +165,12 @@ class OracleMatrix:
         m.K = um
         m.which_precon = which_precon
         lib = load()
-        if which_precon == 1:
+        if which_precon == 1 and precon_diag is not None:
+            if complex_vectors:
+                m.cdiag = hold(np.asarray(precon_diag, dtype=np.complex128), np.complex128)
+            else:
+                m.rdiag = hold(np.asarray(precon_diag, dtype=np.float64), np.float64)
+        elif which_precon == 1:
             if complex_vectors:
                 d = np.empty(n, dtype=np.complex128)
                 lib.orc_c_diagonal(ctypes.byref(um), c_void_p(d.ctypes.data))
@@ -176,6 +189,14 @@ class OracleMatrix:
                 m.calpha_p = hold(np.asarray(alpha, dtype=np.complex128), np.complex128)
             else:
                 m.ralpha_p = hold(np.asarray(alpha, dtype=np.float64), np.float64)
+        if mass is not None:
+            self.keep.append(mass)
+            m.M = ctypes.addressof(mass.um)
+        if damping is not None:
+            self.keep.append(damping)
+            m.C = ctypes.addressof(damping.um)
+        m.block_system = 1 if block else 0
+        m.multishift = 1 if multishift else 0
         self.m = m
         self.n = n
 
@@ -195,6 +216,66 @@ class OracleMatrix:
         f = load().orc_c_precon if self.complex else load().orc_r_precon
         f(ctypes.byref(self.m), c_void_p(v.ctypes.data), c_void_p(w.ctypes.data))
         return w
+
+    @property
+    def rows(self):
+        return (load().orc_c_rows if self.complex else load().orc_r_rows)(ctypes.byref(self.m))
+
+    def _dt(self):
+        return np.complex128 if self.complex else np.float64
+
+    def _call(self, name, *arrays):
+        f = getattr(load(), ("orc_c_" if self.complex else "orc_r_") + name)
+        return f(ctypes.byref(self.m), *[c_void_p(a.ctypes.data) for a in arrays])
+
+    def system_mv(self, v):
+        """real_mv / complex_mv alone (no polynomial, no Pmul)."""
+        v = np.ascontiguousarray(v, dtype=self._dt())
+        w = np.empty(self.rows, dtype=self._dt())
+        self._call("system_mv", v, w)
+        return w
+
+    def pmul(self, v):
+        w = np.array(v, dtype=self._dt())
+        self._call("pmul", w)
+        return w
+
+    def make_deflation(self, V):
+        """make_deflation(A, V): A%V = V, A%AV = real_mv(A, V), A%VtAV."""
+        V = np.asfortranarray(V, dtype=self._dt())
+        nd = V.shape[1]
+        self._V, self._AV = V, np.zeros_like(V, order="F")
+        self._VtAV = np.zeros((nd, nd), dtype=self._dt(), order="F")
+        self.m.n_deflation = nd
+        self.m.V, self.m.AV, self.m.VtAV = self._V.ctypes.data, self._AV.ctypes.data, self._VtAV.ctypes.data
+        f = getattr(load(), ("orc_c_" if self.complex else "orc_r_") + "make_deflation")
+        f(ctypes.byref(self.m))
+        return self._AV, self._VtAV
+
+    def add_deflation(self, b, y):
+        b = np.ascontiguousarray(b, dtype=self._dt())
+        y = np.ascontiguousarray(y, dtype=self._dt())
+        x = np.empty_like(y)
+        self._call("add_deflation", b, y, x)
+        return x
+
+    def ms_add_deflation(self, sigma, b, Y):
+        sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=self._dt())
+        b = np.ascontiguousarray(b, dtype=self._dt())
+        Y = np.asfortranarray(Y, dtype=self._dt())
+        X = np.empty_like(Y, order="F")
+        f = getattr(load(), ("orc_c_" if self.complex else "orc_r_") + "ms_add_deflation")
+        f(ctypes.byref(self.m), c_int(len(sig)), c_void_p(sig.ctypes.data), c_void_p(b.ctypes.data),
+          c_void_p(Y.ctypes.data), c_void_p(X.ctypes.data))
+        return X
+
+    def scaleback_b(self, b, y):
+        """rscaleback / cscaleback with the deflation step."""
+        b = np.ascontiguousarray(b, dtype=self._dt())
+        y = np.ascontiguousarray(y, dtype=self._dt())
+        x = np.empty_like(y)
+        self._call("scaleback_b", b, y, x)
+        return x
 
     def user_mv(self, v):
         dt = np.complex128 if self.complex else np.float64
