@@ -1155,6 +1155,38 @@ static int ms_scaleback_b_entry(idrs_matrix_t A, int ns, const double *sigma, in
     });
 }
 }  // extern "C++"
+// X = add_deflation(A, sigma, b, Y) alone (rms_add_deflation /
+// cms_add_deflation, matrix_module.f90:1508-1623); Y copied without deflation
+extern "C++" {
+template <class T>
+static int ms_add_deflation_entry(idrs_matrix_t A, int ns, const double *sigma, const double *b, const double *Y,
+                                  double *X) {
+    return guarded([&] {
+        need(A && sigma && b && Y && X, "null argument");
+        need(ns >= 1, "bad arguments");
+        hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
+        gen_prepare(A->op);
+        const long n = A->op.rows();
+        std::vector<std::complex<double>> sg(ns);
+        for (int k = 0; k < ns; k++)
+            sg[k] = Tr<T>::is_complex ? std::complex<double>(sigma[2 * k], sigma[2 * k + 1])
+                                      : std::complex<double>(sigma[k], 0.0);
+        DVec<T> db((const T *)b, n, st), dy((const T *)Y, (size_t)n * ns, st), dx((size_t)n * ns);
+        if (A->op.general()) gen_ms_add_deflation<T>(A->op, ns, sg.data(), db.p, dy.p, dx.p, st);
+        else HIP_TRY(hipMemcpyAsync(dx.p, dy.p, sizeof(T) * n * ns, hipMemcpyDeviceToDevice, st));
+        dx.download((T *)X, st);
+    });
+}
+}  // extern "C++"
+int idrs_ms_add_deflation_f64(idrs_matrix_t A, int n_sigma, const double *sigma, const double *b, const double *Y,
+                              double *X) {
+    return ms_add_deflation_entry<double>(A, n_sigma, sigma, b, Y, X);
+}
+int idrs_ms_add_deflation_c128(idrs_matrix_t A, int n_sigma, const double *sigma, const double *b, const double *Y,
+                               double *X) {
+    return ms_add_deflation_entry<c128>(A, n_sigma, sigma, b, Y, X);
+}
 int idrs_ms_scaleback_b_f64(idrs_matrix_t A, int n_sigma, const double *sigma, int degree, const double *alpha_s,
                             const double *b, const double *Y, double *X) {
     return ms_scaleback_b_entry<double>(A, n_sigma, sigma, degree, alpha_s, b, Y, X);

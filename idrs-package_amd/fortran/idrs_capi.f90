@@ -340,6 +340,144 @@ module idrs_capi
          complex(c_double_complex), intent(out) :: x(*)
       end function
 
+! generalized operators: mass, damping, block systems, deflation
+      integer(c_int) function idrs_matrix_set_mass(A, M) bind(C, name="idrs_matrix_set_mass")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: A, M
+      end function
+
+      integer(c_int) function idrs_matrix_set_damping(A, C) bind(C, name="idrs_matrix_set_damping")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: A, C
+      end function
+
+      integer(c_int) function idrs_matrix_set_flags(A, block_system, multishift) bind(C, name="idrs_matrix_set_flags")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: A
+         integer(c_int), value :: block_system, multishift
+      end function
+
+      integer(c_int) function idrs_matrix_rows(A, rows) bind(C, name="idrs_matrix_rows")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: A
+         integer(c_int), intent(out) :: rows
+      end function
+
+      integer(c_int) function idrs_matrix_set_precon_diag(A, PK, PM, PC, shift, is_complex) &
+            bind(C, name="idrs_matrix_set_precon_diag")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: A, PK, PM, PC
+         real(c_double), intent(in), optional :: shift(*)
+         integer(c_int), value :: is_complex
+      end function
+
+      integer(c_int) function idrs_make_deflation_f64(A, nd, V) bind(C, name="idrs_make_deflation_f64")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: A
+         integer(c_int), value :: nd
+         real(c_double), intent(in) :: V(*)
+      end function
+
+      integer(c_int) function idrs_make_deflation_c128(A, nd, V) bind(C, name="idrs_make_deflation_c128")
+         import :: c_int, c_double_complex, c_ptr
+         type(c_ptr), value :: A
+         integer(c_int), value :: nd
+         complex(c_double_complex), intent(in) :: V(*)
+      end function
+
+      integer(c_int) function idrs_pmul_f64(A, v, w) bind(C, name="idrs_pmul_f64")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: A
+         real(c_double), intent(in) :: v(*)
+         real(c_double), intent(out) :: w(*)
+      end function
+
+      integer(c_int) function idrs_pmul_c128(A, v, w) bind(C, name="idrs_pmul_c128")
+         import :: c_int, c_double_complex, c_ptr
+         type(c_ptr), value :: A
+         complex(c_double_complex), intent(in) :: v(*)
+         complex(c_double_complex), intent(out) :: w(*)
+      end function
+
+      integer(c_int) function idrs_system_mv_f64(A, v, w) bind(C, name="idrs_system_mv_f64")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: A
+         real(c_double), intent(in) :: v(*)
+         real(c_double), intent(out) :: w(*)
+      end function
+
+      integer(c_int) function idrs_system_mv_c128(A, v, w) bind(C, name="idrs_system_mv_c128")
+         import :: c_int, c_double_complex, c_ptr
+         type(c_ptr), value :: A
+         complex(c_double_complex), intent(in) :: v(*)
+         complex(c_double_complex), intent(out) :: w(*)
+      end function
+
+      integer(c_int) function idrs_add_deflation_f64(A, b, y, x) bind(C, name="idrs_add_deflation_f64")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: A
+         real(c_double), intent(in) :: b(*), y(*)
+         real(c_double), intent(out) :: x(*)
+      end function
+
+      integer(c_int) function idrs_add_deflation_c128(A, b, y, x) bind(C, name="idrs_add_deflation_c128")
+         import :: c_int, c_double_complex, c_ptr
+         type(c_ptr), value :: A
+         complex(c_double_complex), intent(in) :: b(*), y(*)
+         complex(c_double_complex), intent(out) :: x(*)
+      end function
+
+      integer(c_int) function idrs_scaleback_b_f64(A, b, y, x) bind(C, name="idrs_scaleback_b_f64")
+         import :: c_int, c_double, c_ptr
+         type(c_ptr), value :: A
+         real(c_double), intent(in) :: b(*), y(*)
+         real(c_double), intent(out) :: x(*)
+      end function
+
+      integer(c_int) function idrs_scaleback_b_c128(A, b, y, x) bind(C, name="idrs_scaleback_b_c128")
+         import :: c_int, c_double_complex, c_ptr
+         type(c_ptr), value :: A
+         complex(c_double_complex), intent(in) :: b(*), y(*)
+         complex(c_double_complex), intent(out) :: x(*)
+      end function
+
+      integer(c_int) function idrs_ms_scaleback_b_f64(A, n_sigma, sigma, degree, alpha_s, b, Y, X) &
+            bind(C, name="idrs_ms_scaleback_b_f64")
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: A
+         integer(c_int), value :: n_sigma, degree
+         real(c_double), intent(in) :: sigma(*), b(*), Y(*)
+         complex(c_double_complex), intent(in) :: alpha_s(*)
+         real(c_double), intent(out) :: X(*)
+      end function
+
+      integer(c_int) function idrs_ms_scaleback_b_c128(A, n_sigma, sigma, degree, alpha_s, b, Y, X) &
+            bind(C, name="idrs_ms_scaleback_b_c128")
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: A
+         integer(c_int), value :: n_sigma, degree
+         complex(c_double_complex), intent(in) :: sigma(*), b(*), Y(*), alpha_s(*)
+         complex(c_double_complex), intent(out) :: X(*)
+      end function
+
+      integer(c_int) function idrs_ms_add_deflation_f64(A, n_sigma, sigma, b, Y, X) &
+            bind(C, name="idrs_ms_add_deflation_f64")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: A
+         integer(c_int), value :: n_sigma
+         real(c_double), intent(in) :: sigma(*), b(*), Y(*)
+         real(c_double), intent(out) :: X(*)
+      end function
+
+      integer(c_int) function idrs_ms_add_deflation_c128(A, n_sigma, sigma, b, Y, X) &
+            bind(C, name="idrs_ms_add_deflation_c128")
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: A
+         integer(c_int), value :: n_sigma
+         complex(c_double_complex), intent(in) :: sigma(*), b(*), Y(*)
+         complex(c_double_complex), intent(out) :: X(*)
+      end function
+
       integer(c_int) function idrs_idrs_f64(A, n, b, s, opts, x, stats) bind(C, name="idrs_idrs_f64")
          import :: c_int, c_double, c_ptr, idrs_opts_t, idrs_stats_t
          type(c_ptr), value :: A

@@ -14,12 +14,16 @@
 !   SCALEBACK     rscaleback / cscaleback    matrix_module.f90:808-849
 !                 ms_rscaleback / ms_cscaleback               :724-806
 !   MS_PARAMETERS real_ / complex_ms_parameters               :402-549
-!   Pmul          identity without deflation matrix_module.f90:1384-1431
+!   Pmul          deflation projectors        matrix_module.f90:1369-1431
+!   make_deflation / add_deflation            matrix_module.f90:1284-1623
+!   real_mv / complex_mv (K - sM, K + sC + s^2 M, block systems) :169-264
 !   filter_ritzval, ANALYSE_SPECTRUM         matrix_module.f90:1059-1282
 !                 (host routines of the library)
 !
-! Not offered (stop with a message): deflation, mass/damping matrices,
-! block systems, shift-and-invert.
+! Mass (A%M) and damping (A%C) matrices, block systems and deflation run on
+! the device operator; make_deflation also fills A%rAV/A%rVtAV (A%cAV,
+! A%cVtAV) as the reference does.  Not offered (stop with a message):
+! shift-and-invert.
 !
 module matrix_module
 
@@ -33,7 +37,7 @@ module matrix_module
    private
    public :: matrix, operator(*), operator(/), CHEBYSHEV, NEUMANN, SCALEBACK, COMPUTE_ALPHA, Pmul, &
              real_preconditioner, complex_preconditioner, matrix_handle, MS_PARAMETERS, &
-             filter_ritzval, ANALYSE_SPECTRUM
+             filter_ritzval, ANALYSE_SPECTRUM, make_deflation, add_deflation, real_mv, complex_mv
 
    type :: matrix
       type(user_matrix)              :: K, M, C, P
@@ -48,6 +52,11 @@ module matrix_module
       complex(kind=cp), allocatable  :: calpha_p(:)
       complex(kind=cp), allocatable  :: calpha_s(:,:)
       integer                        :: n_deflation = 0
+      real(kind=rp), allocatable     :: rV(:,:), rAV(:,:), rVtAV(:,:)
+      complex(kind=cp), allocatable  :: cV(:,:), cAV(:,:), cVtAV(:,:)
+! which make_deflation call V comes from (device operators are cached per
+! deflation space)
+      integer                        :: deflation_id = 0
       logical                        :: mass_matrix = .false.
       logical                        :: damping_matrix = .false.
       logical                        :: multishift = .false.
@@ -91,11 +100,23 @@ module matrix_module
       module procedure RANALYSE_SPECTRUM, CANALYSE_SPECTRUM
    end interface
 
+   interface make_deflation
+      module procedure rmake_deflation, cmake_deflation
+   end interface
+
+   interface add_deflation
+      module procedure radd_deflation, cadd_deflation, rms_add_deflation, cms_add_deflation
+   end interface
+
 ! Cache of device operators built from type(matrix) states
    integer, parameter :: NCACHE = 8
    type :: op_entry
       type(c_ptr)                   :: op = c_null_ptr
       type(c_ptr)                   :: k = c_null_ptr
+      type(c_ptr)                   :: m = c_null_ptr, c = c_null_ptr
+      type(c_ptr)                   :: pk = c_null_ptr, pm = c_null_ptr, pc = c_null_ptr
+      logical                       :: block = .false., multishift = .false.
+      integer                       :: defl_id = 0
       integer                       :: which_precon = 0
       logical                       :: has_shift = .false.
       complex(kind=cp)              :: shift = (0._cp,0._cp)
@@ -107,6 +128,7 @@ module matrix_module
    end type op_entry
    type(op_entry), save :: cache(NCACHE)
    integer, save        :: clock = 0
+   integer, save        :: deflation_count = 0
 
 contains
 
@@ -117,19 +139,33 @@ contains
       type(c_ptr)                       :: h
       complex(kind=cp)                  :: seed, shift
       complex(kind=cp), allocatable     :: alpha(:)
-      integer                           :: i, victim, deg
+      integer                           :: i, victim, deg, did
       integer(c_int)                    :: status
+      type(c_ptr)                       :: mh, ch, pk, pm, pc
+      logical                           :: own_diag
+      real(c_double)                    :: sh(2)
 
       if ( .not. c_associated(A%K%h) ) error stop 'matrix: the system matrix A%K is not set'
-      if ( A%n_deflation > 0 ) error stop 'matrix: deflation is not offered by the GPU library'
-      if ( A%block_system .or. A%shift_invert ) &
-         error stop 'matrix: block systems and shift-and-invert are not offered by the GPU library'
-      if ( (A%mass_matrix .or. A%damping_matrix) .and. (A%rseed /= 0._rp .or. A%cseed /= (0._cp,0._cp)) ) &
-         error stop 'matrix: mass and damping matrices are not offered by the GPU library'
+      if ( A%shift_invert ) error stop 'matrix: shift-and-invert is not offered by the GPU library'
+      mh = c_null_ptr
+      ch = c_null_ptr
+      if ( A%mass_matrix ) mh = A%M%h
+      if ( A%damping_matrix ) ch = A%C%h
+      did = 0
+      if ( A%n_deflation > 0 ) did = A%deflation_id
+      if ( A%n_deflation > 0 .and. did == 0 ) error stop 'matrix: n_deflation is set, call make_deflation'
+      pk = c_null_ptr
+      pm = c_null_ptr
+      pc = c_null_ptr
       if ( A%P%which_precon == 1 ) then
-         if ( .not. c_associated(A%P%h, A%K%h) ) &
-            error stop 'matrix: the diagonal preconditioner must be built from A%K'
+         pk = A%P%h
+         pm = A%P%hm
+         pc = A%P%hc
       end if
+! a diagonal preconditioner built from A%K alone folds into the matrix
+! values; any other (e.g. a lumped mass matrix) divides by its diagonal
+      own_diag = A%P%which_precon == 1 .and. c_associated(pk, A%K%h) .and. .not. c_associated(pm) &
+                 .and. .not. c_associated(pc)
 
       if ( cplx ) then
          seed = A%cseed
@@ -159,6 +195,10 @@ contains
       do i = 1, NCACHE
          if ( .not. c_associated(cache(i)%op) ) cycle
          if ( .not. c_associated(cache(i)%k, A%K%h) ) cycle
+         if ( .not. same(cache(i)%m, mh) .or. .not. same(cache(i)%c, ch) ) cycle
+         if ( .not. same(cache(i)%pk, pk) .or. .not. same(cache(i)%pm, pm) .or. .not. same(cache(i)%pc, pc) ) cycle
+         if ( (cache(i)%block .neqv. A%block_system) .or. (cache(i)%multishift .neqv. A%multishift) ) cycle
+         if ( cache(i)%defl_id /= did ) cycle
          if ( cache(i)%which_precon /= A%P%which_precon .or. (cache(i)%has_shift .neqv. A%P%has_shift) ) cycle
          if ( cache(i)%shift /= shift .or. cache(i)%seed /= seed .or. (cache(i)%cplx .neqv. cplx) ) cycle
          if ( cache(i)%degree /= deg ) cycle
@@ -181,7 +221,10 @@ contains
       else
          call idrs_check( idrs_matrix_set_seed_f64(h, real(seed, kind=rp)), 'matrix' )
       end if
-      if ( A%P%which_precon == 1 ) then
+      if ( c_associated(mh) ) call idrs_check( idrs_matrix_set_mass(h, mh), 'matrix' )
+      if ( c_associated(ch) ) call idrs_check( idrs_matrix_set_damping(h, ch), 'matrix' )
+      call idrs_check( idrs_matrix_set_flags(h, merge(1, 0, A%block_system), merge(1, 0, A%multishift)), 'matrix' )
+      if ( own_diag ) then
          if ( A%P%has_shift ) then
             if ( cplx ) then
                call idrs_check( idrs_matrix_set_precon_c128(h, 1, shift), 'matrix' )
@@ -191,10 +234,38 @@ contains
          else
             call idrs_check( idrs_matrix_set_precon_f64(h, 1), 'matrix' )
          end if
+      elseif ( A%P%which_precon == 1 ) then
+         sh = [ real(shift, kind=rp), aimag(shift) ]
+         if ( A%P%has_shift ) then
+            call idrs_check( idrs_matrix_set_precon_diag(h, pk, pm, pc, sh, merge(1, 0, cplx)), 'matrix' )
+         else
+            call idrs_check( idrs_matrix_set_precon_diag(h, pk, pm, pc, is_complex=merge(1, 0, cplx)), 'matrix' )
+         end if
       end if
       call idrs_check( idrs_matrix_set_alpha(h, deg, alpha), 'matrix' )
+      if ( did > 0 ) then
+         if ( cplx ) then
+            if ( allocated(A%cV) ) then
+               call idrs_check( idrs_make_deflation_c128(h, size(A%cV,2), A%cV), 'make_deflation' )
+            else
+               call idrs_check( idrs_make_deflation_c128(h, size(A%rV,2), cmplx(A%rV, 0._rp, kind=cp)), &
+                                'make_deflation' )
+            end if
+         else
+            if ( .not. allocated(A%rV) ) error stop 'matrix: a complex deflation space on a real operator'
+            call idrs_check( idrs_make_deflation_f64(h, size(A%rV,2), A%rV), 'make_deflation' )
+         end if
+      end if
       cache(victim)%op = h
       cache(victim)%k = A%K%h
+      cache(victim)%m = mh
+      cache(victim)%c = ch
+      cache(victim)%pk = pk
+      cache(victim)%pm = pm
+      cache(victim)%pc = pc
+      cache(victim)%block = A%block_system
+      cache(victim)%multishift = A%multishift
+      cache(victim)%defl_id = did
       cache(victim)%which_precon = A%P%which_precon
       cache(victim)%has_shift = A%P%has_shift
       cache(victim)%shift = shift
@@ -204,6 +275,29 @@ contains
       cache(victim)%alpha = alpha
       cache(victim)%stamp = clock
    end function matrix_handle
+
+   logical function same( a, b )
+      type(c_ptr), intent(in) :: a, b
+      if ( c_associated(a) ) then
+         same = c_associated(a, b)
+      else
+         same = .not. c_associated(b)
+      end if
+   end function same
+
+   function real_mv( A, v ) result(w)
+      type(matrix), intent(in)          :: A
+      real(kind=rp), intent(in)         :: v(:)
+      real(kind=rp)                     :: w(size(v))
+      call idrs_check( idrs_system_mv_f64(matrix_handle(A, .false.), v, w), 'real_mv' )
+   end function real_mv
+
+   function complex_mv( A, v ) result(w)
+      type(matrix), intent(in)          :: A
+      complex(kind=cp), intent(in)      :: v(:)
+      complex(kind=cp)                  :: w(size(v))
+      call idrs_check( idrs_system_mv_c128(matrix_handle(A, .true.), v, w), 'complex_mv' )
+   end function complex_mv
 
    function cmatvec( A, v ) result(w)
       type(matrix), intent(in)          :: A
@@ -223,33 +317,113 @@ contains
       type(matrix), intent(in)          :: A
       real(kind=rp), intent(in)         :: v(:)
       real(kind=rp)                     :: w(size(v))
-      if ( A%block_system ) error stop 'matrix: block systems are not offered by the GPU library'
-      w = v/A%P
+      if ( A%block_system ) then
+         call idrs_check( idrs_precon_f64(matrix_handle(A, .false.), v, w), 'precon' )
+      else
+         w = v/A%P
+      end if
    end function rprecon
 
    function cprecon( v, A ) result(w)
       type(matrix), intent(in)          :: A
       complex(kind=cp), intent(in)      :: v(:)
       complex(kind=cp)                  :: w(size(v))
-      if ( A%block_system ) error stop 'matrix: block systems are not offered by the GPU library'
-      w = v/A%P
+      if ( A%block_system ) then
+         call idrs_check( idrs_precon_c128(matrix_handle(A, .true.), v, w), 'precon' )
+      else
+         w = v/A%P
+      end if
    end function cprecon
 
    function rPmul( A, v ) result(w)
       type(matrix), intent(in)          :: A
       real(kind=rp), intent(in)         :: v(:)
       real(kind=rp)                     :: w(size(v))
-      if ( A%n_deflation > 0 ) error stop 'matrix: deflation is not offered by the GPU library'
-      w = v
+      if ( A%n_deflation > 0 ) then
+         call idrs_check( idrs_pmul_f64(matrix_handle(A, .false.), v, w), 'Pmul' )
+      else
+         w = v
+      end if
    end function rPmul
 
    function cPmul( A, v ) result(w)
       type(matrix), intent(in)          :: A
       complex(kind=cp), intent(in)      :: v(:)
       complex(kind=cp)                  :: w(size(v))
-      if ( A%n_deflation > 0 ) error stop 'matrix: deflation is not offered by the GPU library'
-      w = v
+      if ( A%n_deflation > 0 ) then
+         call idrs_check( idrs_pmul_c128(matrix_handle(A, .true.), v, w), 'Pmul' )
+      else
+         w = v
+      end if
    end function cPmul
+
+! make_deflation (matrix_module.f90:1284-1352): keeps V, AV = real_mv(A, V)
+! and VtAV = V^T AV (V^H AV) in A; the device operator forms its own copies
+   subroutine rmake_deflation( A, V )
+      type(matrix), intent(inout)       :: A
+      real(kind=rp), intent(in)         :: V(:,:)
+      integer                           :: j
+      A%n_deflation = size(V,2)
+      if ( allocated(A%rV) ) deallocate(A%rV, A%rAV, A%rVtAV)
+      if ( allocated(A%cV) ) deallocate(A%cV, A%cAV, A%cVtAV)
+      allocate( A%rV(size(V,1),size(V,2)), A%rAV(size(V,1),size(V,2)), A%rVtAV(size(V,2),size(V,2)) )
+      A%rV = V
+      deflation_count = deflation_count + 1
+      A%deflation_id = deflation_count
+      do j = 1, size(V,2)
+         A%rAV(:,j) = real_mv( A, V(:,j) )
+      end do
+      A%rVtAV = matmul( transpose(V), A%rAV )
+   end subroutine rmake_deflation
+
+   subroutine cmake_deflation( A, V )
+      type(matrix), intent(inout)       :: A
+      complex(kind=cp), intent(in)      :: V(:,:)
+      integer                           :: j
+      A%n_deflation = size(V,2)
+      if ( allocated(A%rV) ) deallocate(A%rV, A%rAV, A%rVtAV)
+      if ( allocated(A%cV) ) deallocate(A%cV, A%cAV, A%cVtAV)
+      allocate( A%cV(size(V,1),size(V,2)), A%cAV(size(V,1),size(V,2)), A%cVtAV(size(V,2),size(V,2)) )
+      A%cV = V
+      deflation_count = deflation_count + 1
+      A%deflation_id = deflation_count
+      do j = 1, size(V,2)
+         A%cAV(:,j) = complex_mv( A, V(:,j) )
+      end do
+      A%cVtAV = matmul( conjg(transpose(V)), A%cAV )
+   end subroutine cmake_deflation
+
+! add_deflation (matrix_module.f90:1433-1623): x = V E^-1 V^H b +
+! (I - V E^-1 V^H A) y; with shifts, E = V^H AV - (sigma - seed) I per shift
+   function radd_deflation( A, b, y ) result(x)
+      type(matrix), intent(in)          :: A
+      real(kind=rp), intent(in)         :: b(:), y(:)
+      real(kind=rp)                     :: x(size(y))
+      call idrs_check( idrs_add_deflation_f64(matrix_handle(A, .false.), b, y, x), 'add_deflation' )
+   end function radd_deflation
+
+   function cadd_deflation( A, b, y ) result(x)
+      type(matrix), intent(in)          :: A
+      complex(kind=cp), intent(in)      :: b(:), y(:)
+      complex(kind=cp)                  :: x(size(y))
+      call idrs_check( idrs_add_deflation_c128(matrix_handle(A, .true.), b, y, x), 'add_deflation' )
+   end function cadd_deflation
+
+   function rms_add_deflation( A, sigma, b, y ) result(x)
+      type(matrix), intent(in)          :: A
+      real(kind=rp), intent(in)         :: sigma(:), b(:), y(:,:)
+      real(kind=rp)                     :: x(size(y,1),size(y,2))
+      call idrs_check( idrs_ms_add_deflation_f64(matrix_handle(A, .false.), size(sigma), sigma, b, y, x), &
+                       'add_deflation' )
+   end function rms_add_deflation
+
+   function cms_add_deflation( A, sigma, b, y ) result(x)
+      type(matrix), intent(in)          :: A
+      complex(kind=cp), intent(in)      :: sigma(:), b(:), y(:,:)
+      complex(kind=cp)                  :: x(size(y,1),size(y,2))
+      call idrs_check( idrs_ms_add_deflation_c128(matrix_handle(A, .true.), size(sigma), sigma, b, y, x), &
+                       'add_deflation' )
+   end function cms_add_deflation
 
    subroutine real_preconditioner( A, preconditioner, degree, foci, center, H )
       type(matrix), intent(inout)                 :: A
@@ -422,7 +596,7 @@ contains
       real(kind=rp), intent(in)           :: y(:)
       real(kind=rp), intent(in)           :: b(:)
       real(kind=rp)                       :: x(size(y))
-      call idrs_check( idrs_scaleback_f64(matrix_handle(A, .false.), y, x), 'scaleback' )
+      call idrs_check( idrs_scaleback_b_f64(matrix_handle(A, .false.), b, y, x), 'scaleback' )
    end function rscaleback
 
    function cscaleback( A, b, y ) result(x)
@@ -430,13 +604,13 @@ contains
       complex(kind=cp), intent(in)           :: y(:)
       complex(kind=cp), intent(in)           :: b(:)
       complex(kind=cp)                       :: x(size(y))
-      call idrs_check( idrs_scaleback_c128(matrix_handle(A, .true.), y, x), 'scaleback' )
+      call idrs_check( idrs_scaleback_b_c128(matrix_handle(A, .true.), b, y, x), 'scaleback' )
    end function cscaleback
 
 ! Scale back the multishift solutions (matrix_module.f90:724-806, without
-! deflation and shift-and-invert): column k of x is p_k(K') y(:,k) / D with
-! the shifted polynomial alpha_s(:,k) from MS_PARAMETERS (alpha = 1 without
-! a polynomial preconditioner).
+! shift-and-invert): column k of x is p_k(A) y(:,k) with the shifted
+! polynomial alpha_s(:,k) from MS_PARAMETERS (alpha = 1 without a
+! polynomial preconditioner), then the deflation step for sigma_k, then /A.
    function ms_rscaleback( A, sigma, b, y ) result(x)
       type(matrix), intent(in)               :: A
       real(kind=rp), intent(in)              :: y(:,:)
@@ -456,7 +630,8 @@ contains
          allocate( coef(0:0, size(sigma)) )
          coef = (1._cp, 0._cp)
       end if
-      call idrs_check( idrs_ms_scaleback_f64(matrix_handle(A, .false.), size(sigma), deg, coef, y, x, 0), 'scaleback' )
+      call idrs_check( idrs_ms_scaleback_b_f64(matrix_handle(A, .false.), size(sigma), sigma, deg, coef, b, y, x), &
+                       'scaleback' )
    end function ms_rscaleback
 
    function ms_cscaleback( A, sigma, b, y ) result(x)
@@ -478,7 +653,8 @@ contains
          allocate( coef(0:0, size(sigma)) )
          coef = (1._cp, 0._cp)
       end if
-      call idrs_check( idrs_ms_scaleback_c128(matrix_handle(A, .true.), size(sigma), deg, coef, y, x, 0), 'scaleback' )
+      call idrs_check( idrs_ms_scaleback_b_c128(matrix_handle(A, .true.), size(sigma), sigma, deg, coef, b, y, x), &
+                       'scaleback' )
    end function ms_cscaleback
 
 ! Shifts of the preconditioned system and the scale-back polynomials

@@ -49,9 +49,12 @@ module user_module
       complex(kind=cp), allocatable  :: cmat(:,:)
       integer, allocatable           :: imat(:,:)
 ! Device objects: h is the stored matrix, hp the diagonal preconditioner
-! operator built from h (which_precon = 1)
+! operator built from h (which_precon = 1); hm, hc the mass and damping
+! matrices whose diagonals enter a shifted diagonal preconditioner
       type(c_ptr)                    :: h = c_null_ptr
       type(c_ptr)                    :: hp = c_null_ptr
+      type(c_ptr)                    :: hm = c_null_ptr
+      type(c_ptr)                    :: hc = c_null_ptr
    end type user_matrix
 
    INTERFACE OPERATOR(*)
@@ -259,7 +262,12 @@ contains
    function REAL_PRECON( which_precon, K, M, C, shift ) result(P)
 !
 ! which_precon = 0 : no preconditioner
-! which_precon = 1 : diagonal preconditioner D = diag(K) [- shift]
+! which_precon = 1 : diagonal preconditioner (user_module.f90:1352-1381):
+!    D = diag(K) + shift diag(C) + shift^2 diag(M)   (M and C)
+!        diag(K) + shift diag(C) + shift^2           (C)
+!        diag(K) - shift diag(M)                     (M)
+!        diag(K) - shift                              (neither)
+!    and diag(K) without a shift, whatever M and C
 !
       integer, intent(in)                              :: which_precon
       type(user_matrix), intent(in)                    :: K
@@ -267,62 +275,94 @@ contains
       type(user_matrix), intent(in), optional          :: C
       type(user_matrix)                                :: P
       real(kind=rp), optional                          :: shift
+      real(c_double)                                   :: sh(1)
 
       P%matrix_format = 0
       P%which_precon = which_precon
       P%nrows = K%nrows
       P%ncols = K%ncols
-      if ( present(M) .or. present(C) ) &
-         error stop 'REAL_PRECON: mass and damping matrices are not offered by the GPU library'
       if ( which_precon == 1 ) then
          P%matrix_format = 4
          P%h = K%h
          call idrs_check( idrs_matrix_create(K%h, P%hp), 'REAL_PRECON' )
+         allocate( P%rband(P%nrows,0:0) )
+         P%rband(:,0) = real_diagonal( K )
          if ( present(shift) ) then
             P%has_shift = .true.
             P%rshift = shift
-            call idrs_check( idrs_matrix_set_precon_f64(P%hp, 1, shift), 'REAL_PRECON' )
+            if ( present(M) ) P%hm = M%h
+            if ( present(C) ) P%hc = C%h
+            if ( present(M) .or. present(C) ) then
+               sh(1) = shift
+               call idrs_check( idrs_matrix_set_precon_diag(P%hp, K%h, P%hm, P%hc, sh, 0), 'REAL_PRECON' )
+            else
+               call idrs_check( idrs_matrix_set_precon_f64(P%hp, 1, shift), 'REAL_PRECON' )
+            end if
+! rband(:,0) = the diagonal the preconditioner divides by
+            if ( present(M) .and. present(C) ) then
+               P%rband(:,0) = P%rband(:,0) + shift*real_diagonal(C) + shift**2*real_diagonal(M)
+            elseif ( present(C) ) then
+               P%rband(:,0) = P%rband(:,0) + shift*real_diagonal(C) + shift**2
+            elseif ( present(M) ) then
+               P%rband(:,0) = P%rband(:,0) - shift*real_diagonal(M)
+            else
+               P%rband(:,0) = P%rband(:,0) - shift
+            end if
          else
             call idrs_check( idrs_matrix_set_precon_f64(P%hp, 1), 'REAL_PRECON' )
          end if
-! rband(:,0) = the diagonal the preconditioner divides by
-         allocate( P%rband(P%nrows,0:0) )
-         P%rband(:,0) = real_diagonal( K )
-         if ( present(shift) ) P%rband(:,0) = P%rband(:,0) - shift
       elseif ( which_precon /= 0 ) then
          error stop 'REAL_PRECON: dense and band LU preconditioners are not offered by the GPU library'
       end if
    end function REAL_PRECON
 
    function COMPLEX_PRECON( which_precon, K, M, C, shift ) result(P)
+!
+! COMPLEX_PRECON (user_module.f90:1465-1530): the diagonals of REAL_PRECON
+! with a complex shift
+!
       integer, intent(in)                              :: which_precon
       type(user_matrix), intent(in)                    :: K
       type(user_matrix), intent(in), optional          :: M
       type(user_matrix), intent(in), optional          :: C
       type(user_matrix)                                :: P
       complex(kind=cp), optional                       :: shift
+      real(c_double)                                   :: sh(2)
 
       P%matrix_format = 0
       P%which_precon = which_precon
       P%nrows = K%nrows
       P%ncols = K%ncols
       P%is_complex = .true.
-      if ( present(M) .or. present(C) ) &
-         error stop 'COMPLEX_PRECON: mass and damping matrices are not offered by the GPU library'
       if ( which_precon == 1 ) then
          P%matrix_format = 4
          P%h = K%h
          call idrs_check( idrs_matrix_create(K%h, P%hp), 'COMPLEX_PRECON' )
+         allocate( P%cband(P%nrows,0:0) )
+         P%cband(:,0) = complex_diagonal( K )
          if ( present(shift) ) then
             P%has_shift = .true.
             P%cshift = shift
-            call idrs_check( idrs_matrix_set_precon_c128(P%hp, 1, shift), 'COMPLEX_PRECON' )
+            if ( present(M) ) P%hm = M%h
+            if ( present(C) ) P%hc = C%h
+            if ( present(M) .or. present(C) ) then
+               sh = [ real(shift, kind=rp), aimag(shift) ]
+               call idrs_check( idrs_matrix_set_precon_diag(P%hp, K%h, P%hm, P%hc, sh, 1), 'COMPLEX_PRECON' )
+            else
+               call idrs_check( idrs_matrix_set_precon_c128(P%hp, 1, shift), 'COMPLEX_PRECON' )
+            end if
+            if ( present(M) .and. present(C) ) then
+               P%cband(:,0) = P%cband(:,0) + shift*complex_diagonal(C) + shift**2*complex_diagonal(M)
+            elseif ( present(C) ) then
+               P%cband(:,0) = P%cband(:,0) + shift*complex_diagonal(C) + shift**2
+            elseif ( present(M) ) then
+               P%cband(:,0) = P%cband(:,0) - shift*complex_diagonal(M)
+            else
+               P%cband(:,0) = P%cband(:,0) - shift
+            end if
          else
             call idrs_check( idrs_matrix_set_precon_c128(P%hp, 1), 'COMPLEX_PRECON' )
          end if
-         allocate( P%cband(P%nrows,0:0) )
-         P%cband(:,0) = complex_diagonal( K )
-         if ( present(shift) ) P%cband(:,0) = P%cband(:,0) - shift
       elseif ( which_precon /= 0 ) then
          error stop 'COMPLEX_PRECON: dense and band LU preconditioners are not offered by the GPU library'
       end if
@@ -336,6 +376,8 @@ contains
       if ( c_associated(A%h) .and. A%which_precon == 0 ) status = idrs_user_matrix_destroy(A%h)
       A%h = c_null_ptr
       A%hp = c_null_ptr
+      A%hm = c_null_ptr
+      A%hc = c_null_ptr
    end subroutine user_matrix_free
 
 end module user_module

@@ -218,6 +218,15 @@ int idrs_add_deflation_c128(idrs_matrix_t A, const double *b, const double *y, d
 /* x = scaleback(A, b, y) with its deflation step, matrix_module.f90:808-849 */
 int idrs_scaleback_b_f64(idrs_matrix_t A, const double *b, const double *y, double *x);
 int idrs_scaleback_b_c128(idrs_matrix_t A, const double *b, const double *y, double *x);
+/* X = add_deflation(A, sigma, b, Y), rms_add_deflation /
+   cms_add_deflation (matrix_module.f90:1508-1623): per shift k,
+   V Hs^-1 V^H b + (y_k - V Hs^-1 V^H (real_mv(A, y_k) - (sigma_k - seed) y_k)),
+   Hs = V^H AV - (sigma_k - seed) I; Y, X (rows x n_sigma) column-major host
+   arrays, sigma as for idrs_ms_scaleback_b_*; Y copied without deflation */
+int idrs_ms_add_deflation_f64(idrs_matrix_t A, int n_sigma, const double *sigma, const double *b, const double *Y,
+                              double *X);
+int idrs_ms_add_deflation_c128(idrs_matrix_t A, int n_sigma, const double *sigma, const double *b, const double *Y,
+                               double *X);
 /* X = scaleback(A, sigma, b, Y), matrix_module.f90:724-806 without
    shift-and-invert: per shift k, p_k(A) y_k (alpha_s as for
    idrs_ms_scaleback_*), ms_add_deflation for sigma_k (:1508-1623), x/A;

@@ -26,6 +26,8 @@ program drop_in
    if ( command_argument_count() >= 1 ) then
       call get_command_argument( 1, data_dir )
       call stommel_mm( trim(data_dir) )
+      call wedge_neumann_deflated( trim(data_dir) )
+      call wedge_sommerfeld_multi( trim(data_dir) )
    end if
 
 contains
@@ -59,6 +61,101 @@ contains
       call user_matrix_free( A%P )
       call user_matrix_free( K )
    end subroutine stommel_mm
+
+! The wedge Neumann example's single-frequency chain with deflation
+! (EXAMPLES/MATRIX_MARKET/wedge_neumann.f90 with PP_IDRS's RPP_IDRS steps
+! written out): (K - sigma M) x = b, sigma = (2 pi)^2, V = M 1,
+! make_deflation, IDR(4) on Pmul(A, b) at tol 1e-6 with the default M1,
+! scaleback with the deflation step; err is the true relative residual.
+   subroutine wedge_neumann_deflated( dir )
+      character(len=*), intent(in)  :: dir
+      type(user_matrix)             :: K, M, RHS
+      type(matrix)                  :: A, M1
+      real(kind=rp), allocatable    :: x(:), b(:), y(:), V(:,:)
+      real(kind=rp)                 :: relres, sigma
+      real(kind=rp), parameter      :: pi = 3.141592653589793_rp
+      integer                       :: flag, iter, n
+
+      K = mm_matrix( dir//'/wedge3_K.mtx' )
+      M = mm_matrix( dir//'/wedge3_M.mtx' )
+      RHS = mm_matrix( dir//'/wedge3_b.mtx' )
+      n = K%nrows
+      allocate( b(n), x(n), y(n), V(n,1) )
+      b = RHS%rmat(:,1)
+      sigma = (2._rp*pi)**2
+      A%K = K
+      A%mass_matrix = .true.
+      A%M = M
+      A%rseed = sigma
+      V = 1._rp
+      V(:,1) = M*V(:,1)
+      call make_deflation( A, V )
+      y = idrs( A, Pmul(A,b), 4, M1, 1.e-6_rp, 1000, 1, flag, relres, iter )
+      x = scaleback( A, b, y )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'wedge3_neumann', ' iter', iter, ' flag', flag, &
+         ' relres', relres, ' err', norm( b - K*x + sigma*(M*x) )/norm( b )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'wedge3_neumann_vtav', ' iter', A%n_deflation, &
+         ' flag', 0, ' relres', A%rVtAV(1,1), ' err', abs( A%rVtAV(1,1) - dot_product( V(:,1), K*V(:,1) - sigma*(M*V(:,1)) ) )
+      call user_matrix_free( M )
+      call user_matrix_free( K )
+   end subroutine wedge_neumann_deflated
+
+! The wedge Sommerfeld example with -multi -deflation (wedge_sommerfeld.f90,
+! CPP_MS_IDRS of PP_IDRS written out): the block system of 2 n rows for
+! K + sigma C + sigma^2 M at sigma = 2 pi f i, f = 1, 2; lumped mass
+! preconditioner complex_precon(1, M); seed at the last frequency;
+! V = [0; 1/sqrt(n)]; ms_parameters, MSIDRS(4) on Pmul(A, b), tol 1e-6;
+! scaleback per shift; the solution is the second half of each column; err
+! is each frequency's true relative residual of the damped system.
+   subroutine wedge_sommerfeld_multi( dir )
+      character(len=*), intent(in)  :: dir
+      type(user_matrix)             :: K, M, C, RHS, P
+      type(matrix)                  :: A, M1
+      complex(kind=cp), allocatable :: x(:,:), b(:), rhs_c(:), V(:,:), sol(:,:)
+      complex(kind=cp)              :: sigma(2), sigma_p(2)
+      real(kind=rp)                 :: rr(2)
+      real(kind=rp), parameter      :: pi = 3.141592653589793_rp
+      integer                       :: flag, iter, nn, i
+      character(len=24)             :: name
+
+      K = mm_matrix( dir//'/wedge2_K.mtx' )
+      M = mm_matrix( dir//'/wedge2_M.mtx' )
+      C = mm_matrix( dir//'/wedge2_C.mtx' )
+      RHS = mm_matrix( dir//'/wedge2_b.mtx' )
+      nn = K%nrows
+      allocate( b(2*nn), rhs_c(nn), x(2*nn,2), V(2*nn,1), sol(nn,2) )
+      rhs_c = RHS%rmat(:,1)
+      b(1:nn) = -rhs_c
+      b(nn+1:) = (0._cp, 0._cp)
+      sigma = (2._cp*pi*[1._cp, 2._cp])*(0._cp, 1._cp)
+      P = complex_precon( 1, M )
+      A%K = K
+      A%mass_matrix = .true.
+      A%M = M
+      A%damping_matrix = .true.
+      A%C = C
+      A%P = P
+      A%multishift = .true.
+      A%block_system = .true.
+      A%cseed = sigma(2)
+      V = (0._cp, 0._cp)
+      V(nn+1:,1) = 1._cp/sqrt(real(nn, kind=cp))
+      call make_deflation( A, V )
+      call ms_parameters( A, sigma, sigma_p )
+      x = msidrs( A, Pmul(A,b), sigma_p, 4, M1, 1.e-6_rp, 1000, 1, flag, rr, iter )
+      x = scaleback( A, sigma, b, x )
+      sol = x(nn+1:,:)
+      do i = 1, 2
+         write(name,'(a,i0)') 'wedge2_sommerfeld_f', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', rr(i), ' err', norm( rhs_c - K*sol(:,i) - sigma(i)*(C*sol(:,i)) - sigma(i)**2*(M*sol(:,i)) ) &
+            /norm( rhs_c )
+      end do
+      call user_matrix_free( P )
+      call user_matrix_free( C )
+      call user_matrix_free( M )
+      call user_matrix_free( K )
+   end subroutine wedge_sommerfeld_multi
 
 ! 1-D convection-diffusion, N=60 (EXAMPLES/TOEPLITZ/toeplitz.f90:111-165),
 ! stored in CRS; positional call as in the example.

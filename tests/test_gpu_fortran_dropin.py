@@ -23,7 +23,8 @@ BIN = os.path.join(ROOT, "idrs-package_amd", "fortran", "bin", "drop_in")
 def cases(gpu, tmp_path_factory):
     assert os.path.exists(BIN), "Fortran drop-in program not built (amdflang)"
     d = tmp_path_factory.mktemp("mm")
-    for name in ("stommel6", "stommel6_b"):
+    for name in ("stommel6", "stommel6_b", "wedge3_K", "wedge3_M", "wedge3_b", "wedge2_K", "wedge2_M", "wedge2_C",
+                 "wedge2_b"):
         write_mtx(mm_fixture(name), str(d / f"{name}.mtx"))
     out = subprocess.run([BIN, str(d)], capture_output=True, text=True, timeout=120)
     print(out.stdout, out.stderr)
@@ -179,3 +180,65 @@ def test_stommel_matrix_market(cases):
         slack = max(slack, 2 * rounding_spread(run, b, trials=8)[1])
     assert c["flag"] == fo == 0 and abs(c["iter"] - io) <= slack, (c, io, slack)
     assert c["err"] < 1e-7
+
+
+def _wedge_orc(k, cplx, **kw):
+    mats = []
+    for m in "KMC":
+        fx = mm_fixture(f"wedge{k}_{m}")
+        I, J, V = mm_expanded(fx)
+        mats.append(orc.OracleMatrix(orc.COO, int(fx["size"][0]), coo=(I, J, V), complex_vectors=cplx))
+    return mats, mm_fixture(f"wedge{k}_b")["vals"][:, 0].copy()
+
+
+def test_wedge_neumann_deflated(cases):
+    """K - sigma M with mass matrix, seed (2 pi)^2, deflation V = M 1,
+    make_deflation / Pmul / IDR(4) / scaleback through type(matrix), against
+    the oracle's chain; A%rVtAV as make_deflation leaves it."""
+    c, cv = cases["wedge3_neumann"], cases["wedge3_neumann_vtav"]
+    (Ko, Mo, _), b = _wedge_orc(3, False)
+    n = len(b)
+    sigma = (2 * np.pi) ** 2
+    fx = mm_fixture("wedge3_K")
+    I, J, V = mm_expanded(fx)
+    O = orc.OracleMatrix(orc.COO, n, coo=(I, J, V), seed=sigma, mass=Mo)
+    Vd = Mo.user_mv(np.ones(n)).reshape(n, 1)
+    _, VtAV = O.make_deflation(Vd)
+    run = lambda bb: (lambda q: (q[0], q[3]))(orc.idrs(O, O.pmul(bb), 4, tolerance=1e-6, maximum_iterations=1000))
+    _, io = run(b)
+    slack = 2
+    if abs(c["iter"] - io) > slack:
+        slack = max(slack, 2 * rounding_spread(run, b)[1])
+    assert c["flag"] == 0 and abs(c["iter"] - io) <= slack, (c, io, slack)
+    assert c["err"] < 1e-5
+    # printed with 5 significant digits; err: A%rVtAV against V^T (K V - sigma M V) in full precision
+    assert cv["iter"] == 1 and abs(cv["relres"] - VtAV[0, 0]) <= 1e-4 * abs(VtAV[0, 0])
+    assert cv["err"] <= 1e-10 * abs(VtAV[0, 0])
+
+
+def test_wedge_sommerfeld_multishift_deflated(cases):
+    """The block system of K + sigma C + sigma^2 M, lumped mass
+    preconditioner, deflation [0; 1/sqrt(n)], ms_parameters / MSIDRS /
+    scaleback through type(matrix), as wedge_sommerfeld.f90 -multi
+    -deflation runs it, against the oracle's chain."""
+    (Ko, Mo, Co), rhs = _wedge_orc(2, True)
+    n = len(rhs)
+    sigma = 2 * np.pi * np.array([1.0, 2.0]) * 1j
+    fx = mm_fixture("wedge2_K")
+    I, J, V = mm_expanded(fx)
+    O = orc.OracleMatrix(orc.COO, n, coo=(I, J, V), complex_vectors=True, seed=sigma[-1], mass=Mo, damping=Co,
+                         block=True, multishift=True, which_precon=1, precon_diag=Mo.diagonal())
+    Vd = np.zeros((2 * n, 1), complex)
+    Vd[n:, 0] = 1 / np.sqrt(n)
+    O.make_deflation(Vd)
+    bb = np.concatenate([-rhs, np.zeros(n)]).astype(complex)
+    run = lambda b1: (lambda q: (q["x"], q["iterations"]))(orc.msidrs(O, O.pmul(b1), sigma - sigma[-1], 4,
+                                                                        tolerance=1e-6, maximum_iterations=1000))
+    _, io = run(bb)
+    for k in range(2):
+        c = cases[f"wedge2_sommerfeld_f{k + 1}"]
+        slack = 2
+        if abs(c["iter"] - io) > slack:
+            slack = max(slack, 2 * rounding_spread(run, bb)[1])
+        assert c["flag"] == 0 and abs(c["iter"] - io) <= slack, (c, io, slack)
+        assert c["err"] < 1e-5
