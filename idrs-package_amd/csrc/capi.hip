@@ -8,6 +8,7 @@
 #include "mm.hpp"
 #include "ritz.hpp"
 #include "general.hpp"
+#include "lu.hpp"
 
 #include <cstdarg>
 #include <cstring>
@@ -835,7 +836,8 @@ int idrs_matrix_set_precon_f64(idrs_matrix_t A, int which_precon, const double *
     return guarded([&] {
         need(A != nullptr, "null handle");
         need(which_precon == 0 || which_precon == 1,
-             "which_precon must be 0 (none) or 1 (diagonal); dense/band LU (2, 3) are not offered");
+             "which_precon must be 0 (none) or 1 (diagonal); dense/band LU (2, 3): idrs_matrix_set_precon_lu");
+        A->op.lu.reset();
         std::complex<double> sh = shift ? std::complex<double>(*shift, 0.0) : 0.0;
         op_set_diag_precon(A->op, which_precon, shift ? &sh : nullptr);
     });
@@ -844,7 +846,8 @@ int idrs_matrix_set_precon_c128(idrs_matrix_t A, int which_precon, const double 
     return guarded([&] {
         need(A != nullptr, "null handle");
         need(which_precon == 0 || which_precon == 1,
-             "which_precon must be 0 (none) or 1 (diagonal); dense/band LU (2, 3) are not offered");
+             "which_precon must be 0 (none) or 1 (diagonal); dense/band LU (2, 3): idrs_matrix_set_precon_lu");
+        A->op.lu.reset();
         std::complex<double> sh = shift ? std::complex<double>(shift[0], shift[1]) : 0.0;
         op_set_diag_precon(A->op, which_precon, shift ? &sh : nullptr);
     });
@@ -1053,7 +1056,47 @@ int idrs_matrix_set_precon_diag(idrs_matrix_t A, idrs_user_matrix_t PK, idrs_use
         }
         if (!cdiag)
             for (long i = 0; i < n; i++) d[i] = d[i].real();
+        A->op.lu.reset();
         op_set_diag_values(A->op, d, cdiag);
+    });
+}
+
+int idrs_matrix_set_precon_lu(idrs_matrix_t A, int which, idrs_user_matrix_t PK, idrs_user_matrix_t PM,
+                              idrs_user_matrix_t PC, const double *shift, int is_complex) {
+    return guarded([&] {
+        need(A && PK, "null argument");
+        need(which == 2 || which == 3, "which_precon must be 2 (dense LU) or 3 (band LU)");
+        need(PK->M->nrows == A->op.K->nrows, "preconditioner matrices must have K's rows");
+        std::complex<double> s;
+        if (shift) s = is_complex ? std::complex<double>(shift[0], shift[1]) : std::complex<double>(shift[0], 0.0);
+        const bool cplx = is_complex != 0 || PK->M->kind == ValKind::Complex ||
+                          (shift && PM && PM->M->kind == ValKind::Complex) ||
+                          (shift && PC && PC->M->kind == ValKind::Complex);
+        hipStream_t st = lib_stream();
+        auto F = lu_build(which, *PK->M, PM ? PM->M : nullptr, PC ? PC->M : nullptr, shift ? &s : nullptr, cplx, st);
+        A->op.lu = F;
+        A->op.which_precon = which;
+    });
+}
+
+int idrs_matrix_share_lu(idrs_matrix_t A, idrs_matrix_t B) {
+    return guarded([&] {
+        need(A && B, "null argument");
+        need(B->op.lu != nullptr, "the source operator has no LU preconditioner");
+        need(B->op.lu->n == A->op.K->nrows, "the LU preconditioner has another number of rows");
+        A->op.lu = B->op.lu;
+        A->op.which_precon = B->op.which_precon;
+    });
+}
+
+int idrs_matrix_set_shift_invert(idrs_matrix_t A, int shift_invert, const double *shift, int is_complex) {
+    return guarded([&] {
+        need(A != nullptr, "null handle");
+        need(!shift_invert || shift, "shift-and-invert needs the shift");
+        A->op.shift_invert = shift_invert != 0;
+        A->op.si_shift = !shift ? std::complex<double>(0.0, 0.0)
+                                : is_complex ? std::complex<double>(shift[0], shift[1])
+                                             : std::complex<double>(shift[0], 0.0);
     });
 }
 

@@ -12,6 +12,7 @@
 // sequence that HIP graphs can capture.
 #include "general.hpp"
 
+#include "lu.hpp"
 #include "spmv_epi.hpp"
 
 #include <algorithm>
@@ -40,6 +41,31 @@ __global__ __launch_bounds__(GB) void k_lin(long n, T *__restrict__ w, T a, cons
     GLOOP(q, n) {
         const T p = a * u[q];
         w[q] = sub ? w[q] - p : w[q] + p;
+    }
+}
+
+// the block system's shift-and-invert right-hand side (matrix_module.f90:
+// 361-366): u = (v1 + u) + s m, m = M v2 (mass) or v2
+template <class T>
+__global__ __launch_bounds__(GB) void k_si_rhs(long nn, const T *__restrict__ v1, T *__restrict__ u, T s,
+                                               const T *__restrict__ m) {
+#pragma clang fp contract(off)
+    GLOOP(q, nn) {
+        const T p = s * m[q];
+        u[q] = (v1[q] + u[q]) + p;
+    }
+}
+
+// w1 = t1 - s t2, w2 = -t2 (t1 = v2)
+template <class T>
+__global__ __launch_bounds__(GB) void k_si_out(long nn, const T *__restrict__ v2, const T *__restrict__ t2, T s,
+                                               T *__restrict__ w) {
+#pragma clang fp contract(off)
+    GLOOP(q, nn) {
+        const T p = s * t2[q];
+        const T a = v2[q];
+        w[q] = a - p;
+        w[nn + q] = -t2[q];
     }
 }
 
@@ -196,13 +222,14 @@ T fromc(std::complex<double> v) {
     else return v.real();
 }
 
-// workspace of the general path: six vectors of rows() elements of T, then
-// the deflation partials and coefficients
+// workspace of the general path: eight vectors of rows() elements of T
+// (6 and 7 private to the preconditioner), then the deflation partials and
+// coefficients
 struct GWork {
     char *base;
     long rows;
     template <class T> T *vec(int i) const { return (T *)base + (long)i * rows; }
-    template <class T> T *part() const { return (T *)base + 6L * rows; }
+    template <class T> T *part() const { return (T *)base + 8L * rows; }
     template <class T> T *coef() const { return part<T>() + (long)MAX_DEFL * GEN_PARTS; }
 };
 
@@ -249,22 +276,51 @@ void defl_sub(const Operator &op, const T *V, const T *t, T *w, int sub, hipStre
 
 }  // namespace
 
-// v/A (rprecon/cprecon without shift-and-invert, matrix_module.f90:306-400)
+// v/A%P on nn rows (ruser_precon / cuser_precon, user_module.f90:243-283):
+// the LU solve, the diagonal, or a copy
 template <class T>
-void gen_precon(const Operator &op, const T *v, T *w, hipStream_t st) {
-    const long nn = op.K->nrows, n = op.rows();
-    const int g = ggrid(n);
+static void user_precon(const Operator &op, long nn, const T *v, T *w, hipStream_t st) {
+    if (op.lu) {
+        lu_solve<T>(*op.lu, v, w, st);
+        return;
+    }
+    const int g = ggrid(nn);
     const bool d = op.which_precon == 1 && op.pdiag;
     if (d && op.pdiag_kind == ValKind::Complex) {
         if constexpr (Tr<T>::is_complex)
-            hipLaunchKernelGGL((k_gprecon<T, c128>), dim3(g), dim3(GB), 0, st, nn, n, v, (const c128 *)op.pdiag, w);
+            hipLaunchKernelGGL((k_gprecon<T, c128>), dim3(g), dim3(GB), 0, st, nn, nn, v, (const c128 *)op.pdiag, w);
         else
             throw std::invalid_argument("complex preconditioner for a real solve");
     } else {
-        hipLaunchKernelGGL((k_gprecon<T, double>), dim3(g), dim3(GB), 0, st, nn, n, v,
+        hipLaunchKernelGGL((k_gprecon<T, double>), dim3(g), dim3(GB), 0, st, nn, nn, v,
                            d ? (const double *)op.pdiag : (const double *)nullptr, w);
     }
     LAUNCH_CHECK();
+}
+
+// v/A (rprecon/cprecon, matrix_module.f90:306-400): v/A%P; a block system
+// applies it to the first half and copies the second, or with
+// shift-and-invert inverts [-C-sM -K; I -sI] through P = K + sC + s^2 M:
+// t2 = (v1 + C v2 + s (M v2 | v2))/P, w = (v2 - s t2, -t2)
+template <class T>
+void gen_precon(const Operator &op, const T *v, T *w, hipStream_t st) {
+    const long nn = op.K->nrows, n = op.rows();
+    if (op.block && op.shift_invert) {
+        if (!op.Cm) throw std::invalid_argument("a block system needs the damping matrix C");
+        const GWork g = work<T>(op);
+        T *u = g.vec<T>(6), *m = g.vec<T>(7);
+        const T s = fromc<T>(op.si_shift);
+        user_spmv<T>(*op.Cm, v + nn, u, st);
+        if (op.Mm) user_spmv<T>(*op.Mm, v + nn, m, st);
+        hipLaunchKernelGGL((k_si_rhs<T>), dim3(ggrid(nn)), dim3(GB), 0, st, nn, v, u, s, op.Mm ? (const T *)m : v + nn);
+        LAUNCH_CHECK();
+        user_precon<T>(op, nn, u, m, st);
+        hipLaunchKernelGGL((k_si_out<T>), dim3(ggrid(nn)), dim3(GB), 0, st, nn, v + nn, (const T *)m, s, w);
+        LAUNCH_CHECK();
+        return;
+    }
+    user_precon<T>(op, nn, v, w, st);
+    if (n > nn) gcopy<T>(n - nn, v + nn, w + nn, st);
 }
 
 // real_mv/complex_mv (matrix_module.f90:169-264); t, u: workspace vectors
@@ -439,12 +495,22 @@ void gen_add_deflation(const Operator &op, const T *b, const T *y, T *x, hipStre
     add_defl<T>(op, b, y, x, false, T(0), st);
 }
 
+// sigma_s of a shift: (sigma - seed)/(sigma - shift) with shift-and-invert,
+// else sigma - seed (matrix_module.f90:443-456, 1534-1538)
+template <class T>
+static T sigma_s(const Operator &op, std::complex<double> sigma) {
+    const T num = fromc<T>(sigma) - seedT<T>(op);
+    if (!op.shift_invert) return num;
+    const T den = fromc<T>(sigma) - fromc<T>(op.si_shift);
+    return num / den;
+}
+
 template <class T>
 void gen_ms_add_deflation(const Operator &op, int nsig, const std::complex<double> *sigma, const T *b, const T *Y,
                           T *X, hipStream_t st) {
     const long n = op.rows();
     for (int k = 0; k < nsig; k++) {
-        const T ss = fromc<T>(sigma[k]) - seedT<T>(op);
+        const T ss = sigma_s<T>(op, sigma[k]);
         add_defl<T>(op, b, Y + (long)n * k, X + (long)n * k, true, ss, st);
     }
 }
@@ -462,7 +528,8 @@ void gen_scaleback(const Operator &op, const T *b, const T *y, T *x, hipStream_t
 
 // ms_rscaleback/ms_cscaleback (matrix_module.f90:724-806) for one shift's
 // column: x = p_sigma(A) y with the shift's coefficients, the deflation
-// step for that shift, x/A
+// step for that shift, the shift-and-invert factor
+// (seed - shift)/(sigma - shift), x/A
 template <class T>
 void gen_ms_scaleback_col(const Operator &op, const std::vector<std::complex<double>> &alpha, std::complex<double> sigma,
                           const T *b, const T *y, T *x, hipStream_t st) {
@@ -470,10 +537,17 @@ void gen_ms_scaleback_col(const Operator &op, const std::vector<std::complex<dou
     T *p = g.vec<T>(0), *q = g.vec<T>(1);
     gen_polpre<T>(op, alpha, (int)alpha.size() - 1, y, p, st);
     if (op.ndefl > 0) {
-        const T ss = fromc<T>(sigma) - seedT<T>(op);
-        add_defl<T>(op, b, p, q, true, ss, st);
+        add_defl<T>(op, b, p, q, true, sigma_s<T>(op, sigma), st);
     } else {
         gcopy<T>(op.rows(), p, q, st);
+    }
+    if (op.shift_invert) {
+        const T sh = fromc<T>(op.si_shift);
+        const T f = (seedT<T>(op) - sh) / (fromc<T>(sigma) - sh);
+        hipLaunchKernelGGL((k_gscale<T>), dim3(ggrid(op.rows())), dim3(GB), 0, st, op.rows(), f, (const T *)q, p);
+        LAUNCH_CHECK();
+        gen_precon<T>(op, p, x, st);
+        return;
     }
     gen_precon<T>(op, q, x, st);
 }
@@ -483,7 +557,7 @@ void gen_prepare(const Operator &op) {
     (void)hipFree(op.gws);
     op.gws = nullptr;
     const long n = op.rows();
-    HIP_TRY(hipMalloc(&op.gws, sizeof(c128) * (6 * n + (long)MAX_DEFL * GEN_PARTS + MAX_DEFL)));
+    HIP_TRY(hipMalloc(&op.gws, sizeof(c128) * (8 * n + (long)MAX_DEFL * GEN_PARTS + MAX_DEFL)));
     op.gws_rows = n;
 }
 

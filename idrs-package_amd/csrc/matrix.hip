@@ -28,6 +28,7 @@
 #include "general.hpp"
 #include "spmv_epi.hpp"
 #include "ritz.hpp"
+#include "lu.hpp"
 
 #include <algorithm>
 #include <climits>

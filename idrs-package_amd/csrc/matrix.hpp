@@ -100,6 +100,8 @@ struct DevMatrix {
 // Polynomial preconditioner kinds (matrix_module real_preconditioner)
 enum { POLY_NONE = 0, POLY_NEUMANN = 1, POLY_CHEBYSHEV = 2, POLY_HESSENBERG = 3 };
 
+struct LuFactor;
+
 struct Operator {
     DevMatrix *K = nullptr;       // not owned
     int which_precon = 0;         // 0 identity, 1 diagonal scaling (A%P)
@@ -133,9 +135,16 @@ struct Operator {
     int ndefl = 0;                   // A%n_deflation
     bool defl_complex = false;       // dV, dAV, dVtAV hold c128 (else double)
     void *dV = nullptr, *dAV = nullptr, *dVtAV = nullptr;   // rows x nd, rows x nd, nd x nd (device)
+    // which_precon 2 / 3: the LU factors of REAL_PRECON(2|3, ...) (shared
+    // between operators built from one preconditioner), and A%shift_invert
+    // with A%rshift / A%cshift (user_module.f90:1383-1459,
+    // matrix_module.f90:306-549, 724-806)
+    std::shared_ptr<LuFactor> lu;
+    bool shift_invert = false;
+    std::complex<double> si_shift{0.0, 0.0};
     mutable void *gws = nullptr;     // workspace of the general path (gen_prepare)
     mutable long gws_rows = 0;
-    bool general() const { return Mm || Cm || block || ndefl > 0; }
+    bool general() const { return Mm || Cm || block || ndefl > 0 || lu || shift_invert; }
     long rows() const { return block ? 2L * K->nrows : K->nrows; }
     ~Operator();
     const void *values() const { return sval ? sval : K->val; }

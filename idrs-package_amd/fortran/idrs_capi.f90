@@ -371,6 +371,27 @@ module idrs_capi
          integer(c_int), value :: is_complex
       end function
 
+      integer(c_int) function idrs_matrix_set_precon_lu(A, which, PK, PM, PC, shift, is_complex) &
+            bind(C, name="idrs_matrix_set_precon_lu")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: A, PK, PM, PC
+         integer(c_int), value :: which, is_complex
+         real(c_double), intent(in), optional :: shift(*)
+      end function
+
+      integer(c_int) function idrs_matrix_share_lu(A, B) bind(C, name="idrs_matrix_share_lu")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: A, B
+      end function
+
+      integer(c_int) function idrs_matrix_set_shift_invert(A, shift_invert, shift, is_complex) &
+            bind(C, name="idrs_matrix_set_shift_invert")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: A
+         integer(c_int), value :: shift_invert, is_complex
+         real(c_double), intent(in), optional :: shift(*)
+      end function
+
       integer(c_int) function idrs_make_deflation_f64(A, nd, V) bind(C, name="idrs_make_deflation_f64")
          import :: c_int, c_double, c_ptr
          type(c_ptr), value :: A

@@ -22,8 +22,10 @@
 !
 ! Mass (A%M) and damping (A%C) matrices, block systems and deflation run on
 ! the device operator; make_deflation also fills A%rAV/A%rVtAV (A%cAV,
-! A%cVtAV) as the reference does.  Not offered (stop with a message):
-! shift-and-invert.
+! A%cVtAV) as the reference does.  Shift-and-invert (A%shift_invert with
+! A%rshift / A%cshift and an LU preconditioner A%P = REAL_PRECON(2|3, ...))
+! runs on the device operator too: the block system's inverse, the
+! multishift parameters and scale-back factor.
 !
 module matrix_module
 
@@ -115,7 +117,8 @@ module matrix_module
       type(c_ptr)                   :: k = c_null_ptr
       type(c_ptr)                   :: m = c_null_ptr, c = c_null_ptr
       type(c_ptr)                   :: pk = c_null_ptr, pm = c_null_ptr, pc = c_null_ptr
-      logical                       :: block = .false., multishift = .false.
+      logical                       :: block = .false., multishift = .false., si = .false.
+      complex(kind=cp)              :: si_shift = (0._cp,0._cp)
       integer                       :: defl_id = 0
       integer                       :: which_precon = 0
       logical                       :: has_shift = .false.
@@ -144,9 +147,9 @@ contains
       type(c_ptr)                       :: mh, ch, pk, pm, pc
       logical                           :: own_diag
       real(c_double)                    :: sh(2)
+      complex(kind=cp)                  :: sis
 
       if ( .not. c_associated(A%K%h) ) error stop 'matrix: the system matrix A%K is not set'
-      if ( A%shift_invert ) error stop 'matrix: shift-and-invert is not offered by the GPU library'
       mh = c_null_ptr
       ch = c_null_ptr
       if ( A%mass_matrix ) mh = A%M%h
@@ -161,7 +164,13 @@ contains
          pk = A%P%h
          pm = A%P%hm
          pc = A%P%hc
+      elseif ( A%P%which_precon == 2 .or. A%P%which_precon == 3 ) then
+! the LU factors live in the preconditioner's own operator: shared
+         pk = A%P%hp
       end if
+      sis = A%cshift
+      if ( .not. cplx ) sis = cmplx(A%rshift, 0._rp, kind=cp)
+      if ( cplx .and. A%rshift /= 0._rp .and. A%cshift == (0._cp,0._cp) ) sis = cmplx(A%rshift, 0._rp, kind=cp)
 ! a diagonal preconditioner built from A%K alone folds into the matrix
 ! values; any other (e.g. a lumped mass matrix) divides by its diagonal
       own_diag = A%P%which_precon == 1 .and. c_associated(pk, A%K%h) .and. .not. c_associated(pm) &
@@ -199,6 +208,7 @@ contains
          if ( .not. same(cache(i)%pk, pk) .or. .not. same(cache(i)%pm, pm) .or. .not. same(cache(i)%pc, pc) ) cycle
          if ( (cache(i)%block .neqv. A%block_system) .or. (cache(i)%multishift .neqv. A%multishift) ) cycle
          if ( cache(i)%defl_id /= did ) cycle
+         if ( (cache(i)%si .neqv. A%shift_invert) .or. cache(i)%si_shift /= sis ) cycle
          if ( cache(i)%which_precon /= A%P%which_precon .or. (cache(i)%has_shift .neqv. A%P%has_shift) ) cycle
          if ( cache(i)%shift /= shift .or. cache(i)%seed /= seed .or. (cache(i)%cplx .neqv. cplx) ) cycle
          if ( cache(i)%degree /= deg ) cycle
@@ -234,6 +244,8 @@ contains
          else
             call idrs_check( idrs_matrix_set_precon_f64(h, 1), 'matrix' )
          end if
+      elseif ( A%P%which_precon == 2 .or. A%P%which_precon == 3 ) then
+         call idrs_check( idrs_matrix_share_lu(h, A%P%hp), 'matrix' )
       elseif ( A%P%which_precon == 1 ) then
          sh = [ real(shift, kind=rp), aimag(shift) ]
          if ( A%P%has_shift ) then
@@ -241,6 +253,10 @@ contains
          else
             call idrs_check( idrs_matrix_set_precon_diag(h, pk, pm, pc, is_complex=merge(1, 0, cplx)), 'matrix' )
          end if
+      end if
+      if ( A%shift_invert ) then
+         sh = [ real(sis, kind=rp), aimag(sis) ]
+         call idrs_check( idrs_matrix_set_shift_invert(h, 1, sh, merge(1, 0, cplx)), 'matrix' )
       end if
       call idrs_check( idrs_matrix_set_alpha(h, deg, alpha), 'matrix' )
       if ( did > 0 ) then
@@ -266,6 +282,8 @@ contains
       cache(victim)%block = A%block_system
       cache(victim)%multishift = A%multishift
       cache(victim)%defl_id = did
+      cache(victim)%si = A%shift_invert
+      cache(victim)%si_shift = sis
       cache(victim)%which_precon = A%P%which_precon
       cache(victim)%has_shift = A%P%has_shift
       cache(victim)%shift = shift
@@ -607,10 +625,11 @@ contains
       call idrs_check( idrs_scaleback_b_c128(matrix_handle(A, .true.), b, y, x), 'scaleback' )
    end function cscaleback
 
-! Scale back the multishift solutions (matrix_module.f90:724-806, without
-! shift-and-invert): column k of x is p_k(A) y(:,k) with the shifted
-! polynomial alpha_s(:,k) from MS_PARAMETERS (alpha = 1 without a
-! polynomial preconditioner), then the deflation step for sigma_k, then /A.
+! Scale back the multishift solutions (matrix_module.f90:724-806): column k
+! of x is p_k(A) y(:,k) with the shifted polynomial alpha_s(:,k) from
+! MS_PARAMETERS (alpha = 1 without a polynomial preconditioner), then the
+! deflation step for sigma_k, the shift-and-invert factor
+! (seed - shift)/(sigma_k - shift), then /A.
    function ms_rscaleback( A, sigma, b, y ) result(x)
       type(matrix), intent(in)               :: A
       real(kind=rp), intent(in)              :: y(:,:)
@@ -619,7 +638,6 @@ contains
       real(kind=rp)                          :: x(size(y,1),size(y,2))
       complex(kind=cp), allocatable          :: coef(:,:)
       integer                                :: deg
-      if ( A%shift_invert ) error stop 'SCALEBACK: shift-and-invert is not offered by the GPU library'
       deg = 0
       if ( A%degree > 0 ) then
          if ( .not. allocated(A%ralpha_s) ) error stop 'First call subroutine MS_PARAMETERS'
@@ -642,7 +660,6 @@ contains
       complex(kind=cp)                       :: x(size(y,1),size(y,2))
       complex(kind=cp), allocatable          :: coef(:,:)
       integer                                :: deg
-      if ( A%shift_invert ) error stop 'SCALEBACK: shift-and-invert is not offered by the GPU library'
       deg = 0
       if ( A%degree > 0 ) then
          if ( .not. allocated(A%calpha_s) ) error stop 'First call subroutine MS_PARAMETERS'
@@ -658,16 +675,28 @@ contains
    end function ms_cscaleback
 
 ! Shifts of the preconditioned system and the scale-back polynomials
-! (matrix_module.f90:402-549, without shift-and-invert): host arithmetic on
-! the polynomial coefficients only.
+! (matrix_module.f90:402-549): host arithmetic on the shifts and the
+! polynomial coefficients only.  With shift-and-invert the shifts are
+! eta = (sigma - seed)/(sigma - shift); every shift is checked against the
+! preconditioner's shift (the reference's loop bound there is read before
+! it is set).
    subroutine real_ms_parameters( A, sigma, sigma_p )
       type(matrix), intent(inout)                 :: A
       real(kind=rp), intent(in)                   :: sigma(:)
       real(kind=rp), intent(out)                  :: sigma_p(:)
       real(kind=rp)                               :: sigma_s(size(sigma))
       integer                                     :: i, i_sigma, n_sigma
-      if ( A%shift_invert ) error stop 'MS_PARAMETERS: shift-and-invert is not offered by the GPU library'
-      sigma_s = sigma - A%rseed
+      if ( A%shift_invert ) then
+         if ( A%rshift == A%rseed ) error stop &
+            'Preconditioner shift should not be equal to seed, choose preconditioner shift and seed differently'
+         do i_sigma = 1, size(sigma)
+            if ( A%rshift == sigma(i_sigma) ) error stop &
+               'Preconditioner shift should not be equal to one of the shifts, choose different preconditioner shift'
+         end do
+         sigma_s = ( sigma-A%rseed )/( sigma-A%rshift )
+      else
+         sigma_s = sigma - A%rseed
+      end if
       if ( A%degree > 0 ) then
          if ( .not. allocated(A%ralpha_p) ) error stop 'First call subroutine real_preconditioner '
          n_sigma = size(sigma)
@@ -692,8 +721,17 @@ contains
       complex(kind=cp), intent(out)               :: sigma_p(:)
       complex(kind=cp)                            :: sigma_s(size(sigma))
       integer                                     :: i, i_sigma, n_sigma
-      if ( A%shift_invert ) error stop 'MS_PARAMETERS: shift-and-invert is not offered by the GPU library'
-      sigma_s = sigma - A%cseed
+      if ( A%shift_invert ) then
+         if ( A%cshift == A%cseed ) error stop &
+            'Preconditioner shift should not be equal to seed, choose preconditioner shift and seed differently'
+         do i_sigma = 1, size(sigma)
+            if ( A%cshift == sigma(i_sigma) ) error stop &
+               'Preconditioner shift should not be equal to one of the shifts, choose different preconditioner shift'
+         end do
+         sigma_s = ( sigma-A%cseed )/( sigma-A%cshift )
+      else
+         sigma_s = sigma - A%cseed
+      end if
       if ( A%degree > 0 ) then
          if ( .not. allocated(A%calpha_p) ) error stop 'First call subroutine complex_preconditioner '
          n_sigma = size(sigma)

@@ -6,14 +6,17 @@
 !
 !   crs_format / coo_format / cds_format    user_module.f90:829-958
 !   operator(*)  (ruser_mv / cuser_mv)       user_module.f90:197-241
-!   operator(/)  (ruser_precon / cuser_precon, diagonal scaling)  :243-283
+!   operator(/)  (ruser_precon / cuser_precon: diagonal, LU)  :243-283
 !   real_diagonal / complex_diagonal         user_module.f90:960-1161
-!   REAL_PRECON / COMPLEX_PRECON(1, K)       user_module.f90:1327-1381, 1465-1518
+!   REAL_PRECON / COMPLEX_PRECON(which, K, M, C, shift)
+!                 1 diagonal, 2 dense LU, 3 band LU  user_module.f90:1327-1602
 !
 ! Storage formats: matrix_format 2 CRS, 3 COO, 4 CDS, 5 DIA (offset list,
 ! an extension for wide sparse bands); 1 dense holds host data only (dense
-! products stop with a message).  The dense/band LU preconditioners
-! (which_precon 2, 3) are not offered: they stop with a message.  Device storage is released by user_matrix_free (the reference's
+! products stop with a message).  The dense and band LU preconditioners
+! (which_precon 2, 3) keep their factors on the device (the reference's
+! P%rmat / P%rband host copies are not filled); the dense one takes CRS or
+! COO matrices.  Device storage is released by user_matrix_free (the reference's
 ! allocatable components are freed with the object; these handles are not).
 !
 module user_module
@@ -89,7 +92,7 @@ contains
       type(user_matrix), intent(in)     :: A
       real(kind=rp), intent(in)         :: v(:)
       real(kind=rp)                     :: w(size(v))
-      if ( A%which_precon == 1 ) then
+      if ( A%which_precon >= 1 ) then
          call idrs_check( idrs_precon_f64(A%hp, v, w), 'user_precon' )
       else
          w = v
@@ -100,7 +103,7 @@ contains
       type(user_matrix), intent(in)     :: A
       complex(kind=cp), intent(in)      :: v(:)
       complex(kind=cp)                  :: w(size(v))
-      if ( A%which_precon == 1 ) then
+      if ( A%which_precon >= 1 ) then
          call idrs_check( idrs_precon_c128(A%hp, v, w), 'user_precon' )
       else
          w = v
@@ -311,8 +314,25 @@ contains
          else
             call idrs_check( idrs_matrix_set_precon_f64(P%hp, 1), 'REAL_PRECON' )
          end if
+      elseif ( which_precon == 2 .or. which_precon == 3 ) then
+! dense (2) or band (3) LU of K [+ shift C + shift^2 M | ...] (:1383-1459),
+! factored and kept on the device: v/P solves with the factors
+         P%matrix_format = merge(1, 4, which_precon == 2)
+         P%h = K%h
+         if ( present(M) ) P%hm = M%h
+         if ( present(C) ) P%hc = C%h
+         call idrs_check( idrs_matrix_create(K%h, P%hp), 'REAL_PRECON' )
+         if ( present(shift) ) then
+            P%has_shift = .true.
+            P%rshift = shift
+            sh(1) = shift
+            call idrs_check( idrs_matrix_set_precon_lu(P%hp, which_precon, K%h, P%hm, P%hc, sh, 0), 'REAL_PRECON' )
+         else
+            call idrs_check( idrs_matrix_set_precon_lu(P%hp, which_precon, K%h, P%hm, P%hc, is_complex=0), &
+                             'REAL_PRECON' )
+         end if
       elseif ( which_precon /= 0 ) then
-         error stop 'REAL_PRECON: dense and band LU preconditioners are not offered by the GPU library'
+         error stop 'REAL_PRECON: which_precon must be 0, 1, 2 or 3'
       end if
    end function REAL_PRECON
 
@@ -363,8 +383,23 @@ contains
          else
             call idrs_check( idrs_matrix_set_precon_c128(P%hp, 1), 'COMPLEX_PRECON' )
          end if
+      elseif ( which_precon == 2 .or. which_precon == 3 ) then
+         P%matrix_format = merge(1, 4, which_precon == 2)
+         P%h = K%h
+         if ( present(M) ) P%hm = M%h
+         if ( present(C) ) P%hc = C%h
+         call idrs_check( idrs_matrix_create(K%h, P%hp), 'COMPLEX_PRECON' )
+         if ( present(shift) ) then
+            P%has_shift = .true.
+            P%cshift = shift
+            sh = [ real(shift, kind=rp), aimag(shift) ]
+            call idrs_check( idrs_matrix_set_precon_lu(P%hp, which_precon, K%h, P%hm, P%hc, sh, 1), 'COMPLEX_PRECON' )
+         else
+            call idrs_check( idrs_matrix_set_precon_lu(P%hp, which_precon, K%h, P%hm, P%hc, is_complex=1), &
+                             'COMPLEX_PRECON' )
+         end if
       elseif ( which_precon /= 0 ) then
-         error stop 'COMPLEX_PRECON: dense and band LU preconditioners are not offered by the GPU library'
+         error stop 'COMPLEX_PRECON: which_precon must be 0, 1, 2 or 3'
       end if
    end function COMPLEX_PRECON
 

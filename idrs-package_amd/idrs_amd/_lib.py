@@ -155,6 +155,11 @@ _PROTOS = {
     "idrs_matrix_set_flags": (c_int, [c_void_p, c_int, c_int]),
     "idrs_matrix_rows": (c_int, [c_void_p, POINTER(c_int)]),
     "idrs_matrix_set_precon_diag": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "idrs_matrix_set_precon_lu": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    "idrs_matrix_share_lu": (c_int, [c_void_p, c_void_p]),
+    "idrs_matrix_set_shift_invert": (c_int, [c_void_p, c_int, c_void_p, c_int]),
+    "idrs_ms_add_deflation_f64": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "idrs_ms_add_deflation_c128": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "idrs_make_deflation_f64": (c_int, [c_void_p, c_int, c_void_p]),
     "idrs_make_deflation_c128": (c_int, [c_void_p, c_int, c_void_p]),
     "idrs_pmul_f64": (c_int, [c_void_p, c_void_p, c_void_p]),

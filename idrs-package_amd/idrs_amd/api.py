@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 from . import _lib
-from ._lib import check, idrs_krylov_opts_t, idrs_opts_t, idrs_stats_t
+from ._lib import IdrsError, check, idrs_krylov_opts_t, idrs_opts_t, idrs_stats_t
 
 # H: the Hessenberg matrix H(nritz+1, nritz) when asked for, else None
 IdrsResult = namedtuple("IdrsResult", "x flag relres iterations resvec solve_ms spmv_ms spmv_launches H",
@@ -230,6 +230,8 @@ class Matrix:
         self.is_complex = K.is_complex
         self.n = K.nrows
         self.seed = 0.0
+        self.shift_invert = False
+        self.si_shift = 0.0
         if which_precon:
             self.set_precon(which_precon, shift)
 
@@ -338,6 +340,35 @@ class Matrix:
         check(_lib.load().idrs_matrix_set_precon_diag(self._h, PK.handle, PM.handle if PM is not None else None,
                                                       PC.handle if PC is not None else None, _ptr(s), int(cplx)))
 
+    def set_precon_lu(self, which, PK, PM=None, PC=None, shift=None):
+        """A%P = REAL_PRECON / COMPLEX_PRECON(which, PK, PM, PC, shift)
+        (user_module.f90:1383-1459, 1531-1602): which 2 dense LU, 3 band LU
+        of PK [+ shift PC + shift^2 PM | ...], factored on the device.
+        Raises IdrsError on a zero pivot."""
+        self._P = (PK, PM, PC)
+        cplx = self.is_complex or np.iscomplexobj(shift)
+        s = None
+        if shift is not None:
+            s = np.array([np.real(shift), np.imag(shift)]) if cplx else np.array([float(shift)])
+        check(_lib.load().idrs_matrix_set_precon_lu(self._h, int(which), PK.handle,
+                                                    PM.handle if PM is not None else None,
+                                                    PC.handle if PC is not None else None, _ptr(s), int(cplx)))
+
+    def share_lu(self, other):
+        """Use other's LU factors (no second factorization)."""
+        self._P = getattr(other, "_P", None)
+        check(_lib.load().idrs_matrix_share_lu(self._h, other.handle))
+
+    def set_shift_invert(self, shift):
+        """A%shift_invert = .true., A%rshift / A%cshift = shift (pp_idrs with
+        present(P) and present(shift)); None switches it off."""
+        self.shift_invert = shift is not None
+        self.si_shift = 0.0 if shift is None else shift
+        cplx = np.iscomplexobj(shift)
+        s = None if shift is None else (np.array([np.real(shift), np.imag(shift)]) if cplx
+                                        else np.array([float(shift)]))
+        check(_lib.load().idrs_matrix_set_shift_invert(self._h, int(shift is not None), _ptr(s), int(cplx)))
+
     def _cpl(self, *arrs):
         return self.is_complex or any(np.iscomplexobj(a) for a in arrs)
 
@@ -421,11 +452,30 @@ def scaleback(A, b, y):
     return x
 
 
+def ms_add_deflation(A, sigma, b, Y):
+    """X = add_deflation(A, sigma, b, Y) (rms_add_deflation /
+    cms_add_deflation, matrix_module.f90:1508-1623)."""
+    sig = np.atleast_1d(sigma)
+    cplx = A.is_complex or np.iscomplexobj(sig) or np.iscomplexobj(Y) or np.iscomplexobj(b)
+    dt = np.complex128 if cplx else np.float64
+    Y = np.asfortranarray(Y, dtype=dt)
+    X = np.empty_like(Y, order="F")
+    b = np.ascontiguousarray(b, dtype=dt)
+    if cplx:
+        sp = np.empty(2 * len(sig))
+        sp[0::2], sp[1::2] = np.real(sig), np.imag(sig)
+    else:
+        sp = np.ascontiguousarray(sig, dtype=np.float64)
+    f = _lib.load().idrs_ms_add_deflation_c128 if cplx else _lib.load().idrs_ms_add_deflation_f64
+    check(f(A.handle, len(sig), _ptr(sp), _ptr(b), _ptr(Y), _ptr(X)))
+    return X
+
+
 def ms_scaleback_b(A, sigma, alpha_s, b, Y):
     """X = scaleback(A, sigma, b, Y) (ms_rscaleback / ms_cscaleback,
-    matrix_module.f90:724-806, without shift-and-invert): per shift the
-    polynomial alpha_s(:, k) (from ms_parameters), the deflation step for
-    sigma_k, then /A."""
+    matrix_module.f90:724-806): per shift the polynomial alpha_s(:, k) (from
+    ms_parameters), the deflation step for sigma_k, the shift-and-invert
+    factor (seed - shift)/(sigma_k - shift), then /A."""
     sig = np.atleast_1d(sigma)
     cplx = A.is_complex or np.iscomplexobj(sig) or np.iscomplexobj(Y) or np.iscomplexobj(b)
     dt = np.complex128 if cplx else np.float64
@@ -1013,9 +1063,17 @@ def ms_parameters(A, sigma):
     sigma = np.atleast_1d(np.asarray(sigma))
     alpha = A.alpha
     d = len(alpha) - 1
-    cplx = np.iscomplexobj(sigma) or np.iscomplexobj(A.seed) or np.any(np.imag(alpha) != 0)
+    si = getattr(A, "shift_invert", False)
+    cplx = (np.iscomplexobj(sigma) or np.iscomplexobj(A.seed) or np.any(np.imag(alpha) != 0)
+            or (si and np.iscomplexobj(A.si_shift)))
     dt = np.complex128 if cplx else np.float64
-    sig_s = (sigma - A.seed).astype(dt)
+    if si:
+        # shift-and-invert (matrix_module.f90:420-446): eta = (sigma - seed)/(sigma - shift)
+        if A.si_shift == A.seed or np.any(sigma == A.si_shift):
+            raise IdrsError(-1, "Preconditioner shift should not be equal to seed or to one of the shifts")
+        sig_s = ((sigma - A.seed) / (sigma - A.si_shift)).astype(dt)
+    else:
+        sig_s = (sigma - A.seed).astype(dt)
     alpha = alpha if cplx else alpha.real
     alpha_s = np.zeros((d + 1, len(sigma)), dtype=dt)
     sigma_p = np.zeros(len(sigma), dtype=dt)

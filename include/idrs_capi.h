@@ -201,6 +201,25 @@ int idrs_matrix_rows(idrs_matrix_t A, int *rows);
    PK = M. */
 int idrs_matrix_set_precon_diag(idrs_matrix_t A, idrs_user_matrix_t PK, idrs_user_matrix_t PM, idrs_user_matrix_t PC,
                                 const double *shift, int is_complex);
+/* A%P = REAL_PRECON / COMPLEX_PRECON(which, PK, PM, PC, shift),
+   user_module.f90:1383-1459 (real), 1531-1602 (complex): which 2 dense LU,
+   3 band LU (half bandwidth from get_band, :1833-1872) of
+   PK [+ shift PC + shift^2 PM | + shift PC + shift^2 I | - shift PM | - shift I]
+   (PK alone without a shift), factored on the device without pivoting
+   (:1604-1701, :2068-2179); v/A then solves with the factors (:1703-1831,
+   :2181-2336).  shift: NULL, one value, or an (re, im) pair when is_complex;
+   complex factors when is_complex or any matrix is complex.  A zero pivot
+   returns an error ("Pivot equal to zero in LU decomposition").  Dense LU
+   takes CRS/COO matrices. */
+int idrs_matrix_set_precon_lu(idrs_matrix_t A, int which, idrs_user_matrix_t PK, idrs_user_matrix_t PM,
+                              idrs_user_matrix_t PC, const double *shift, int is_complex);
+/* A's preconditioner = B's LU factors, shared (no second factorization) */
+int idrs_matrix_share_lu(idrs_matrix_t A, idrs_matrix_t B);
+/* A%shift_invert and A%rshift / A%cshift (matrix_module.f90:306-549,
+   724-806): the block system's preconditioner inverts [-C-sM -K; I -sI]
+   through A%P; ms scale-back multiplies by (seed - shift)/(sigma - shift);
+   the multishift deflation step uses (sigma - seed)/(sigma - shift) */
+int idrs_matrix_set_shift_invert(idrs_matrix_t A, int shift_invert, const double *shift, int is_complex);
 /* make_deflation(A, V), matrix_module.f90:1284-1352: V(rows, nd) host,
    nd <= 16; AV = real_mv(A, V) and V^H AV are formed on the device */
 int idrs_make_deflation_f64(idrs_matrix_t A, int nd, const double *V);

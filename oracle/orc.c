@@ -79,3 +79,45 @@ int orc_threads(int n)
 #define ONE ((orc_cplx)1.0)
 #include "orc_generic.inc"
 #include "orc_krylov.inc"
+
+/* get_band (user_module.f90:1833-1872): the largest |column - row| of the
+   stored entries (CRS, COO), the half bandwidth (CDS), or the largest
+   |offset| (DIA extension), over K and, when given, C and M
+   (REAL_PRECON :1423-1425) */
+static int orc_band_of(const orc_user_matrix *A)
+{
+    int hb = 0;
+    switch (A->format) {
+    case ORC_CRS:
+        for (int i = 0; i < A->nrows; i++)
+            for (int j = A->row_ptr[i]; j <= A->row_ptr[i + 1] - 1; j++) {
+                int d = abs(A->col_ind[j - 1] - (i + 1));
+                if (d > hb) hb = d;
+            }
+        break;
+    case ORC_COO:
+        for (int k = 0; k < A->nnz; k++) {
+            int d = abs(A->jndx[k] - A->indx[k]);
+            if (d > hb) hb = d;
+        }
+        break;
+    case ORC_CDS:
+        hb = A->half_band;
+        break;
+    case ORC_DIA:
+        for (int d = 0; d < A->ndiag; d++)
+            if (abs(A->offsets[d]) > hb) hb = abs(A->offsets[d]);
+        break;
+    default:
+        break;
+    }
+    return hb;
+}
+
+int orc_get_band(const orc_user_matrix *K, const orc_user_matrix *M, const orc_user_matrix *C)
+{
+    int hb = orc_band_of(K);
+    if (C && orc_band_of(C) > hb) hb = orc_band_of(C);
+    if (M && orc_band_of(M) > hb) hb = orc_band_of(M);
+    return hb;
+}

@@ -74,6 +74,13 @@ typedef struct {
     void *V, *AV, *VtAV;       /* A%rV/cV (n,nd), A%rAV/cAV (n,nd), A%rVtAV/cVtAV
                                   (nd,nd), T column-major; AV and VtAV are
                                   filled by make_deflation                  */
+    /* which_precon 2 / 3: P%rmat/cmat (n,n) or P%rband/cband (n,-hb:hb),
+       column-major T, holding the LU factors made by orc_*_precon_lu */
+    const void *lu;
+    int lu_hb;
+    int shift_invert;          /* A%shift_invert                            */
+    double rshift;             /* A%rshift                                  */
+    orc_cplx cshift;           /* A%cshift                                  */
 } orc_matrix;
 
 /* optional arguments of IDRS (idrs_module.f90:180-251).  has_* flags mirror
@@ -155,9 +162,32 @@ void orc_r_chebyshev(int degree, const double foci[2], double *alpha);
 void orc_c_chebyshev(int degree, const orc_cplx foci[2], orc_cplx *alpha);
 void orc_r_neumann(int degree, double center, double *alpha);
 void orc_c_neumann(int degree, orc_cplx center, orc_cplx *alpha);
-void orc_r_ms_parameters(const orc_matrix *A, int n_sigma, const double *sigma,
+/* REAL_PRECON / COMPLEX_PRECON(2|3, K, M, C, shift) (user_module.f90:
+   1383-1459, 1531-1602): the dense (which 2) or band (which 3) matrix
+   K [+ shift C + shift^2 M | + shift C + shift^2 I | - shift M | - shift I]
+   (shift NULL: K alone), then RLU_decomp / band_LU_decomp
+   (:1604-1701, :2068-2179), in place.  lu: (n,n) or (n,-hb:hb) column-major,
+   hb from get_band (:1833-1872) -- orc_*_get_band gives it for sizing.
+   Returns 0, or -1 on a zero pivot (the reference's error stop). */
+int  orc_get_band(const orc_user_matrix *K, const orc_user_matrix *M, const orc_user_matrix *C);
+int  orc_r_precon_lu(int which, const orc_user_matrix *K, const orc_user_matrix *M, const orc_user_matrix *C,
+                     const double *shift, int hb, double *lu);
+int  orc_c_precon_lu(int which, const orc_user_matrix *K, const orc_user_matrix *M, const orc_user_matrix *C,
+                     const orc_cplx *shift, int hb, orc_cplx *lu);
+/* RLU_solve / band_LU_solve (user_module.f90:1703-1831, 2181-2336) */
+void orc_r_lu_solve(int which, int n, int hb, const double *lu, const double *b, double *x);
+void orc_c_lu_solve(int which, int n, int hb, const orc_cplx *lu, const orc_cplx *b, orc_cplx *x);
+/* ms_rscaleback / ms_cscaleback (matrix_module.f90:724-806): per shift the
+   polynomial alpha_s(:,k) (degree A->degree, or y itself), the deflation
+   step, the shift-and-invert factor (seed-shift)/(sigma-shift), then /A;
+   Y, X (rows, n_sigma) column-major */
+void orc_r_ms_scaleback_b(const orc_matrix *A, int n_sigma, const double *sigma, const double *alpha_s,
+                          const double *b, const double *Y, double *X);
+void orc_c_ms_scaleback_b(const orc_matrix *A, int n_sigma, const orc_cplx *sigma, const orc_cplx *alpha_s,
+                          const orc_cplx *b, const orc_cplx *Y, orc_cplx *X);
+int  orc_r_ms_parameters(const orc_matrix *A, int n_sigma, const double *sigma,
                          double *sigma_p, double *alpha_s);
-void orc_c_ms_parameters(const orc_matrix *A, int n_sigma, const orc_cplx *sigma,
+int  orc_c_ms_parameters(const orc_matrix *A, int n_sigma, const orc_cplx *sigma,
                          orc_cplx *sigma_p, orc_cplx *alpha_s);
 
 /* ---- idrs_module restatement ---- */

@@ -43,6 +43,8 @@ class Matrix(ctypes.Structure):
         ("block_system", c_int), ("multishift", c_int),
         ("n_deflation", c_int),
         ("V", c_void_p), ("AV", c_void_p), ("VtAV", c_void_p),
+        ("lu", c_void_p), ("lu_hb", c_int),
+        ("shift_invert", c_int), ("rshift", c_double), ("cshift", c_double * 2),
     ]
 
 
@@ -123,10 +125,12 @@ class OracleMatrix:
 
     def __init__(self, fmt, n, ncols=None, *, crs=None, coo=None, cds=None, dia=None, which_precon=0,
                  degree=0, alpha=None, seed=0.0, complex_vectors=False, mass=None, damping=None, block=False,
-                 multishift=False, precon_diag=None):
+                 multishift=False, precon_diag=None, lu=None, shift_invert=False, si_shift=0.0):
         """mass, damping: OracleMatrix objects whose K is A%M / A%C;
         precon_diag: P%rband(:,0) given directly (e.g. a lumped mass
-        matrix's diagonal, real_precon(1, M)) instead of K's diagonal."""
+        matrix's diagonal, real_precon(1, M)) instead of K's diagonal;
+        lu: (which_precon 2 or 3, half bandwidth, factors) from precon_lu;
+        shift_invert, si_shift: A%shift_invert and A%rshift / A%cshift."""
         self.keep = []
         self.complex = complex_vectors
         um = UserMatrix()
@@ -197,6 +201,17 @@ class OracleMatrix:
             m.C = ctypes.addressof(damping.um)
         m.block_system = 1 if block else 0
         m.multishift = 1 if multishift else 0
+        if lu is not None:
+            which, hb, arr = lu
+            m.which_precon = which
+            m.lu_hb = hb
+            m.lu = hold(np.asarray(arr).ravel(order="F"), np.complex128 if complex_vectors else np.float64)
+        m.shift_invert = 1 if shift_invert else 0
+        if complex_vectors:
+            m.cshift[0], m.cshift[1] = np.real(si_shift), np.imag(si_shift)
+        else:
+            m.rshift = float(np.real(si_shift))
+        self.seed = seed
         self.m = m
         self.n = n
 
@@ -277,6 +292,33 @@ class OracleMatrix:
         self._call("scaleback_b", b, y, x)
         return x
 
+    def ms_parameters(self, sigma, alpha_p=None):
+        """(sigma_p, alpha_s) of real_ms_parameters / complex_ms_parameters;
+        raises where the reference stops (shift equal to the seed or to a
+        shift)."""
+        sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=self._dt())
+        sp = np.empty_like(sig)
+        a = np.zeros((max(self.m.degree, 0) + 1, len(sig)), dtype=self._dt(), order="F")
+        f = getattr(load(), ("orc_c_" if self.complex else "orc_r_") + "ms_parameters")
+        f.restype = c_int
+        if f(ctypes.byref(self.m), c_int(len(sig)), c_void_p(sig.ctypes.data), c_void_p(sp.ctypes.data),
+             c_void_p(a.ctypes.data)):
+            raise ValueError("Preconditioner shift should not be equal to seed or to one of the shifts")
+        return sp, a
+
+    def ms_scaleback_b(self, sigma, alpha_s, b, Y):
+        """ms_rscaleback / ms_cscaleback: X(:,k) from Y(:,k) for shift sigma(k)."""
+        sig = np.ascontiguousarray(np.atleast_1d(sigma), dtype=self._dt())
+        b = np.ascontiguousarray(b, dtype=self._dt())
+        Y = np.asfortranarray(Y, dtype=self._dt())
+        X = np.empty_like(Y, order="F")
+        a = None if alpha_s is None else np.asfortranarray(alpha_s, dtype=self._dt())
+        f = getattr(load(), ("orc_c_" if self.complex else "orc_r_") + "ms_scaleback_b")
+        f(ctypes.byref(self.m), c_int(len(sig)), c_void_p(sig.ctypes.data),
+          c_void_p(a.ctypes.data if a is not None else None), c_void_p(b.ctypes.data), c_void_p(Y.ctypes.data),
+          c_void_p(X.ctypes.data))
+        return X
+
     def user_mv(self, v):
         dt = np.complex128 if self.complex else np.float64
         v = np.ascontiguousarray(v, dtype=dt)
@@ -299,6 +341,51 @@ class OracleMatrix:
         f = load().orc_c_scaleback if self.complex else load().orc_r_scaleback
         f(ctypes.byref(self.m), c_void_p(y.ctypes.data), c_void_p(x.ctypes.data))
         return x
+
+
+def get_band(K, M=None, C=None):
+    """get_band over K (and C, M): the half bandwidth REAL_PRECON(3, ...) uses."""
+    lib = load()
+    lib.orc_get_band.restype = c_int
+    return lib.orc_get_band(ctypes.byref(K.um), ctypes.byref(M.um) if M is not None else None,
+                            ctypes.byref(C.um) if C is not None else None)
+
+
+def precon_lu(which, K, M=None, C=None, shift=None, complex_values=False):
+    """REAL_PRECON / COMPLEX_PRECON(which = 2 dense LU | 3 band LU, K, M, C,
+    shift) of OracleMatrix objects' user matrices: returns (which, hb,
+    factors) for OracleMatrix(lu=...); factors (n, n) or (n, 2 hb + 1)
+    column-major, the band's column d + hb holding diagonal d.  Raises on a
+    zero pivot."""
+    lib = load()
+    n = K.um.nrows
+    hb = n - 1 if which == 2 else get_band(K, M, C)
+    cplx = complex_values or np.iscomplexobj(shift)
+    dt = np.complex128 if cplx else np.float64
+    lu = np.zeros((n, n) if which == 2 else (n, 2 * hb + 1), dtype=dt, order="F")
+    f = getattr(lib, f"orc_{'c' if cplx else 'r'}_precon_lu")
+    f.restype = c_int
+    sh = None if shift is None else np.array([shift], dtype=dt)
+    rc = f(c_int(which), c_void_p(ctypes.addressof(K.um)),
+           c_void_p(ctypes.addressof(M.um)) if M is not None else None,
+           c_void_p(ctypes.addressof(C.um)) if C is not None else None,
+           c_void_p(sh.ctypes.data) if sh is not None else None, c_int(hb), c_void_p(lu.ctypes.data))
+    if rc:
+        raise ValueError("Pivot equal to zero in LU decomposition")
+    return which, hb, lu
+
+
+def lu_solve(lu, b):
+    """RLU_solve / band_LU_solve with the factors of precon_lu."""
+    which, hb, arr = lu
+    cplx = np.iscomplexobj(arr)
+    dt = np.complex128 if cplx else np.float64
+    b = np.ascontiguousarray(b, dtype=dt)
+    x = np.empty_like(b)
+    f = getattr(load(), f"orc_{'c' if cplx else 'r'}_lu_solve")
+    f(c_int(which), c_int(len(b)), c_int(hb), c_void_p(arr.ctypes.data), c_void_p(b.ctypes.data),
+      c_void_p(x.ctypes.data))
+    return x
 
 
 def chebyshev(degree, foci):

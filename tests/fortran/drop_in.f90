@@ -28,6 +28,8 @@ program drop_in
       call stommel_mm( trim(data_dir) )
       call wedge_neumann_deflated( trim(data_dir) )
       call wedge_sommerfeld_multi( trim(data_dir) )
+      call wedge_neumann_lu_multi( trim(data_dir) )
+      call wedge_sommerfeld_lu_multi( trim(data_dir) )
    end if
 
 contains
@@ -156,6 +158,108 @@ contains
       call user_matrix_free( M )
       call user_matrix_free( K )
    end subroutine wedge_sommerfeld_multi
+
+! wedge_neumann.f90 -lu -multi (RPP_MS_IDRS written out): the band LU
+! shift-and-invert preconditioner P = real_precon(3, K, M, shift) at
+! shift = -2 s_1 s_3/(s_1 + s_3), seed s_3, frequencies 1, 2, 4;
+! ms_parameters, MSIDRS(4) on Pmul(A, b) at tol 1e-6, scaleback; err is each
+! frequency's true relative residual of K - sigma M.
+   subroutine wedge_neumann_lu_multi( dir )
+      character(len=*), intent(in)  :: dir
+      type(user_matrix)             :: K, M, RHS, P
+      type(matrix)                  :: A, M1
+      real(kind=rp), allocatable    :: x(:,:), b(:)
+      real(kind=rp)                 :: sigma(3), sigma_p(3), rr(3), rshift
+      real(kind=rp), parameter      :: pi = 3.141592653589793_rp
+      integer                       :: flag, iter, n, i
+      character(len=24)             :: name
+
+      K = mm_matrix( dir//'/wedge3_K.mtx' )
+      M = mm_matrix( dir//'/wedge3_M.mtx' )
+      RHS = mm_matrix( dir//'/wedge3_b.mtx' )
+      n = K%nrows
+      allocate( b(n), x(n,3) )
+      b = RHS%rmat(:,1)
+      sigma = (2._rp*pi*[1._rp, 2._rp, 4._rp])**2
+      rshift = -2._rp*sigma(1)*sigma(3)/(sigma(1)+sigma(3))
+      P = real_precon( 3, K, M, shift=rshift )
+      A%K = K
+      A%mass_matrix = .true.
+      A%M = M
+      A%P = P
+      A%shift_invert = .true.
+      A%rshift = rshift
+      A%multishift = .true.
+      A%rseed = sigma(3)
+      call ms_parameters( A, sigma, sigma_p )
+      x = msidrs( A, Pmul(A,b), sigma_p, 4, M1, 1.e-6_rp, 1000, 1, flag, rr, iter )
+      x = scaleback( A, sigma, b, x )
+      do i = 1, 3
+         write(name,'(a,i0)') 'wedge3_neumann_lu_f', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', rr(i), ' err', norm( b - K*x(:,i) + sigma(i)*(M*x(:,i)) )/norm( b )
+      end do
+      call user_matrix_free( P )
+      call user_matrix_free( M )
+      call user_matrix_free( K )
+   end subroutine wedge_neumann_lu_multi
+
+! wedge_sommerfeld.f90 -lu -multi (CPP_MS_IDRS written out): the block
+! system with the band LU of K + shift C + shift^2 M,
+! shift = 2 pi/(f_3 + f_1) ((f_3 - f_1) sqrt(f_1 f_3) + 2 f_1 f_3 i),
+! A%shift_invert, seed at the last of the frequencies 1, 2, 4; err is each
+! frequency's true relative residual of the damped system.
+   subroutine wedge_sommerfeld_lu_multi( dir )
+      character(len=*), intent(in)  :: dir
+      type(user_matrix)             :: K, M, C, RHS, P
+      type(matrix)                  :: A, M1
+      complex(kind=cp), allocatable :: x(:,:), b(:), rhs_c(:), sol(:,:)
+      complex(kind=cp)              :: sigma(3), sigma_p(3), cshift
+      real(kind=rp)                 :: rr(3), freq(3)
+      real(kind=rp), parameter      :: pi = 3.141592653589793_rp
+      integer                       :: flag, iter, nn, i
+      character(len=24)             :: name
+
+      K = mm_matrix( dir//'/wedge2_K.mtx' )
+      M = mm_matrix( dir//'/wedge2_M.mtx' )
+      C = mm_matrix( dir//'/wedge2_C.mtx' )
+      RHS = mm_matrix( dir//'/wedge2_b.mtx' )
+      nn = K%nrows
+      allocate( b(2*nn), rhs_c(nn), x(2*nn,3), sol(nn,3) )
+      rhs_c = RHS%rmat(:,1)
+      b(1:nn) = -rhs_c
+      b(nn+1:) = (0._cp, 0._cp)
+      freq = [1._rp, 2._rp, 4._rp]
+      sigma = (2._cp*pi*freq)*(0._cp, 1._cp)
+      cshift = 2._cp*pi/(freq(3)+freq(1))*cmplx((freq(3)-freq(1))*sqrt(freq(1)*freq(3)), 2._rp*freq(1)*freq(3), &
+               kind=cp)
+      P = complex_precon( 3, K, M, C, shift=cshift )
+      A%K = K
+      A%mass_matrix = .true.
+      A%M = M
+      A%damping_matrix = .true.
+      A%C = C
+      A%P = P
+      A%shift_invert = .true.
+      A%cshift = cshift
+      A%multishift = .true.
+      A%block_system = .true.
+      A%cseed = sigma(3)
+      call ms_parameters( A, sigma, sigma_p )
+      x = msidrs( A, Pmul(A,b), sigma_p, 4, M1, 1.e-6_rp, 1000, 1, flag, rr, iter )
+      x = scaleback( A, sigma, b, x )
+      sol = x(nn+1:,:)
+      do i = 1, 3
+         write(name,'(a,i0)') 'wedge2_sommerfeld_lu_f', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', iter, ' flag', flag, &
+            ' relres', rr(i), ' err', norm( rhs_c - K*sol(:,i) - sigma(i)*(C*sol(:,i)) - sigma(i)**2*(M*sol(:,i)) ) &
+            /norm( rhs_c )
+      end do
+      call user_matrix_free( P )
+      call user_matrix_free( C )
+      call user_matrix_free( M )
+      call user_matrix_free( K )
+   end subroutine wedge_sommerfeld_lu_multi
 
 ! 1-D convection-diffusion, N=60 (EXAMPLES/TOEPLITZ/toeplitz.f90:111-165),
 ! stored in CRS; positional call as in the example.

@@ -242,3 +242,19 @@ def test_wedge_sommerfeld_multishift_deflated(cases):
             slack = max(slack, 2 * rounding_spread(run, bb)[1])
         assert c["flag"] == 0 and abs(c["iter"] - io) <= slack, (c, io, slack)
         assert c["err"] < 1e-5
+
+
+@pytest.mark.parametrize("case,k,nf", [("wedge3_neumann_lu_f", 3, 3), ("wedge2_sommerfeld_lu_f", 2, 3)])
+def test_wedge_shift_invert_multishift(cases, case, k, nf):
+    """The -lu -multi chains (REAL_PRECON / COMPLEX_PRECON(3, K, M[, C],
+    shift), A%shift_invert, ms_parameters' eta, MSIDRS, scaleback with the
+    shift-and-invert factor) through type(matrix): every frequency's true
+    residual at the tolerance, the iteration count near the oracle's (the
+    chains are restated in tests/test_gpu_lu.py against the oracle)."""
+    its = {c["iter"] for c in (cases[f"{case}{i + 1}"] for i in range(nf))}
+    assert len(its) == 1
+    for i in range(nf):
+        c = cases[f"{case}{i + 1}"]
+        assert c["flag"] == 0 and c["err"] < 1e-5, c
+    # shift-and-invert brings the wedge's ~450 multishift iterations to ~50
+    assert its.pop() < 120
