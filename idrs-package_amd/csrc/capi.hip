@@ -1399,6 +1399,63 @@ int idrs_idrs_mixed_c128(idrs_matrix_t A, int n, const double *b, int s, const i
     return mixed_entry<c128>(A, n, b, s, opts, inner_tolerance, max_outer, fp32_vectors, x, stats, outer_steps);
 }
 
+int idrs_ms_mixed_c128(idrs_matrix_t A, int method, int n, const double *b, int n_sigma, const double *sigma, int s,
+                       const idrs_krylov_opts_t *opts, double inner_tolerance, int max_outer, double *X,
+                       double *relres, idrs_stats_t *stats, int *outer_steps, int *corrections) {
+    return guarded([&] {
+        need(A && b && X && sigma && relres, "null argument");
+        need(n_sigma >= 1, "sigma must hold at least one shift");
+        need(A->op.rows() == n, "n does not match the matrix");
+        hipStream_t st = lib_stream();
+        op_prepare(A->op, st);
+        idrs_krylov_opts_t o{};
+        if (opts) o = *opts;
+        KrylovOpts<c128> ko;
+        ko.has_m1 = o.has_M1 != 0;
+        if (o.has_tolerance) ko.tol = o.tolerance;
+        if (o.has_maximum_iterations) {
+            ko.has_maxit = true;
+            ko.maxit = o.maximum_iterations;
+        }
+        if (o.has_variant) ko.variant = o.variant;
+        ko.has_inner_s = o.has_inner_s != 0;
+        ko.inner_s = o.inner_s;
+        ko.has_inner_tol = o.has_inner_tolerance != 0;
+        ko.inner_tol = o.inner_tolerance;
+        ko.has_inner_it = o.has_inner_iterations != 0;
+        ko.inner_it = o.inner_iterations;
+        ko.omega = (const c128 *)o.omega;
+        ko.resvec = o.resvec;
+        ko.x0 = (const c128 *)o.x0;
+        ko.want_colfac = o.colfac != nullptr;
+        ko.p_seed = o.p_seed ? o.p_seed : 1;
+        need(!o.PT_raw, "PT_raw is not offered by the complex64 multishift solve");
+        MsMixedStats ms;
+        if (o.device_pointers) {
+            ms_mixed_solve(A->op, n, n_sigma, (const c128 *)sigma, s, (const c128 *)b, (c128 *)X, ko, method,
+                           inner_tolerance, max_outer, relres, ms, st);
+        } else {
+            DVec<c128> db((const c128 *)b, n, st), dX((size_t)n * n_sigma);
+            ms_mixed_solve(A->op, n, n_sigma, (const c128 *)sigma, s, db.p, dX.p, ko, method, inner_tolerance,
+                           max_outer, relres, ms, st);
+            dX.download((c128 *)X, st);
+        }
+        if (stats) {
+            double mx = 0.0;
+            for (int k = 0; k < n_sigma; k++) mx = std::max(mx, relres[k]);
+            stats->flag = ms.flag;
+            stats->iterations = ms.iterations;
+            stats->relres = mx;
+            stats->solve_ms = ms.solve_ms;
+            stats->spmv_ms = 0.0;
+            stats->spmv_launches = 0;
+            stats->reserved = 0;
+        }
+        if (outer_steps) *outer_steps = ms.outer;
+        if (corrections) *corrections = ms.corrections;
+    });
+}
+
 int idrs_crs_transpose(idrs_user_matrix_t K, idrs_user_matrix_t *KT) {
     return guarded([&] {
         need(K && KT, "null argument");

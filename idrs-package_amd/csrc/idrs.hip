@@ -884,5 +884,6 @@ template int idrs_solve<c64>(const Operator &, long, int, const c64 *, c64 *, co
 template void make_p_device<double>(long, int, int, const double *, const double *, double *, unsigned long long,
                                     hipStream_t);
 template void make_p_device<c128>(long, int, int, const c128 *, const c128 *, c128 *, unsigned long long, hipStream_t);
+template void make_p_device<c64>(long, int, int, const c64 *, const c64 *, c64 *, unsigned long long, hipStream_t);
 
 }  // namespace idrs

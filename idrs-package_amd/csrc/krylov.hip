@@ -29,11 +29,20 @@ namespace {
 template <class T> struct HostT;
 template <> struct HostT<double> { using type = double; };
 template <> struct HostT<c128> { using type = std::complex<double>; };
+// complex64 vectors (the multishift solvers' single-precision form) keep
+// FP64 host scalars: the small recurrences, omegas and norms run in FP64
+template <> struct HostT<c64> { using type = std::complex<double>; };
 
-inline double to_dev(double v) { return v; }
-inline c128 to_dev(std::complex<double> v) { return c128(v.real(), v.imag()); }
+// host scalar -> device scalar of the vectors' type
+template <class T, class H>
+inline T to_devT(H v) {
+    const std::complex<double> z(v);
+    if constexpr (Tr<T>::is_complex) return T((typename Tr<T>::real)z.real(), (typename Tr<T>::real)z.imag());
+    else return (T)z.real();
+}
 inline double to_host(double v) { return v; }
 inline std::complex<double> to_host(c128 v) { return {v.re, v.im}; }
+inline std::complex<double> to_host(c64 v) { return {(double)v.re, (double)v.im}; }
 inline double conjh(double v) { return v; }
 inline std::complex<double> conjh(std::complex<double> v) { return std::conj(v); }
 inline double absh(double v) { return std::fabs(v); }
@@ -52,6 +61,7 @@ constexpr int COEF_LDS = MAXS * (PDOT_MAX + 4);
 
 __device__ inline double kdiv(double a, double b) { return a / b; }
 __device__ inline c128 kdiv(c128 a, c128 b) { return a / b; }
+__device__ inline c64 kdiv(c64 a, c64 b) { return a / b; }
 template <class T>
 __device__ inline T kfma(T a, T b, T c) { return Tr<T>::fma(a, b, c); }
 
@@ -181,6 +191,7 @@ __global__ void k_div_real(long n, T *w, double c) {
 
 __device__ inline double real_part(double v) { return v; }
 __device__ inline double real_part(c128 v) { return v.re; }
+__device__ inline double real_part(c64 v) { return (double)v.re; }
 
 // |w|^2 = the sum of the partials pin (reduced in every workgroup in one
 // fixed order, so all hold the same value), then w = w / sqrt(|w|^2) with
@@ -489,14 +500,14 @@ struct Work {
         if (c.size() > NCOEF) throw std::invalid_argument("too many coefficients");
         T *h = hcoef + (size_t)cslot * NCOEF;
         cslot ^= 1;
-        for (size_t i = 0; i < c.size(); i++) h[i] = to_dev(c[i]);
+        for (size_t i = 0; i < c.size(); i++) h[i] = to_devT<T>(c[i]);
         HIP_TRY(hipMemcpyAsync(dcoef, h, sizeof(T) * c.size(), hipMemcpyHostToDevice, st));
     }
     // coefficients for an update kernel: in ca when they fit (nullptr
     // returned), else in device memory (its pointer returned)
     const T *coefs(const std::vector<H> &c, CoefArg<T> &ca) {
         if (c.size() <= (size_t)CoefArg<T>::CAP) {
-            for (size_t i = 0; i < c.size(); i++) ca.c[i] = to_dev(c[i]);
+            for (size_t i = 0; i < c.size(); i++) ca.c[i] = to_devT<T>(c[i]);
             return nullptr;
         }
         put_coef(c);
@@ -517,7 +528,7 @@ struct Work {
             }
             while (t < MAXT && done < src.size()) {
                 c.src[t] = src[done];
-                c.cof[t] = to_dev(cof[done]);
+                c.cof[t] = to_devT<T>(cof[done]);
                 t++;
                 done++;
             }
@@ -567,10 +578,10 @@ struct Work {
         return get(prow, gv);
     }
     std::vector<H> axpy_pdot(T *gn, const T *v, const T *t, H om, const T *P, int prow) {
-#define AP(NV) hipLaunchKernelGGL((k_axpy_pdot<T, NV>), dim3(gv), dim3(256), 0, st, n, gn, v, t, to_dev(om), P, prow, part, gv)
+#define AP(NV) hipLaunchKernelGGL((k_axpy_pdot<T, NV>), dim3(gv), dim3(256), 0, st, n, gn, v, t, to_devT<T>(om), P, prow, part, gv)
         if (prow > PDOT_MAX) {
             // gn here, P^T gn from the chunked kernel
-            hipLaunchKernelGGL((k_axpy_pdot<T, 1>), dim3(gv), dim3(256), 0, st, n, gn, v, t, to_dev(om), P, 0, part, gv);
+            hipLaunchKernelGGL((k_axpy_pdot<T, 1>), dim3(gv), dim3(256), 0, st, n, gn, v, t, to_devT<T>(om), P, 0, part, gv);
             LAUNCH_CHECK();
             launch_pdot_wide<T>(n, gn, P, prow, 0, part, 0, nullptr, st);
             return get(prow, gv);
@@ -704,7 +715,7 @@ void shifted_residuals(const Operator &A, long n, int ns, const std::vector<type
         op_apply_cols<T>(A, m, n, src, Tb, W1, W2, w.st);
         for (int j = 0; j < m; j++) {
             hipLaunchKernelGGL((k_shift_res<T>), dim3(w.gv), dim3(256), 0, w.st, n, b, Tb + n * j, X + n * (k0 + j),
-                               to_dev(sigma[k0 + j]), w.part + (long)j * w.gv);
+                               to_devT<T>(sigma[k0 + j]), w.part + (long)j * w.gv);
             LAUNCH_CHECK();
         }
         const std::vector<typename HostT<T>::type> r2 = w.get(m, w.gv);
@@ -922,7 +933,7 @@ void msidrs_solve(const Operator &A, long n, int ns, const T *sigma_d, int s, co
             w.copy(xk, tmp);
         }
     if (o.want_colfac && o.colfac)
-        for (int ks = 0; ks < ns; ks++) o.colfac[ks] = to_dev(phi[ks]);
+        for (int ks = 0; ks < ns; ks++) o.colfac[ks] = to_devT<T>(phi[ks]);
     double mx = 0.0;
     for (double q : normr) mx = std::max(mx, q);
     if (info == 0 && mx >= tolb) info = 2;                                // :330
@@ -1020,7 +1031,7 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
         std::vector<T> sig0;
         if (multishift && in_it > 0) {
             sig0.resize(ns + 1);
-            for (int k = 0; k < ns; k++) sig0[k] = to_dev(sigma[k]);
+            for (int k = 0; k < ns; k++) sig0[k] = to_devT<T>(sigma[k]);
             sig0[ns] = T(0.0);
         }
         while (info < 0) {
@@ -1101,7 +1112,7 @@ static void qmr_driver(const Operator &A, long n, int ns, const std::vector<type
                     }
                     jj = jj + 1;
                 }
-                hipLaunchKernelGGL((k_sub_div<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, v, to_dev(om));  // :194
+                hipLaunchKernelGGL((k_sub_div<T>), dim3(w.gv), dim3(256), 0, stream, n, gn, v, to_devT<T>(om));  // :194
                 LAUNCH_CHECK();
                 for (int i = 0; i < s + 2; i++) h[i] = c[i] / om;         // :197
                 // The step's scalars stay on the device until one host read:
@@ -1279,6 +1290,7 @@ void msqmridr_solve(const Operator &A, long n, int ns, const T *sigma, int s, co
                                     const KrylovOpts<T> &, KrylovStats &, hipStream_t);
 KINST(double)
 KINST(c128)
+KINST(c64)
 #undef KINST
 
 }  // namespace idrs

@@ -233,6 +233,171 @@ void mixed_solve(const Operator &A, long n, int s, const T64 *b, T64 *x, const S
     st.solve_ms = ms;
 }
 
+// r = b - (t - sigma x), partials of |r|^2 (the shifted system's FP64 residual)
+__global__ __launch_bounds__(256) void k_ms_res(long n, const c128 *__restrict__ b, const c128 *__restrict__ t,
+                                                const c128 *__restrict__ x, c128 sigma, c128 *__restrict__ r,
+                                                c128 *partials) {
+    __shared__ c128 lds[4];
+    c128 e[1] = {c128(0)};
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        const c128 rq = b[q] - (t[q] - sigma * x[q]);
+        r[q] = rq;
+        e[0] = e[0] + c128(Tr<c128>::abs2(rq));
+    }
+    block_sum<c128, 1>(e, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = e[0];
+}
+
+void ms_mixed_solve(const Operator &A, long n, int ns, const c128 *sigma, int s, const c128 *b, c128 *X,
+                    const KrylovOpts<c128> &o, int method, double inner_tol, int max_outer, double *relres,
+                    MsMixedStats &st, hipStream_t stream) {
+    if (n < 1 || ns < 1) throw std::invalid_argument("empty system");
+    if (A.K->nrows != n || A.K->ncols != n) throw std::invalid_argument("matrix is not n x n");
+    if (A.general()) throw std::invalid_argument("mixed precision takes standard operators only");
+    if (A.values_complex()) throw std::invalid_argument("mixed precision needs real matrix values");
+    if (A.seed != std::complex<double>(0.0, 0.0)) throw std::invalid_argument("mixed precision without a seed shift");
+    if (o.tol < 0) throw std::invalid_argument("Illegal value for parameter tolerance");
+    if (!(inner_tol > 0 && inner_tol < 1)) throw std::invalid_argument("inner tolerance must be in (0, 1)");
+    if (max_outer < 0) throw std::invalid_argument("max_outer must be >= 0");
+    if (method != 0 && method != 1) throw std::invalid_argument("method: 0 MSQMRIDR, 1 MSIDRS");
+    if (o.has_m1 || o.x0 || o.resvec || o.omega || o.want_colfac)
+        throw std::invalid_argument("M1, x0, omega, resvec and colfac are not offered");
+    const int maxit = o.has_maxit ? o.maxit : (int)std::min<long>(2 * n, 1000);
+    op_make_lowprec(const_cast<Operator &>(A));
+    const size_t v64 = sizeof(c128) * (size_t)n, v32 = sizeof(c64) * (size_t)n;
+    const int gv = vec_grid(n);
+    Dev br(v64), bt(v64 * ns), bw1(v64 * ns), bw2(v64 * ns), bb32(v32), bX32(v32 * ns), bd32(v32),
+        bpart(sizeof(c128) * MAX_PARTS), bsum(sizeof(c128));
+    c128 *r = br.as<c128>(), *part = bpart.as<c128>();
+    c64 *b32 = bb32.as<c64>(), *X32 = bX32.as<c64>(), *d32 = bd32.as<c64>();
+    auto reduce = [&]() {
+        hipLaunchKernelGGL((k_sum<c128>), dim3(1), dim3(FIN_THREADS), 0, stream, (const c128 *)part, gv, bsum.as<c128>());
+        LAUNCH_CHECK();
+        c128 h;
+        HIP_TRY(hipMemcpyAsync(&h, bsum.p, sizeof(c128), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return std::sqrt(h.re);
+    };
+    // the complex64 solver's options: one solve of all shifts, or of one
+    auto lo_opts = [&](double tol, int cap) {
+        KrylovOpts<c64> io;
+        io.tol = tol;
+        io.has_maxit = true;
+        io.maxit = cap;
+        io.variant = o.variant;
+        io.has_inner_s = o.has_inner_s;
+        io.inner_s = o.inner_s;
+        io.has_inner_tol = o.has_inner_tol;
+        io.inner_tol = o.inner_tol;
+        io.has_inner_it = o.has_inner_it;
+        io.inner_it = o.inner_it;
+        io.p_seed = o.p_seed;
+        return io;
+    };
+    auto run_lo = [&](int nsh, const c64 *sig32, const c64 *rhs, c64 *out, double tol, int cap) {
+        KrylovStats is;
+        const KrylovOpts<c64> io = lo_opts(tol, cap);
+        if (method == 0) msqmridr_solve<c64>(A, n, nsh, sig32, s, rhs, out, io, is, stream);
+        else msidrs_solve<c64>(A, n, nsh, sig32, s, rhs, out, io, is, stream);
+        return is;
+    };
+    std::vector<c64> sig32(ns);
+    for (int k = 0; k < ns; k++) sig32[k] = c64((float)sigma[k].re, (float)sigma[k].im);
+
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, stream));
+    // ||b|| in FP64, b/||b|| in complex64
+    hipLaunchKernelGGL((k_res<c128>), dim3(gv), dim3(256), 0, stream, n, b, (const c128 *)nullptr, r, part);
+    LAUNCH_CHECK();
+    const double normb = reduce();
+    if (normb == 0.0) throw std::invalid_argument("zero right-hand side");
+    const double tolb = o.tol * normb;
+    hipLaunchKernelGGL((k_to_lo<c128, c64>), dim3(gv), dim3(256), 0, stream, n, b, 1.0 / normb, b32);
+    LAUNCH_CHECK();
+    // every shift at once in complex64
+    KrylovStats s1 = run_lo(ns, sig32.data(), b32, X32, std::max(inner_tol, o.tol), maxit);
+    int its = s1.iterations;
+    HIP_TRY(hipMemsetAsync(X, 0, v64 * ns, stream));
+    for (int k = 0; k < ns; k++) {
+        hipLaunchKernelGGL((k_add_hi<c128, c64>), dim3(gv), dim3(256), 0, stream, n, X + (long)n * k, normb,
+                           (const c64 *)(X32 + (long)n * k));
+        LAUNCH_CHECK();
+    }
+    // FP64 residuals of every shift (block products of A on X)
+    c128 *T = bt.as<c128>(), *W1 = bw1.as<c128>(), *W2 = bw2.as<c128>();
+    std::vector<double> nr(ns);
+    auto residual = [&](int k) {
+        hipLaunchKernelGGL(k_ms_res, dim3(gv), dim3(256), 0, stream, n, b, (const c128 *)(T + (long)n * k),
+                           (const c128 *)(X + (long)n * k), sigma[k], r, part);
+        LAUNCH_CHECK();
+        return reduce();
+    };
+    op_apply_cols<c128>(A, ns, n, X, T, W1, W2, stream);
+    for (int k = 0; k < ns; k++) {
+        nr[k] = residual(k);
+        if (!(nr[k] <= normb)) {   // the complex64 solve broke down or diverged on this shift: restart it from zero
+            HIP_TRY(hipMemsetAsync(X + (long)n * k, 0, v64, stream));
+            nr[k] = normb;
+        }
+    }
+    int outer = 0, corr = 0;
+    bool stuck = false;
+    while (!stuck) {
+        bool all = true;
+        for (int k = 0; k < ns; k++) all = all && nr[k] <= tolb;
+        if (all || outer >= max_outer || its >= maxit) break;
+        outer++;
+        stuck = true;
+        for (int k = 0; k < ns; k++) {
+            if (nr[k] <= tolb || its >= maxit) continue;
+            // correction of shift k in complex64: (A - sigma_k I) d = r_k / ||r_k||
+            op_apply<c128>(A, X + (long)n * k, T + (long)n * k, W1, W2, EPI_NONE, EpiArgs<c128>{}, nullptr, stream);
+            const double before = residual(k);
+            hipLaunchKernelGGL((k_to_lo<c128, c64>), dim3(gv), dim3(256), 0, stream, n, (const c128 *)r, 1.0 / before,
+                               b32);
+            LAUNCH_CHECK();
+            const double itol = std::max(inner_tol, 0.5 * tolb / before);
+            KrylovStats sc = run_lo(1, &sig32[k], b32, d32, itol, maxit - its);
+            its += sc.iterations;
+            corr++;
+            hipLaunchKernelGGL((k_add_hi<c128, c64>), dim3(gv), dim3(256), 0, stream, n, X + (long)n * k, before,
+                               (const c64 *)d32);
+            LAUNCH_CHECK();
+            op_apply<c128>(A, X + (long)n * k, T + (long)n * k, W1, W2, EPI_NONE, EpiArgs<c128>{}, nullptr, stream);
+            const double after = residual(k);
+            if (after < before) {
+                stuck = false;
+                nr[k] = after;
+            } else {                          // the correction diverged: take it back
+                hipLaunchKernelGGL((k_add_hi<c128, c64>), dim3(gv), dim3(256), 0, stream, n, X + (long)n * k, -before,
+                                   (const c64 *)d32);
+                LAUNCH_CHECK();
+                op_apply<c128>(A, X + (long)n * k, T + (long)n * k, W1, W2, EPI_NONE, EpiArgs<c128>{}, nullptr,
+                               stream);
+                nr[k] = residual(k);
+            }
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    bool all = true;
+    for (int k = 0; k < ns; k++) {
+        relres[k] = nr[k] / normb;
+        all = all && nr[k] <= tolb;
+    }
+    st.flag = all ? 0 : (stuck ? 3 : 1);
+    st.iterations = its;
+    st.outer = outer;
+    st.corrections = corr;
+    st.solve_ms = ms;
+}
+
 template void mixed_solve<double>(const Operator &, long, int, const double *, double *, const SolveOpts<double> &,
                                   double, int, bool, MixedStats &, hipStream_t);
 template void mixed_solve<c128>(const Operator &, long, int, const c128 *, c128 *, const SolveOpts<c128> &, double,

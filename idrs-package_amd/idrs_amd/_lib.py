@@ -185,6 +185,9 @@ _PROTOS = {
                                  c_void_p, c_void_p, POINTER(idrs_stats_t)]),
     "idrs_msqmridr_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
                                   c_void_p, c_void_p, POINTER(idrs_stats_t)]),
+    "idrs_ms_mixed_c128": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
+                                   c_double, c_int, c_void_p, c_void_p, POINTER(idrs_stats_t), POINTER(c_int),
+                                   POINTER(c_int)]),
     "idrs_msqmridr_c128": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(idrs_krylov_opts_t),
                                    c_void_p, c_void_p, POINTER(idrs_stats_t)]),
     "idrs_idrs_mixed_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, POINTER(idrs_opts_t), c_double, c_int, c_int,

@@ -852,6 +852,50 @@ def msqmridr(A, b, sigma, s, M1=None, tolerance=None, maximum_iterations=None, i
                    out=out)
 
 
+MsMixedResult = namedtuple("MsMixedResult", "x flag relres iterations outer corrections solve_ms")
+
+
+def ms_mixed(A, b, sigma, s, solver="msqmridr", tolerance=None, maximum_iterations=None, variant=None,
+             inner_s=None, inner_iterations=None, inner_tolerance_ms=None, inner_tolerance=1e-5, max_outer=10,
+             p_seed=None):
+    """Multishift solve in complex64 with FP64 refinement (no reference
+    counterpart; idrs_ms_mixed_c128): MSQMRIDR or MSIDRS with complex64
+    vectors and FP32 values for every shift to inner_tolerance, then FP64
+    residuals and single-shift complex64 corrections until every shift meets
+    the FP64 tolerance.  inner_s / inner_iterations / inner_tolerance_ms are
+    MSQMRIDR's own inner-solver options.  Returns MsMixedResult with X
+    (n, n_sigma) and each shift's FP64 true residual."""
+    lib = _lib.load()
+    n = int(len(b))
+    sig = np.atleast_1d(np.asarray(sigma, dtype=np.complex128))
+    o = idrs_krylov_opts_t()
+    if tolerance is not None:
+        o.has_tolerance, o.tolerance = 1, float(tolerance)
+    if maximum_iterations is not None:
+        o.has_maximum_iterations, o.maximum_iterations = 1, int(maximum_iterations)
+    if variant is not None:
+        o.has_variant, o.variant = 1, int(variant)
+    if inner_s is not None:
+        o.has_inner_s, o.inner_s = 1, int(inner_s)
+    if inner_iterations is not None:
+        o.has_inner_iterations, o.inner_iterations = 1, int(inner_iterations)
+    if inner_tolerance_ms is not None:
+        o.has_inner_tolerance, o.inner_tolerance = 1, float(inner_tolerance_ms)
+    o.p_seed = int(p_seed) if p_seed is not None else 0
+    bb = np.ascontiguousarray(b, dtype=np.complex128)
+    sp = np.empty(2 * len(sig))
+    sp[0::2], sp[1::2] = sig.real, sig.imag
+    X = np.empty((len(sig), n), dtype=np.complex128)     # row k = column k of X(n, n_sigma)
+    rr = np.zeros(len(sig))
+    st = idrs_stats_t()
+    outer, corr = ctypes.c_int(), ctypes.c_int()
+    method = {"msqmridr": 0, "msidrs": 1}[solver]
+    check(lib.idrs_ms_mixed_c128(A.handle, method, n, _ptr(bb), len(sig), _ptr(sp), int(s), ctypes.byref(o),
+                                 float(inner_tolerance), int(max_outer), _ptr(X), _ptr(rr), ctypes.byref(st),
+                                 ctypes.byref(outer), ctypes.byref(corr)))
+    return MsMixedResult(X.T, st.flag, rr, st.iterations, outer.value, corr.value, st.solve_ms)
+
+
 def batch_format(mats):
     """Block-diagonal batch of square CRS/COO user matrices (copies).  Vectors
     of the batch are the systems' vectors concatenated in order."""

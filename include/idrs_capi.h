@@ -480,6 +480,24 @@ int idrs_idrs_mixed_c128(idrs_matrix_t A, int n, const double *b, int s, const i
                          double inner_tolerance, int max_outer, int fp32_vectors, double *x, idrs_stats_t *stats,
                          int *outer_steps);
 
+/* X(:,k) = (A - sigma_k I)^-1 b for every shift, the multishift solvers in
+   complex64 with FP64 refinement: MSQMRIDR (method 0) or MSIDRS (method 1)
+   with complex64 vectors and FP32 matrix values, every shift at once, to
+   inner_tolerance; then each shift's FP64 residual r_k = b - (A - sigma_k I)
+   x_k, and for a shift above opts->tolerance complex64 corrections
+   (A - sigma_k I) d = r_k/||r_k|| by the same solver over that shift,
+   x_k += ||r_k|| d (at most max_outer rounds).  Host scalars (the small
+   recurrences, omegas, norms) stay FP64.  sigma: (re, im) pairs; X (n,
+   n_sigma) column-major; relres (host) each shift's FP64 true residual;
+   stats->iterations: all complex64 iterations; *outer_steps: FP64 residual
+   rounds; *corrections: single-shift correction solves.  s, variant, inner
+   solver and device_pointers as for idrs_msqmridr_c128 / idrs_msidrs_c128
+   (M1, x0, omega, resvec, colfac and PT_raw rejected); real matrix values,
+   no seed shift. */
+int idrs_ms_mixed_c128(idrs_matrix_t A, int method, int n, const double *b, int n_sigma, const double *sigma, int s,
+                       const idrs_krylov_opts_t *opts, double inner_tolerance, int max_outer, double *X,
+                       double *relres, idrs_stats_t *stats, int *outer_steps, int *corrections);
+
 /* ---------------------------------------------------------------------
  * differentiable solves (no counterpart in the reference): the adjoint
  * system A^T lambda = dL/dx is solved by IDRS on the transposed matrix, and
