@@ -262,3 +262,109 @@ def test_wedge_sommerfeld_multishift_block(gpu, solver, deflation):
         S = Kd + sg * Cd + sg ** 2 * Md
         res = [np.linalg.norm(b - S @ Y[n:, k]) / np.linalg.norm(b) for Y in (X, Xo)]
         assert res[0] <= max(1e-5, 10 * res[1]), (k, res)
+
+
+def _room(grid):
+    """EXAMPLES/STENCIL/user_module.f90 laplace_matrix / damping_matrix /
+    point_source on one image, assembled as COO: 16 2^(grid-1) points per
+    direction on a 4 m cube, the 7-point stencil with Sommerfeld walls
+    (the ghost value mirrors the second node, v(0) = v(2), so a wall node's
+    outer coefficient lands on its inner neighbour -- kept as a separate
+    entry, in the stencil's term order c, w, e, s, n, f, b), the damping
+    matrix (2/h) rp on wall nodes (rp = 1; the faces assign, not add), and
+    the point source 1/(hx hy hz) at node Nn/2."""
+    N = 16 * 2 ** (grid - 1)
+    h = 4.0 / (N - 1)
+    c, o = 2 / h ** 2 * 3, -1 / h ** 2
+    idx = lambda i, j, k: i + N * (j + N * k)
+    I, J, V = [], [], []
+    for k in range(N):
+        for j in range(N):
+            for i in range(N):
+                r = idx(i, j, k)
+                terms = [(idx(i, j, k), c)]
+                for (a, lim, mk) in ((i, N, lambda t: idx(t, j, k)), (j, N, lambda t: idx(i, t, k)),
+                                     (k, N, lambda t: idx(i, j, t))):
+                    lo = a - 1 if a > 0 else a + 1          # ghost mirrors the second node
+                    hi = a + 1 if a < lim - 1 else a - 1
+                    terms += [(mk(lo), o), (mk(hi), o)]
+                for col, v in terms:
+                    I.append(r + 1)
+                    J.append(col + 1)
+                    V.append(v)
+    n = N ** 3
+    K = (np.array(I, np.int32), np.array(J, np.int32), np.array(V), n)
+    wall = np.zeros(n, bool)
+    g = np.arange(N)
+    ii, jj, kk = np.meshgrid(g, g, g, indexing="ij")
+    wall[idx(ii, jj, kk).ravel()] = ((ii == 0) | (ii == N - 1) | (jj == 0) | (jj == N - 1) | (kk == 0)
+                                     | (kk == N - 1)).ravel()
+    w = np.nonzero(wall)[0]
+    C = ((w + 1).astype(np.int32), (w + 1).astype(np.int32), np.full(len(w), 2.0 / h), n)
+    b = np.zeros(n, complex)
+    b[n // 2 - 1] = 1.0 / h ** 3
+    return K, C, b
+
+
+@pytest.mark.parametrize("deflation", [False, True])
+def test_room_single_frequency(gpu, deflation):
+    """room.f90 without -multi: (K + sigma C + sigma^2 I) x = b at
+    sigma = 2 pi 100 i / 340 (grid 1, 16^3), pp_idrs(K, b, C=C, seed=sigma[, V])
+    with no preconditioner: IDR(4) on Pmul(A, b), tol 1e-6; with
+    -deflation V = 1/sqrt(n)."""
+    K, C, b = _room(1)
+    n = K[3]
+    sigma = 1j * 2 * np.pi * 100.0 / 340.0
+    pr = Pair(K, True, seed=sigma, C=C)
+    if deflation:
+        V = np.full((n, 1), 1 / np.sqrt(n), complex)
+        pr.A.make_deflation(V)
+        pr.O.make_deflation(V)
+    kw = dict(tolerance=1e-6, maximum_iterations=1000)
+    g = idrs_amd.idrs(pr.A, pr.A.pmul(b), 4, **kw)
+    o = orc.idrs(pr.O, pr.O.pmul(b), 4, **kw)
+    print(f"room deflation={deflation}: gpu {g.iterations} oracle {o[3]}", flush=True)
+    assert g.flag == o[1] == 0
+    run = lambda bb: (lambda q: (q[0], q[3]))(orc.idrs(pr.O, pr.O.pmul(bb), 4, **kw))
+    assert _count_ok(g.iterations, o[3], run, b)
+    x, xo = idrs_amd.scaleback(pr.A, b, g.x), pr.O.scaleback_b(b, o[0])
+    Kd = np.zeros((n, n))
+    np.add.at(Kd, (K[0] - 1, K[1] - 1), K[2])
+    Cd = np.zeros(n)
+    Cd[C[0] - 1] = C[2]
+    res = [np.linalg.norm(b - (Kd @ y + sigma * (Cd * y) + sigma ** 2 * y)) / np.linalg.norm(b) for y in (x, xo)]
+    assert res[0] <= max(1e-5, 10 * res[1]), res
+
+
+def test_room_multishift_block(gpu):
+    """room.f90 -multi -deflation at grid 2 (32^3, block system of 65536
+    rows): [-C -K; I 0] - sigma I (no mass matrix), shifts
+    2 pi f i / 340 for f = 100, 200, seed at the last, V = [0; 1/sqrt(n)],
+    MSIDRS(4) on Pmul(A, b) with b = [-rhs; 0], scaleback; each frequency's
+    true residual of K + sigma C + sigma^2 I."""
+    K, C, rhs = _room(2)
+    n = K[3]
+    sigma = 1j * 2 * np.pi * np.array([100.0, 200.0]) / 340.0
+    seed = sigma[-1]
+    pr = Pair(K, True, seed=seed, C=C, block=True, multishift=True)
+    V = np.zeros((2 * n, 1), complex)
+    V[n:, 0] = 1 / np.sqrt(n)
+    pr.A.make_deflation(V)
+    pr.O.make_deflation(V)
+    bb = np.concatenate([-rhs, np.zeros(n)])
+    kw = dict(tolerance=1e-6, maximum_iterations=1000)
+    g = idrs_amd.msidrs(pr.A, pr.A.pmul(bb), sigma - seed, 4, **kw)
+    o = orc.msidrs(pr.O, pr.O.pmul(bb), sigma - seed, 4, **kw)
+    print(f"room multishift: gpu {g.iterations} oracle {o['iterations']}", flush=True)
+    assert g.flag == o["flag"] == 0
+    run = lambda b1: (lambda q: (q["x"], q["iterations"]))(orc.msidrs(pr.O, pr.O.pmul(b1), sigma - seed, 4, **kw))
+    assert _count_ok(g.iterations, o["iterations"], run, bb)
+    X = idrs_amd.ms_scaleback_b(pr.A, sigma, np.ones((1, 2)), bb, np.asarray(g.x))
+    import scipy.sparse as sps
+    Ks = sps.csr_matrix((K[2], (K[0] - 1, K[1] - 1)), shape=(n, n))
+    Cd = np.zeros(n)
+    Cd[C[0] - 1] = C[2]
+    for k, s in enumerate(sigma):
+        x = X[n:, k]
+        r = np.linalg.norm(rhs - (Ks @ x + s * (Cd * x) + s * s * x)) / np.linalg.norm(rhs)
+        assert r < 1e-5, (k, r)
