@@ -11,6 +11,7 @@ program drop_in
    use matrix_module
    use idrs_module
    use ritz_module
+   use pp_idrs_module
 
    implicit none
 
@@ -30,6 +31,7 @@ program drop_in
       call wedge_sommerfeld_multi( trim(data_dir) )
       call wedge_neumann_lu_multi( trim(data_dir) )
       call wedge_sommerfeld_lu_multi( trim(data_dir) )
+      call pp_idrs_chains( trim(data_dir) )
    end if
 
 contains
@@ -260,6 +262,75 @@ contains
       call user_matrix_free( M )
       call user_matrix_free( K )
    end subroutine wedge_sommerfeld_lu_multi
+
+! The same wedge chains through pp_idrs (PP_IDRS/pp_idrs_module.f90 with
+! its default settings: IDR(4), tol 1e-6, maxit 1000): the Neumann single
+! frequency with deflation (RPP_IDRS), the Sommerfeld multishift block
+! system with the lumped mass preconditioner and deflation (CPP_MS_IDRS),
+! and with the band LU shift-and-invert preconditioner (CPP_MS_IDRS with
+! shift).  err: each frequency's true relative residual, recomputed here.
+   subroutine pp_idrs_chains( dir )
+      character(len=*), intent(in)  :: dir
+      type(user_matrix)             :: K, M, C, RHS, P
+      real(kind=rp), allocatable    :: b(:), x(:), V(:,:)
+      complex(kind=cp), allocatable :: rhs_c(:), sol(:,:), Vc(:,:)
+      complex(kind=cp)              :: sigma(3), cshift
+      real(kind=rp)                 :: rsig, freq(3)
+      real(kind=rp), parameter      :: pi = 3.141592653589793_rp
+      integer                       :: n, i
+      character(len=32)             :: name
+
+      call pp_idrs_settings( verbose = .false. )
+      K = mm_matrix( dir//'/wedge3_K.mtx' )
+      M = mm_matrix( dir//'/wedge3_M.mtx' )
+      RHS = mm_matrix( dir//'/wedge3_b.mtx' )
+      n = K%nrows
+      allocate( b(n), x(n), V(n,1) )
+      b = RHS%rmat(:,1)
+      rsig = (2._rp*pi)**2
+      V = 1._rp
+      V(:,1) = M*V(:,1)
+      x = pp_idrs( K, b, M, P=P, seed=rsig, V=V )
+      write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', 'pp_wedge3_neumann', ' iter', 0, ' flag', 0, &
+         ' relres', 0._rp, ' err', norm( b - K*x + rsig*(M*x) )/norm( b )
+      call user_matrix_free( M )
+      call user_matrix_free( K )
+
+      K = mm_matrix( dir//'/wedge2_K.mtx' )
+      M = mm_matrix( dir//'/wedge2_M.mtx' )
+      C = mm_matrix( dir//'/wedge2_C.mtx' )
+      RHS = mm_matrix( dir//'/wedge2_b.mtx' )
+      n = K%nrows
+      allocate( rhs_c(n), sol(n,3), Vc(2*n,1) )
+      rhs_c = RHS%rmat(:,1)
+      freq = [1._rp, 2._rp, 4._rp]
+      sigma = (2._cp*pi*freq)*(0._cp, 1._cp)
+      Vc = (0._cp, 0._cp)
+      Vc(n+1:,1) = 1._cp/sqrt(real(n, kind=cp))
+      P = complex_precon( 1, M )
+      sol = pp_idrs( K, rhs_c, sigma, M, C, P, seed=sigma(3), V=Vc )
+      do i = 1, 3
+         write(name,'(a,i0)') 'pp_wedge2_multi_f', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', 0, ' flag', 0, &
+            ' relres', 0._rp, ' err', norm( rhs_c - K*sol(:,i) - sigma(i)*(C*sol(:,i)) - sigma(i)**2*(M*sol(:,i)) ) &
+            /norm( rhs_c )
+      end do
+      call user_matrix_free( P )
+      cshift = 2._cp*pi/(freq(3)+freq(1))*cmplx((freq(3)-freq(1))*sqrt(freq(1)*freq(3)), 2._rp*freq(1)*freq(3), &
+               kind=cp)
+      P = complex_precon( 3, K, M, C, shift=cshift )
+      sol = pp_idrs( K, rhs_c, sigma, M, C, P, seed=sigma(3), shift=cshift )
+      do i = 1, 3
+         write(name,'(a,i0)') 'pp_wedge2_lu_f', i
+         write(*,'(a,1x,a,a,i6,a,i3,a,es12.4,a,es12.4)') 'CASE', trim(name), ' iter', 0, ' flag', 0, &
+            ' relres', 0._rp, ' err', norm( rhs_c - K*sol(:,i) - sigma(i)*(C*sol(:,i)) - sigma(i)**2*(M*sol(:,i)) ) &
+            /norm( rhs_c )
+      end do
+      call user_matrix_free( P )
+      call user_matrix_free( C )
+      call user_matrix_free( M )
+      call user_matrix_free( K )
+   end subroutine pp_idrs_chains
 
 ! 1-D convection-diffusion, N=60 (EXAMPLES/TOEPLITZ/toeplitz.f90:111-165),
 ! stored in CRS; positional call as in the example.

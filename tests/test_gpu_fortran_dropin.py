@@ -258,3 +258,17 @@ def test_wedge_shift_invert_multishift(cases, case, k, nf):
         assert c["flag"] == 0 and c["err"] < 1e-5, c
     # shift-and-invert brings the wedge's ~450 multishift iterations to ~50
     assert its.pop() < 120
+
+
+def test_pp_idrs_wedge_chains(cases):
+    """pp_idrs (the PP_IDRS driver with its default settings) on the wedge
+    examples: the Neumann single frequency with deflation, the Sommerfeld
+    multishift block system with the lumped mass preconditioner and
+    deflation, and with the band LU shift-and-invert preconditioner; every
+    frequency's true residual at the default tolerance 1e-6 (the chains'
+    steps are the ones the hand-written cases above check against the
+    oracle)."""
+    assert cases["pp_wedge3_neumann"]["err"] < 1e-5
+    for i in range(3):
+        assert cases[f"pp_wedge2_multi_f{i + 1}"]["err"] < 1e-5
+        assert cases[f"pp_wedge2_lu_f{i + 1}"]["err"] < 1e-5
