@@ -221,6 +221,99 @@ __global__ __launch_bounds__(NT) void k_lu_solve(long n, int dense, int hb, long
     }
 }
 
+// Wavefront helpers on doubles: lane 0's value in every lane (read first
+// lane: the whole wavefront is active), and a shift of the whole wavefront
+// by one lane towards lane 0 (DPP wave_shl:1, lane 63 receives 0).
+__device__ inline double lane0(double v) {
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+__device__ inline c128 lane0(c128 v) { return {lane0(v.re), lane0(v.im)}; }
+__device__ inline double shl1(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline c128 shl1(c128 v) { return {shl1(v.re), shl1(v.im)}; }
+
+// Band solve for a half bandwidth below 64 in the reference's column order
+// (band_LU_solve, user_module.f90:2181-2258), one wavefront: lane j holds
+// the running value of row i+j (forward) or i-j (back).  A step finishes
+// lane 0's row, gives it to every lane (read first lane), subtracts its
+// multiple of the pivot column in lanes 1..hb, and shifts the window by
+// one lane (DPP), a new row entering at lane hb.  Each row receives its
+// subtractions in the reference's order, each product rounded before the
+// difference, so the result equals the column sweep's bit for bit.  The
+// columns and entering values are loaded PF steps ahead.
+template <class T, class FT>
+__global__ __launch_bounds__(WAVE) void k_band_solve_wave(long n, int hb, long ld, const FT *__restrict__ F,
+                                                          const T *__restrict__ b, T *x) {
+#pragma clang fp contract(off)
+    constexpr int PF = 16;
+    const int j = threadIdx.x;
+    const bool act = j >= 1 && j <= hb;
+    // forward: column i of L is L(i+j, i) at (i+j)*ld + hb - j; b(i+1+hb) enters after step i
+    auto fcol = [&](long i) -> FT { return (act && i + j < n) ? F[(i + j) * ld + hb - j] : FT(0); };
+    auto fin = [&](long i) -> T { return (j == hb && i + 1 + hb < n) ? b[i + 1 + hb] : T(0); };
+    FT cq[PF];
+    T eq[PF];
+#pragma unroll
+    for (int t = 0; t < PF; t++) {
+        cq[t] = fcol(t);
+        eq[t] = fin(t);
+    }
+    T p = (j <= hb && j < n) ? b[j] : T(0);
+    for (long i0 = 0; i0 < n; i0 += PF) {
+#pragma unroll
+        for (int t = 0; t < PF; t++) {
+            const long i = i0 + t;
+            if (i < n) {
+                const T xi = lane0(p);
+                if (j == 0) x[i] = xi;
+                const T prod = xi * cq[t];
+                p = p - prod;
+                p = shl1(p);
+                if (j == hb) p = eq[t];
+                cq[t] = fcol(i + PF);
+                eq[t] = fin(i + PF);
+            }
+        }
+    }
+    __syncthreads();
+    // back: column k of U is U(k-j, k) at (k-j)*ld + hb + j, U(k,k) at k*ld + hb;
+    // y(k-1-hb) enters after step k
+    auto bcol = [&](long k) -> FT { return (act && k >= 0 && k - j >= 0) ? F[(k - j) * ld + hb + j] : FT(0); };
+    auto bdia = [&](long k) -> FT { return k >= 0 ? F[k * ld + hb] : FT(1); };
+    auto bin = [&](long k) -> T { return (j == hb && k - 1 - hb >= 0) ? x[k - 1 - hb] : T(0); };
+    FT dq[PF];
+#pragma unroll
+    for (int t = 0; t < PF; t++) {
+        cq[t] = bcol(n - 1 - t);
+        dq[t] = bdia(n - 1 - t);
+        eq[t] = bin(n - 1 - t);
+    }
+    p = (j <= hb && n - 1 - j >= 0) ? x[n - 1 - j] : T(0);
+    for (long k0 = n - 1; k0 >= 0; k0 -= PF) {
+#pragma unroll
+        for (int t = 0; t < PF; t++) {
+            const long k = k0 - t;
+            if (k >= 0) {
+                const T pk = lane0(p);
+                const T xk = pk / dq[t];
+                if (j == 0) x[k] = xk;
+                const T prod = xk * cq[t];
+                p = p - prod;
+                p = shl1(p);
+                if (j == hb) p = eq[t];
+                cq[t] = bcol(k - PF);
+                dq[t] = bdia(k - PF);
+                eq[t] = bin(k - PF);
+            }
+        }
+    }
+}
+
 int egrid(long n) { return (int)std::max(1L, std::min((n + LB - 1) / LB, 4096L)); }
 
 // get_band (user_module.f90:1833-1872) of a device matrix
@@ -380,6 +473,9 @@ void lu_solve(const LuFactor &F, const T *v, T *w, hipStream_t st) {
             if (dense)
                 hipLaunchKernelGGL((k_lu_solve<T, c128, 1024>), dim3(1), dim3(1024), 0, st, F.n, 1, F.hb, F.ld,
                                    (const c128 *)F.lu, v, w);
+            else if (F.hb < WAVE)
+                hipLaunchKernelGGL((k_band_solve_wave<T, c128>), dim3(1), dim3(WAVE), 0, st, F.n, F.hb, F.ld,
+                                   (const c128 *)F.lu, v, w);
             else
                 hipLaunchKernelGGL((k_lu_solve<T, c128, WAVE>), dim3(1), dim3(WAVE), 0, st, F.n, 0, F.hb, F.ld,
                                    (const c128 *)F.lu, v, w);
@@ -389,6 +485,9 @@ void lu_solve(const LuFactor &F, const T *v, T *w, hipStream_t st) {
     } else {
         if (dense)
             hipLaunchKernelGGL((k_lu_solve<T, double, 1024>), dim3(1), dim3(1024), 0, st, F.n, 1, F.hb, F.ld,
+                               (const double *)F.lu, v, w);
+        else if (F.hb < WAVE)
+            hipLaunchKernelGGL((k_band_solve_wave<T, double>), dim3(1), dim3(WAVE), 0, st, F.n, F.hb, F.ld,
                                (const double *)F.lu, v, w);
         else
             hipLaunchKernelGGL((k_lu_solve<T, double, WAVE>), dim3(1), dim3(WAVE), 0, st, F.n, 0, F.hb, F.ld,

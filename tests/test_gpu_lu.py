@@ -7,12 +7,17 @@ use them: EXAMPLES/RANDOM/band_real.f90 (band LU of the matrix itself) and
 the wedge problems with -lu (EXAMPLES/MATRIX_MARKET/wedge_neumann.f90,
 wedge_sommerfeld.f90 through PP_IDRS's RPP_MS_IDRS / CPP_MS_IDRS steps).
 
-Bars: the factors are formed in the reference's order on both sides; the
-solves sum each row in a different order, so preconditioner applications
-agree within 1e-12 of the largest entry (times the factor's growth on the
-random band example); solves: the oracle's iteration count within 2 or
-twice its own rounding spread, true residuals of every shifted system
-within ten times the oracle's (or 1e-5).
+Bars: the factors are formed in the reference's order on both sides.  The
+band solve of a half bandwidth below 64 sweeps columns in the reference's
+order, so on real vectors its result equals the oracle's bit for bit; the
+dense and wider band solves sum each row in another order, and complex
+products and quotients round differently from C's, so there preconditioner
+applications agree within 1e-12 of the largest entry (times the factor's
+growth on the random band example).  Solves: the oracle's iteration count
+within 2, or within twice the larger of the oracle's and the GPU's own
+rounding spread (on wedge_neumann -lu -multi the GPU alone takes 49, 51 or
+52 iterations when b moves by 1e-16), true residuals of every shifted
+system within ten times the oracle's (or 1e-5).
 """
 import numpy as np
 import pytest
@@ -74,6 +79,8 @@ def test_lu_precon_matches_oracle(gpu, cplx, which, case):
         assert lu[1] == (6 if "C" in case and sh is not None else 4)
     v = RNG.standard_normal(n) + (1j * RNG.standard_normal(n) if cplx else 0)
     assert _close(v / A, orc.lu_solve(lu, v))
+    if which == 3 and not cplx:
+        assert np.array_equal(v / A, orc.lu_solve(lu, v))
     # the operator's product divides by the factors first (real_mv's t = v/A)
     O = _om(K, cplx or np.iscomplexobj(sh), lu=lu)
     assert _close(A * v, O.matvec(v))
@@ -142,7 +149,7 @@ def test_random_band_real(gpu):
     lu = orc.precon_lu(3, O)
     assert lu[1] == hb
     b = O.user_mv(np.ones(n))
-    assert _close(b / A, orc.lu_solve(lu, b), 1e-11)
+    assert np.array_equal(b / A, orc.lu_solve(lu, b))
     Oa = orc.OracleMatrix(orc.CDS, n, cds=(hb, R), lu=lu)
     g = idrs_amd.idrs(A, b, 4, tolerance=1e-6, maximum_iterations=1000)
     o = orc.idrs(Oa, b, 4, tolerance=1e-6, maximum_iterations=1000)
@@ -166,12 +173,13 @@ def _wedge(k, cplx):
     return mats, mm_fixture(f"wedge{k}_b")["vals"][:, 0].copy()
 
 
-def _ms_check(g, o, run, b, res_g, res_o):
+def _ms_check(g, o, run, run_g, b, res_g, res_o):
     print(f"  gpu {g.iterations} flag {g.flag}, oracle {o['iterations']} flag {o['flag']}; "
           f"residuals {np.max(res_g):.2e} / {np.max(res_o):.2e}", flush=True)
     assert g.flag == o["flag"] == 0
     if abs(g.iterations - o["iterations"]) > 2:
-        assert abs(g.iterations - o["iterations"]) <= 2 * rounding_spread(run, b)[1]
+        spread = max(rounding_spread(run, b)[1], rounding_spread(run_g, b, trials=8)[1])
+        assert abs(g.iterations - o["iterations"]) <= 2 * spread
     for rg, ro in zip(res_g, res_o):
         assert rg <= max(1e-5, 10 * ro), (res_g, res_o)
 
@@ -209,7 +217,8 @@ def test_wedge_neumann_lu_multishift(gpu):
     res = lambda Z: [np.linalg.norm(b - Kd @ Z[:, k] + s * (Md @ Z[:, k])) / np.linalg.norm(b)
                      for k, s in enumerate(sigma)]
     run = lambda bb: (lambda q: (q["x"], q["iterations"]))(orc.msidrs(O, O.pmul(bb), eta_o, 4, **kw))
-    _ms_check(g, o, run, b, res(X), res(Xo))
+    run_g = lambda bb: (lambda q: (np.asarray(q.x), q.iterations))(idrs_amd.msidrs(A, A.pmul(bb), eta, 4, **kw))
+    _ms_check(g, o, run, run_g, b, res(X), res(Xo))
 
 
 def test_wedge_sommerfeld_lu_multishift_block(gpu):
@@ -251,7 +260,8 @@ def test_wedge_sommerfeld_lu_multishift_block(gpu):
     res = lambda Z: [np.linalg.norm(rhs - (Kd + s * Cd + s * s * Md) @ Z[n:, k]) / np.linalg.norm(rhs)
                      for k, s in enumerate(sigma)]
     run = lambda b1: (lambda q: (q["x"], q["iterations"]))(orc.msidrs(O, O.pmul(b1), eta_o, 4, **kw))
-    _ms_check(g, o, run, bb, res(X), res(Xo))
+    run_g = lambda b1: (lambda q: (np.asarray(q.x), q.iterations))(idrs_amd.msidrs(A, A.pmul(b1), eta, 4, **kw))
+    _ms_check(g, o, run, run_g, bb, res(X), res(Xo))
 
 
 def test_wedge_neumann_lu_single_frequency(gpu):
@@ -277,6 +287,8 @@ def test_wedge_neumann_lu_single_frequency(gpu):
     assert g.flag == o[1] == 0
     if abs(g.iterations - o[3]) > 2:
         run = lambda bb: (lambda q: (q[0], q[3]))(orc.idrs(O, bb, 4, **kw))
-        assert abs(g.iterations - o[3]) <= 2 * rounding_spread(run, b)[1]
+        run_g = lambda bb: (lambda q: (np.asarray(q.x), q.iterations))(idrs_amd.idrs(A, bb, 4, **kw))
+        spread = max(rounding_spread(run, b)[1], rounding_spread(run_g, b, trials=8)[1])
+        assert abs(g.iterations - o[3]) <= 2 * spread
     x = idrs_amd.scaleback(A, b, g.x)
     assert np.linalg.norm(b - Kd @ x + sigma * (Md @ x)) / np.linalg.norm(b) < 1e-5
