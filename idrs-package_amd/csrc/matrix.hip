@@ -1279,15 +1279,18 @@ static void refresh_diag(DevMatrix &K, hipStream_t st) {
     LAUNCH_CHECK();
 }
 
-static void *sliced_copy(const Sell &S, const void *src, bool cplx, hipStream_t st);
+static void *sliced_copy(const Sell &S, const void *src, bool cplx, hipStream_t st, void *dst = nullptr);
 
-// the sliced copy's values after K.val changed
+// the sliced copy's values after K.val changed, gathered into the existing
+// buffer (a free would synchronize the device on every refresh); the FP32
+// copy is dropped and rebuilt when next needed
 static void refresh_sliced(DevMatrix &K, hipStream_t st) {
     if (!K.sell) return;
-    (void)hipFree(K.sell->sl_val);
-    (void)hipFree(K.sell->sl_val32);
-    K.sell->sl_val = K.sell->sl_val32 = nullptr;
-    K.sell->sl_val = sliced_copy(*K.sell, K.val, K.kind == ValKind::Complex, st);
+    if (K.sell->sl_val32) {
+        (void)hipFree(K.sell->sl_val32);
+        K.sell->sl_val32 = nullptr;
+    }
+    K.sell->sl_val = sliced_copy(*K.sell, K.val, K.kind == ValKind::Complex, st, K.sell->sl_val);
 }
 
 void transpose_refresh(const DevMatrix &K, DevMatrix &KT, hipStream_t st) {
@@ -1314,8 +1317,10 @@ void set_values(DevMatrix &K, const void *vals, bool device, hipStream_t st) {
     const size_t vsz = K.kind == ValKind::Complex ? sizeof(c128) : sizeof(double);
     if (K.nnz)
         HIP_TRY(hipMemcpyAsync(K.val, vals, vsz * K.nnz, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-    (void)hipFree(K.val32);
-    K.val32 = nullptr;
+    if (K.val32) {
+        (void)hipFree(K.val32);
+        K.val32 = nullptr;
+    }
     refresh_sliced(K, st);
     if (K.nrows) refresh_diag(K, st);
 }
@@ -1481,10 +1486,9 @@ __global__ void k_sell_gather(long nent, const int *__restrict__ perm, const V *
     }
 }
 
-static void *sliced_copy(const Sell &S, const void *src, bool cplx, hipStream_t st) {
+static void *sliced_copy(const Sell &S, const void *src, bool cplx, hipStream_t st, void *dst) {
     const size_t vsz = cplx ? sizeof(c128) : sizeof(double);
-    void *dst = nullptr;
-    HIP_TRY(hipMalloc(&dst, vsz * (S.nent ? S.nent : 1)));
+    if (!dst) HIP_TRY(hipMalloc(&dst, vsz * (S.nent ? S.nent : 1)));
     const int g = vec_grid(S.nent);
     if (cplx)
         hipLaunchKernelGGL((k_sell_gather<c128>), dim3(g), dim3(256), 0, st, S.nent, S.sl_perm, (const c128 *)src,

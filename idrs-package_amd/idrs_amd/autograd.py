@@ -13,6 +13,7 @@ computed by one kernel over the pattern.  All of it runs in libidrs_hip.so;
 torch supplies device memory and the autograd graph.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -36,20 +37,41 @@ class SparseSystem:
         self.KT = UserMatrix(h.value, False)
         self.A = Matrix(self.K)
         self.AT = Matrix(self.KT)
+        # the tensor whose values K holds (a weak reference, its version
+        # counter and storage address), and whether KT holds their transpose
+        self._held = None
+        self._t_fresh = True
 
     def set_values(self, values):
-        """values: contiguous float64 CUDA tensor of nnz entries (CRS order)."""
+        """values: contiguous float64 CUDA tensor of nnz entries (CRS order).
+        Setting again the tensor object K already holds, unmodified since
+        (same version counter, same storage), copies nothing: an address
+        alone would not do, since a freed tensor's memory is handed to the
+        next one.  The transpose is refreshed when the adjoint solve needs
+        it (refresh_transpose)."""
         import torch
 
         if values.numel() != self.nnz:
             raise ValueError(f"expected {self.nnz} values, got {values.numel()}")
+        if self._held is not None:
+            ref, ver, ptr = self._held
+            if ref() is values and values._version == ver and values.data_ptr() == ptr:
+                return
         v = values.detach().contiguous()
         # the library reads v on its own stream: torch's pending work on v
         # (and on anything else the call reads) must have finished
         torch.cuda.current_stream().synchronize()
-        lib = _lib.load()
-        check(lib.idrs_user_matrix_set_values_f64(self.K.handle, ctypes.c_void_p(v.data_ptr()), 1))
-        check(lib.idrs_crs_transpose_refresh(self.K.handle, self.KT.handle))
+        self._held = None
+        self._t_fresh = False
+        check(_lib.load().idrs_user_matrix_set_values_f64(self.K.handle, ctypes.c_void_p(v.data_ptr()), 1))
+        if v.data_ptr() == values.data_ptr():
+            self._held = (weakref.ref(values), values._version, values.data_ptr())
+
+    def refresh_transpose(self):
+        """KT = K^T for the values K holds (one permutation gather)."""
+        if not self._t_fresh:
+            check(_lib.load().idrs_crs_transpose_refresh(self.K.handle, self.KT.handle))
+            self._t_fresh = True
 
     def pattern_grad(self, lam, x, out):
         import torch
@@ -89,6 +111,7 @@ def _function():
             system = ctx.system
             # the pattern's values may have been reset by a later forward
             system.set_values(values)
+            system.refresh_transpose()
             lam = _solve(system.AT, gx.detach().contiguous(), ctx.s, ctx.tol, ctx.maxit, "adjoint")
             gvals = None
             if ctx.needs_input_grad[0]:

@@ -142,3 +142,53 @@ def test_set_values_waits_for_torch(gpu):
         del v
         c2 = P.Crs(crs.n, crs.row_ptr, crs.col_ind, crs.val * scale + float(k))
         assert rel(sysm.K * x, c2.matvec(x)) < 1e-14, k
+
+
+def test_sparse_system_held_values(gpu, monkeypatch):
+    """SparseSystem.set_values copies nothing for the tensor it already
+    holds, and copies again after an in-place change of that tensor (its
+    version counter moves) or for another tensor; the transpose follows on
+    refresh_transpose.  Both matrices use the sliced copy, whose values are
+    gathered again into the buffer they already have."""
+    import torch
+
+    import idrs_amd.autograd as AG
+
+    real = _lib.load()
+    calls = []
+
+    class Counting:
+        def __getattr__(self, name):
+            f = getattr(real, name)
+            if name != "idrs_user_matrix_set_values_f64":
+                return f
+            return lambda *a: (calls.append(name), f(*a))[1]
+
+    monkeypatch.setattr(AG._lib, "load", lambda: Counting())
+    dev = torch.device("cuda:0")
+    crs = P.random_dominant(5000, k=7, seed=8)
+    sysm = idrs_amd.SparseSystem(crs.n, crs.row_ptr, crs.col_ind, crs.val)
+    sysm.K.set_group(1)
+    sysm.KT.set_group(1)
+    x = np.random.default_rng(9).standard_normal(crs.n)
+
+    def holds(vals, ncalls):
+        c = P.Crs(crs.n, crs.row_ptr, crs.col_ind, vals)
+        assert len(calls) == ncalls
+        assert rel(sysm.K * x, c.matvec(x)) < 1e-14
+        sysm.refresh_transpose()
+        assert rel(sysm.KT * x, c.to_scipy().T @ x) < 1e-14
+
+    v = torch.tensor(crs.val * 2.0, device=dev)
+    sysm.set_values(v)
+    holds(crs.val * 2.0, 1)
+    sysm.set_values(v)                  # held and unchanged: no copy
+    holds(crs.val * 2.0, 1)
+    v.mul_(-0.5)                        # in place: the version counter moves
+    sysm.set_values(v)
+    holds(crs.val * -1.0, 2)
+    w = v + 3.0
+    sysm.set_values(w)
+    holds(crs.val * -1.0 + 3.0, 3)
+    sysm.set_values(v)
+    holds(crs.val * -1.0, 4)
