@@ -448,40 +448,69 @@ __global__ __launch_bounds__(SPMV_THREADS) void k_spmv_dia(SpmvArgs<VT, T> A, Ep
     const T *zp = A.z;   // Horner vector; products without one read nothing here
     // diagonal d is stored at d - (d > ud); the implicit unit diagonal
     // (d == ud) enters as fmav(1, x, acc), the same operation as a stored
-    // 1.0.  With a known diagonal count the offsets, value streams and x
-    // bases are wave-uniform (scalar registers), rows index them in 32 bits.
+    // 1.0.  With a known diagonal count the offsets and value streams are
+    // wave-uniform (scalar registers), rows index them in 32 bits.
     int offr[ND > 0 ? ND : 1];
     const VT *vb[ND > 0 ? ND : 1];
-    const T *xb[ND > 0 ? ND : 1];
     if constexpr (ND > 0) {
 #pragma unroll
         for (int d = 0; d < ND; d++) {
             offr[d] = A.off8[d];
             vb[d] = val + (size_t)(d - (d > ud ? 1 : 0)) * (size_t)n;
-            xb[d] = x + offr[d];
         }
     }
     for (long c = c0; c < c1; c++) {
         const long row = c * SPMV_THREADS + threadIdx.x;
         if (row >= n) continue;
         T acc = T(0);
-        const T zq = zp ? zp[row] : T(0);
+        T zq;
         T pe[PC];
-        if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
         if constexpr (ND > 0) {
-            const int r32 = (int)row;
+            // 32-bit byte offsets from the wave-uniform bases (one offset
+            // register for every stream of the row instead of a 64-bit
+            // address each); the launcher takes this path only while the
+            // offsets fit: n * 8 * max(1, prow) < 2^32
+            const unsigned bo = (unsigned)row * (unsigned)sizeof(T);
+            const unsigned vbo = (unsigned)row * (unsigned)sizeof(VT);
+            zq = zp ? *(const T *)((const char *)zp + bo) : T(0);
+            if constexpr (EPI == EPI_PDOT) {
+                const T *p = (const T *)((const char *)E.P + (unsigned)row * (unsigned)(E.prow * sizeof(T)));
+                epi_load_row<T, PC>(pe, E, p);
+            }
             VT vv[ND];
             T xx[ND];
 #pragma unroll
             for (int d = 0; d < ND; d++) {
+                // every load in bounds without a branch: the stored value
+                // of row `row` exists on each diagonal, and a column outside
+                // the matrix reads x(row) instead (the launcher takes this
+                // path only for nrows <= ncols); both enter as zero
                 const long cc = row + offr[d];
                 const bool ok = cc >= 0 && cc < ncols;
-                vv[d] = !ok ? VT(0) : (d == ud ? VT(1) : ldnt(vb[d] + r32));
-                xx[d] = ok ? xb[d][r32] : T(0);
+                if constexpr (std::is_same<VT, T>::value) {
+                    const unsigned xo = (unsigned)(ok ? cc : row) * (unsigned)sizeof(T);
+                    const VT v = d == ud ? VT(1) : ldnt((const VT *)((const char *)vb[d] + vbo));
+                    const T xv = *(const T *)((const char *)x + xo);
+                    vv[d] = ok ? v : VT(0);
+                    xx[d] = ok ? xv : T(0);
+                } else {
+                    // values of another type: loads under the column test, so
+                    // that the conversion is not placed next to each load
+                    // (which would wait for it before the next is issued)
+                    vv[d] = !ok ? VT(0) : (d == ud ? VT(1) : ldnt((const VT *)((const char *)vb[d] + vbo)));
+                    xx[d] = ok ? *(const T *)((const char *)x + (unsigned)cc * (unsigned)sizeof(T)) : T(0);
+                }
             }
 #pragma unroll
             for (int d = 0; d < ND; d++) acc = fmav(vv[d], xx[d], acc);
+            const T yv = finish_value(A, row, acc, zq);
+            *(T *)((char *)A.y + bo) = yv;
+            if constexpr (EPI != EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
+            epi_accum<T, EPI, NV, PC>(e, pe, E, yv, 0, 1);
+            continue;
         } else {
+            zq = zp ? zp[row] : T(0);
+            if constexpr (EPI == EPI_PDOT) epi_load<T, EPI, PC>(pe, E, row, 0, 1);
             for (int d = 0; d < ndiag; d++) {
                 const long cc = row + s_off[d];
                 const long sd = d - (d > ud ? 1 : 0);
@@ -609,7 +638,11 @@ static void launch_spmv(const DevMatrix &K, const SpmvArgs<VT, T> &A, EpiArgs<T>
         default: hipLaunchKernelGGL((k_spmv_crs<VT, T, EPI, NV, 64>), gd, bd, 0, st, A, E, info); break;
         }
     } else {
-        switch (K.ndiag) {
+        // the unrolled kernels address rows by 32-bit byte offsets
+        const unsigned long long rowb = sizeof(T) * (EPI == EPI_PDOT ? (unsigned long long)std::max(1, E.prow) : 1ull);
+        const int nd =
+            K.nrows <= K.ncols && (unsigned long long)K.ncols * rowb < (1ull << 32) ? K.ndiag : 0;
+        switch (nd) {
         case 3: hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV, 3>), gd, bd, 0, st, A, E, info); break;
         case 5: hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV, 5>), gd, bd, 0, st, A, E, info); break;
         case 7: hipLaunchKernelGGL((k_spmv_dia<VT, T, EPI, NV, 7>), gd, bd, 0, st, A, E, info); break;

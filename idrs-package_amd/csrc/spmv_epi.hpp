@@ -105,6 +105,25 @@ __device__ inline void epi_load(T (&pe)[PC], const EpiArgs<T> &E, long row, int 
     }
 }
 
+// The whole P row at p (PDOT with step 1 from component 0): as epi_load.
+template <class T, int PC>
+__device__ inline void epi_load_row(T (&pe)[PC], const EpiArgs<T> &E, const T *p) {
+    if constexpr (std::is_same<T, double>::value && PC % 2 == 0) {
+        if (E.prow == PC) {
+#pragma unroll
+            for (int k = 0; k < PC; k += 2) {
+                const dbl2_nt v = __builtin_nontemporal_load((const dbl2_nt *)(p + k));
+                pe[k] = v.x;
+                pe[k + 1] = v.y;
+            }
+            return;
+        }
+    }
+    const int last = E.prow - 1;
+#pragma unroll
+    for (int k = 0; k < PC; k++) pe[k] = ldnt(p + min(k, last));
+}
+
 // Accumulates a row into the epilogue sums: PDOT adds P components
 // c, c+step, ... (zero beyond prow) times y into e[0..PC); OMEGA adds
 // (r,r), (y,y), (y,r); NORM adds |y|^2.
