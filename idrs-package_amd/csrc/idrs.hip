@@ -183,23 +183,56 @@ __global__ void k_copy_conj(long n, const T *__restrict__ r, T *__restrict__ PT)
     GRID_LOOP(q, n) PT[q] = Tr<T>::conj(r[q]);
 }
 
-// alpha(k) = dot_product(PT(j,:),PT(k,:)) [real] / dot_product(PT(k,:),PT(j,:)) [complex], k < j
-template <class T, int NV>
-__global__ __launch_bounds__(256) void k_gs_dots(long n, int s, int j, const T *__restrict__ PT, T *partials,
+// PT(j,:) / rho, the normalisation of a finished row
+template <class T>
+__device__ inline T gs_scaled(T p, double rho) {
+    if constexpr (Tr<T>::is_complex)
+        return T(p.re / rho, p.im / rho);
+    else
+        return p / rho;
+}
+
+// alpha(k) = dot_product(PT(j,:),PT(k,:)) [real] / dot_product(PT(k,:),PT(j,:)) [complex], k < j.
+// SC: row j-1 is first divided by st->rho and stored (k_gs_scale's step in
+// the same pass; the dot with it uses the stored value)
+template <class T, int NV, bool SC>
+__global__ __launch_bounds__(256) void k_gs_dots(long n, int s, int j, const State<T> *st, T *PT, T *partials,
                                                  int nparts) {
     __shared__ T lds[NV * 4];
+    const double rho = SC ? st->rho : 1.0;
     T e[NV];
 #pragma unroll
     for (int k = 0; k < NV; k++) e[k] = T(0);
     GRID_LOOP(q, n) {
         const T pj = PT[j + (long)s * q];
+        T pl = T(0);
+        if constexpr (SC) {
+            pl = gs_scaled(PT[j - 1 + (long)s * q], rho);
+            PT[j - 1 + (long)s * q] = pl;
+        }
 #pragma unroll
         for (int k = 0; k < NV; k++)
-            if (k < j) e[k] = fmaT(Tr<T>::conj(PT[k + (long)s * q]), pj, e[k]);
+            if (k < j) e[k] = fmaT(Tr<T>::conj((SC && k == j - 1) ? pl : PT[k + (long)s * q]), pj, e[k]);
     }
     block_sum<T, NV>(e, lds);
     if (threadIdx.x == 0)
         for (int k = 0; k < NV; k++) partials[(long)k * nparts + blockIdx.x] = e[k];
+}
+
+// row 0 of a given P: PT = PT_raw (every row) and partials = |PT(0,:)|^2,
+// the copy and k_gs_update_norm's j = 0 pass in one
+template <class T>
+__global__ __launch_bounds__(256) void k_gs_first(long n, int s, const T *PT_raw, T *PT, T *partials) {
+    __shared__ T lds[4];
+    T e[1] = {T(0)};
+    GRID_LOOP(q, n) {
+        for (int i = 1; i < s; i++) PT[i + (long)s * q] = PT_raw[i + (long)s * q];
+        const T p = PT_raw[(long)s * q];
+        PT[(long)s * q] = p;
+        e[0] = e[0] + T(Tr<T>::abs2(p));
+    }
+    block_sum<T, 1>(e, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = e[0];
 }
 
 // k_gs_dots for rows j above PDOT_MAX: grid row y sums the coefficients
@@ -260,13 +293,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_fin_rho(State<T> *st, const T *
 template <class T>
 __global__ void k_gs_scale(long n, int s, int j, const State<T> *st, T *PT) {
     const double rho = st->rho;
-    GRID_LOOP(q, n) {
-        T p = PT[j + (long)s * q];
-        if constexpr (Tr<T>::is_complex)
-            PT[j + (long)s * q] = T(p.re / rho, p.im / rho);
-        else
-            PT[j + (long)s * q] = p / rho;
-    }
+    GRID_LOOP(q, n) PT[j + (long)s * q] = gs_scaled(PT[j + (long)s * q], rho);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,21 +546,35 @@ static void launch_omega_update(long n, const State<T> *st, const T *v, const T 
     LAUNCH_CHECK();
 }
 
+// the dots of row j with rows 0..j-1; with scale_prev row j-1 is
+// normalised in the same pass (returns false where that is not offered:
+// the caller scales it first)
 template <class T>
-static void launch_gs_dots(long n, int s, int j, const T *PT, T *partials, hipStream_t st) {
+static bool launch_gs_dots(long n, int s, int j, const State<T> *sta, T *PT, T *partials, bool scale_prev,
+                           hipStream_t st) {
     int g = vec_grid(n);
     if (j > PDOT_MAX) {
+        if (scale_prev) return false;
         hipLaunchKernelGGL((k_gs_dots_wide<T>), dim3(g, ceil_div(j, PDOT_MAX)), dim3(256), 0, st, n, s, j, PT,
                            partials, g);
         LAUNCH_CHECK();
-        return;
+        return true;
     }
-    if (j <= 1) hipLaunchKernelGGL((k_gs_dots<T, 1>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
-    else if (j <= 2) hipLaunchKernelGGL((k_gs_dots<T, 2>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
-    else if (j <= 4) hipLaunchKernelGGL((k_gs_dots<T, 4>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
-    else if (j <= 8) hipLaunchKernelGGL((k_gs_dots<T, 8>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
-    else hipLaunchKernelGGL((k_gs_dots<T, 16>), dim3(g), dim3(256), 0, st, n, s, j, PT, partials, g);
+#define GD(NV)                                                                                                  \
+    do {                                                                                                        \
+        if (scale_prev)                                                                                         \
+            hipLaunchKernelGGL((k_gs_dots<T, NV, true>), dim3(g), dim3(256), 0, st, n, s, j, sta, PT, partials, g); \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_gs_dots<T, NV, false>), dim3(g), dim3(256), 0, st, n, s, j, sta, PT, partials, g); \
+    } while (0)
+    if (j <= 1) GD(1);
+    else if (j <= 2) GD(2);
+    else if (j <= 4) GD(4);
+    else if (j <= 8) GD(8);
+    else GD(16);
+#undef GD
     LAUNCH_CHECK();
+    return true;
 }
 
 // MAKE_P body on the device, using 'st' for the Gram-Schmidt coefficients.
@@ -548,26 +589,35 @@ static void make_p_impl(long n, int s, int method, const T *PT_raw, const T *r, 
         return;
     }
     const long cnt = (long)rows * n;
-    if (PT_raw) {
-        HIP_TRY(hipMemcpyAsync(PT, PT_raw, sizeof(T) * cnt, hipMemcpyDeviceToDevice, stream));
-    } else {
+    if (!PT_raw) {
         hipLaunchKernelGGL((k_fill_uniform<T>), dim3(vec_grid(cnt)), dim3(256), 0, stream, cnt, PT, seed);
         LAUNCH_CHECK();
     }
-    // classical Gram-Schmidt over the rows, then normalise (one row for 3/4)
+    // classical Gram-Schmidt over the rows, then normalise (one row for
+    // 3/4).  The copy of a given P rides on row 0's norm pass, and each
+    // row's normalisation on the next row's dot pass (the last row's on a
+    // pass of its own): the same operations in the same order as separate
+    // passes.
     for (int j = 0; j < rows; j++) {
         if (j > 0) {
-            launch_gs_dots<T>(n, rows, j, PT, partials, stream);
+            if (!launch_gs_dots<T>(n, rows, j, st, PT, partials, true, stream)) {
+                hipLaunchKernelGGL((k_gs_scale<T>), dim3(g), dim3(256), 0, stream, n, rows, j - 1, st, PT);
+                LAUNCH_CHECK();
+                launch_gs_dots<T>(n, rows, j, st, PT, partials, false, stream);
+            }
             hipLaunchKernelGGL((k_fin_gs<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, partials, g, j);
             LAUNCH_CHECK();
         }
-        hipLaunchKernelGGL((k_gs_update_norm<T>), dim3(g), dim3(256), 0, stream, n, rows, j, st, PT, partials);
+        if (j == 0 && PT_raw)
+            hipLaunchKernelGGL((k_gs_first<T>), dim3(g), dim3(256), 0, stream, n, rows, PT_raw, PT, partials);
+        else
+            hipLaunchKernelGGL((k_gs_update_norm<T>), dim3(g), dim3(256), 0, stream, n, rows, j, st, PT, partials);
         LAUNCH_CHECK();
         hipLaunchKernelGGL((k_fin_rho<T>), dim3(1), dim3(FIN_THREADS), 0, stream, st, partials, g);
         LAUNCH_CHECK();
-        hipLaunchKernelGGL((k_gs_scale<T>), dim3(g), dim3(256), 0, stream, n, rows, j, st, PT);
-        LAUNCH_CHECK();
     }
+    hipLaunchKernelGGL((k_gs_scale<T>), dim3(g), dim3(256), 0, stream, n, rows, rows - 1, st, PT);
+    LAUNCH_CHECK();
 }
 
 template <class T>
