@@ -653,6 +653,58 @@ static T *upload_diag_as(const Operator *M1, long n, Buf &buf, hipStream_t strea
     return buf.as<T>();
 }
 
+// Pinned status words and events of a solve's host loop, kept for reuse:
+// hipHostMalloc and hipHostFree cost tens of microseconds each and the free
+// waits for the device.  A pool, so that nested or concurrent solves take
+// sets of their own; a set stays with the device it was made on.
+namespace {
+struct LoopRes {
+    int dev = -1;
+    int *h_info = nullptr;
+    hipEvent_t pe[2] = {nullptr, nullptr}, e0 = nullptr, e1 = nullptr;
+};
+std::mutex g_loop_mu;
+std::vector<LoopRes> g_loop_res;
+
+LoopRes loop_res_take() {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> g(g_loop_mu);
+        for (size_t i = 0; i < g_loop_res.size(); i++)
+            if (g_loop_res[i].dev == dev) {
+                LoopRes r = g_loop_res[i];
+                g_loop_res.erase(g_loop_res.begin() + (long)i);
+                return r;
+            }
+    }
+    LoopRes r;
+    r.dev = dev;
+    HIP_TRY(hipHostMalloc((void **)&r.h_info, 2 * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&r.pe[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&r.pe[1], hipEventDisableTiming));
+    HIP_TRY(hipEventCreate(&r.e0));
+    HIP_TRY(hipEventCreate(&r.e1));
+    return r;
+}
+void loop_res_give(const LoopRes &r) {
+    std::lock_guard<std::mutex> g(g_loop_mu);
+    g_loop_res.push_back(r);
+}
+// gives its set back to the pool on every way out of the solve
+struct LoopResHold {
+    LoopRes r;
+    bool held = false;
+    void take() {
+        r = loop_res_take();
+        held = true;
+    }
+    ~LoopResHold() {
+        if (held) loop_res_give(r);
+    }
+};
+}  // namespace
+
 template <class T>
 int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOpts<T> &o, SolveStats &out,
                hipStream_t stream) {
@@ -728,12 +780,14 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     // synchronised solve
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::chrono::steady_clock::time_point t0;
+    LoopResHold lres;
     if (ar) {
         HIP_TRY(hipStreamSynchronize(stream));
         t0 = std::chrono::steady_clock::now();
     } else {
-        HIP_TRY(hipEventCreate(&e0));
-        HIP_TRY(hipEventCreate(&e1));
+        lres.take();
+        e0 = lres.r.e0;
+        e1 = lres.r.e1;
         HIP_TRY(hipEventRecord(e0, stream));
     }
     tm.reset();
@@ -830,22 +884,8 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     // flag, events); the allocation-free solve instead reads the flag after
     // every cycle with a blocking copy into host memory it already has
     int h_local[2] = {-1, -1};
-    int *h_info = h_local;
-    hipEvent_t pe[2] = {nullptr, nullptr};
-    if (!ar) {
-        HIP_TRY(hipHostMalloc((void **)&h_info, 2 * sizeof(int), hipHostMallocDefault));
-        HIP_TRY(hipEventCreateWithFlags(&pe[0], hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&pe[1], hipEventDisableTiming));
-    }
-    auto release = [&]() {
-        if (!ar) {
-            (void)hipHostFree(h_info);
-            (void)hipEventDestroy(pe[0]);
-            (void)hipEventDestroy(pe[1]);
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-        }
-    };
+    int *h_info = ar ? h_local : lres.r.h_info;
+    hipEvent_t pe[2] = {lres.r.pe[0], lres.r.pe[1]};
     long max_cycles = cap_it / (s + 1) + 2;
     // Small systems are bound by the host's launch rate (~55 launches per
     // cycle): capture one cycle in a HIP graph and replay it -- the same
@@ -863,7 +903,6 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(stream, &g);
             if (g) (void)hipGraphDestroy(g);
-            release();
             throw;
         }
         HIP_TRY(hipStreamEndCapture(stream, &graph));
@@ -914,12 +953,10 @@ int idrs_solve(const Operator &A, long n, int s, const T *b, T *x, const SolveOp
     } catch (...) {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
-        release();
         throw;
     }
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    release();
     return 0;
 }
 
