@@ -37,18 +37,28 @@
 namespace idrs {
 
 // ---------------------------------------------------------------------------
-// small caching device allocator (solves of equal size reuse workspaces)
+// small caching allocators (solves of equal size reuse workspaces; a free
+// would wait for the device): device buffers by (device, size), pinned
+// host buffers by (size, flags).  Buffers stay allocated until the process
+// ends.
 // ---------------------------------------------------------------------------
 namespace {
 std::mutex g_pool_mu;
-std::multimap<size_t, void *> g_pool;
+std::multimap<std::pair<int, size_t>, void *> g_pool;
+std::multimap<std::pair<size_t, unsigned>, void *> g_pinned;
+size_t pool_round(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    return bytes ? bytes : 256;
+}
+}  // namespace
 
 void *pool_alloc(size_t bytes) {
-    bytes = (bytes + 255) & ~(size_t)255;
-    if (bytes == 0) bytes = 256;
+    bytes = pool_round(bytes);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
     {
         std::lock_guard<std::mutex> g(g_pool_mu);
-        auto it = g_pool.find(bytes);
+        auto it = g_pool.find({dev, bytes});
         if (it != g_pool.end()) {
             void *p = it->second;
             g_pool.erase(it);
@@ -61,11 +71,33 @@ void *pool_alloc(size_t bytes) {
 }
 void pool_free(void *p, size_t bytes) {
     if (!p) return;
-    bytes = (bytes + 255) & ~(size_t)255;
-    if (bytes == 0) bytes = 256;
+    int dev = 0;
+    if (hipPointerAttribute_t at{}; hipPointerGetAttributes(&at, p) == hipSuccess) dev = at.device;
     std::lock_guard<std::mutex> g(g_pool_mu);
-    g_pool.emplace(bytes, p);
+    g_pool.emplace(std::make_pair(dev, pool_round(bytes)), p);
 }
+void *pinned_take(size_t bytes, unsigned flags) {
+    bytes = pool_round(bytes);
+    {
+        std::lock_guard<std::mutex> g(g_pool_mu);
+        auto it = g_pinned.find({bytes, flags});
+        if (it != g_pinned.end()) {
+            void *p = it->second;
+            g_pinned.erase(it);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    HIP_TRY(hipHostMalloc(&p, bytes, flags));
+    return p;
+}
+void pinned_give(void *p, size_t bytes, unsigned flags) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(g_pool_mu);
+    g_pinned.emplace(std::make_pair(pool_round(bytes), flags), p);
+}
+
+namespace {
 // a device buffer from the caching pool, or (with an arena) a piece of the
 // caller's region that is not freed
 struct Buf {

@@ -95,6 +95,13 @@ int idrs_ritz_solve(const Operator &A, long n, int s, const T *b, T *x, SolveOpt
                     SolveStats &st, hipStream_t stream);
 
 // Standalone P generation (MAKE_P) for tests: PT(s,n) orthonormalised, device.
+// caching allocators shared by the solvers (idrs.hip): device buffers and
+// pinned host buffers that solves of equal size reuse
+void *pool_alloc(size_t bytes);
+void pool_free(void *p, size_t bytes);
+void *pinned_take(size_t bytes, unsigned flags);
+void pinned_give(void *p, size_t bytes, unsigned flags);
+
 template <class T>
 void make_p_device(long n, int s, int method, const T *PT_raw, const T *r, T *PT, unsigned long long seed,
                    hipStream_t stream);

@@ -395,25 +395,28 @@ struct Work {
     unsigned *hseq = nullptr, *hseq_d = nullptr;
     unsigned seq = 0;
     int cslot = 0;                             // hcoef half the next put_coef fills
-    std::vector<void *> bufs;
+    // device buffers (from the solvers' caching pool) and their sizes
+    std::vector<std::pair<void *, size_t>> bufs;
+    static constexpr unsigned MAPPED = hipHostMallocCoherent | hipHostMallocMapped;
     explicit Work(long n_, hipStream_t s) : st(s), n(n_), gv(vec_grid(n_)) {
         part = alloc(MAX_NV * (size_t)MAX_PARTS);
         part2 = alloc(MAX_PARTS);
         dcoef = alloc(NCOEF);
         dsc = alloc(MAX_RED);
-        HIP_TRY(hipHostMalloc((void **)&hres, sizeof(T) * 2 * MAX_RED, hipHostMallocCoherent | hipHostMallocMapped));
-        HIP_TRY(hipHostMalloc((void **)&hseq, 64, hipHostMallocCoherent | hipHostMallocMapped));
+        hres = (T *)pinned_take(sizeof(T) * 2 * MAX_RED, MAPPED);
+        hseq = (unsigned *)pinned_take(64, MAPPED);
         *hseq = 0;
         HIP_TRY(hipHostGetDevicePointer((void **)&hres_d, hres, 0));
         HIP_TRY(hipHostGetDevicePointer((void **)&hseq_d, hseq, 0));
-        HIP_TRY(hipHostMalloc((void **)&hcoef, sizeof(T) * 2 * NCOEF, hipHostMallocDefault));
+        hcoef = (T *)pinned_take(sizeof(T) * 2 * NCOEF, hipHostMallocDefault);
     }
+    // back to the pools once the stream has finished with them
     ~Work() {
         (void)hipStreamSynchronize(st);
-        for (void *p : bufs) (void)hipFree(p);
-        (void)hipHostFree(hres);
-        (void)hipHostFree(hseq);
-        (void)hipHostFree(hcoef);
+        for (auto &b : bufs) pool_free(b.first, b.second);
+        pinned_give(hres, sizeof(T) * 2 * MAX_RED, MAPPED);
+        pinned_give(hseq, 64, MAPPED);
+        pinned_give(hcoef, sizeof(T) * 2 * NCOEF, hipHostMallocDefault);
     }
     // waits for k_red_host's sequence number; after a second of spinning
     // (long queued work) it waits on the stream instead
@@ -430,9 +433,9 @@ struct Work {
         }
     }
     T *alloc(size_t count) {
-        void *p = nullptr;
-        HIP_TRY(hipMalloc(&p, sizeof(T) * (count ? count : 1)));
-        bufs.push_back(p);
+        const size_t bytes = sizeof(T) * (count ? count : 1);
+        void *p = pool_alloc(bytes);
+        bufs.emplace_back(p, bytes);
         return (T *)p;
     }
     T *zeros(size_t count) {
