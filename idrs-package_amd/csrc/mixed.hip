@@ -67,10 +67,13 @@ __global__ __launch_bounds__(FIN_THREADS) void k_sum(const T *partials, int npar
     if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// a device buffer from the solvers' caching pool (every call runs on the
+// library's one stream, so a buffer handed back is reused in stream order)
 struct Dev {
     void *p = nullptr;
-    explicit Dev(size_t bytes) { HIP_TRY(hipMalloc(&p, bytes ? bytes : 1)); }
-    ~Dev() { (void)hipFree(p); }
+    size_t n = 0;
+    explicit Dev(size_t bytes) : n(bytes ? bytes : 1) { p = pool_alloc(n); }
+    ~Dev() { pool_free(p, n); }
     Dev(const Dev &) = delete;
     Dev &operator=(const Dev &) = delete;
     template <class V> V *as() const { return (V *)p; }

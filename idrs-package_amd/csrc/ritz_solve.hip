@@ -26,8 +26,9 @@ namespace {
 template <class T>
 struct DBuf {
     T *p = nullptr;
-    explicit DBuf(size_t cnt) { HIP_TRY(hipMalloc((void **)&p, sizeof(T) * (cnt ? cnt : 1))); }
-    ~DBuf() { (void)hipFree(p); }
+    size_t bytes = 0;
+    explicit DBuf(size_t cnt) : bytes(sizeof(T) * (cnt ? cnt : 1)) { p = (T *)pool_alloc(bytes); }
+    ~DBuf() { pool_free(p, bytes); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
 };
