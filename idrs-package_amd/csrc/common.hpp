@@ -222,6 +222,15 @@ __device__ inline cplx<R> ldnt(const cplx<R> *p) {
     const R *q = reinterpret_cast<const R *>(p);
     return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
 }
+// non-temporal stores, for outputs no later kernel reads soon
+template <class V>
+__device__ inline void stnt(V *p, V v) { __builtin_nontemporal_store(v, p); }
+template <class R>
+__device__ inline void stnt(cplx<R> *p, cplx<R> v) {
+    R *q = reinterpret_cast<R *>(p);
+    __builtin_nontemporal_store(v.re, q);
+    __builtin_nontemporal_store(v.im, q + 1);
+}
 typedef double dbl2_nt __attribute__((ext_vector_type(2)));
 
 // XCD-aware workgroup index: gfx950 dispatches consecutive workgroups round

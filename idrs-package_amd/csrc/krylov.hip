@@ -323,11 +323,11 @@ __global__ __launch_bounds__(256) void k_qmr_update(long n, int ns, int s, int h
             T m = T(0);
             for (int j = 0; j < nw; j++) {
                 const int pj = (head + j) % nw;
-                m = kfma(Ws[(long)n * pj + q], c[j], m);
+                m = kfma(ldnt(&Ws[(long)n * pj + q]), c[j], m);
             }
             const T wn = kdiv(vt[vt_stride * ks + q] - m, c[s + 1]);
-            Ws[(long)n * head + q] = wn;
-            X[(long)n * ks + q] = kfma(c[s + 2], wn, X[(long)n * ks + q]);
+            stnt(&Ws[(long)n * head + q], wn);
+            stnt(&X[(long)n * ks + q], kfma(c[s + 2], wn, ldnt(&X[(long)n * ks + q])));
         }
     }
 }
