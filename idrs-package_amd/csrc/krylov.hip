@@ -281,10 +281,10 @@ __global__ __launch_bounds__(256) void k_msidrs_update(long n, int ns, int s, in
             const T *c = (STAGED ? sc : coef) + ks * stride;
             T t = vq - (c[0] * v0q) * c[1];
             T *Wk = W + (long)n * s * ks;
-            for (int i = 0; i < s; i++) t = kfma(-Wk[(long)n * i + q], c[2 + i], t);
+            for (int i = 0; i < s; i++) t = kfma(-ldnt(&Wk[(long)n * i + q]), c[2 + i], t);
             const T wk = kdiv(t, c[s + 2]);
-            Wk[(long)n * k + q] = wk;
-            X[(long)n * ks + q] = kfma(c[s + 3], wk, X[(long)n * ks + q]);
+            stnt(&Wk[(long)n * k + q], wk);
+            stnt(&X[(long)n * ks + q], kfma(c[s + 3], wk, ldnt(&X[(long)n * ks + q])));
         }
     }
 }
