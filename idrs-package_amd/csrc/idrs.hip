@@ -474,7 +474,7 @@ __global__ __launch_bounds__(256) void k_orth_update(long n, int k, const State<
         G[(long)k * n + q] = gk;
         U[(long)k * n + q] = uk;
         const T rq = fmaT(-beta, gk, r[q]);
-        x[q] = fmaT(beta, uk, x[q]);
+        stnt(&x[q], fmaT(beta, uk, ldnt(&x[q])));   // x is read again only by the next update
         r[q] = rq;
         e[0] = e[0] + T(Tr<T>::abs2(rq));
         if (next) {
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(256) void k_omega_update(long n, const State<T> *st
     GRID_LOOP(q, n) {
         const T vq = v[q];
         const T rq = fmaT(-om, t[q], r[q]);
-        x[q] = fmaT(om, vq, x[q]);
+        stnt(&x[q], fmaT(om, vq, ldnt(&x[q])));
         r[q] = rq;
         e[0] = e[0] + T(Tr<T>::abs2(rq));
         if constexpr (std::is_same<T, double>::value && NV % 2 == 0) {
