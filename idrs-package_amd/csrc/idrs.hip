@@ -471,8 +471,8 @@ __global__ __launch_bounds__(256) void k_orth_update(long n, int k, const State<
             gk = fmaT(-sa[i], ldnt(&G[(long)i * n + q]), gk);
             uk = fmaT(-sa[i], ldnt(&U[(long)i * n + q]), uk);
         }
-        G[(long)k * n + q] = gk;
-        U[(long)k * n + q] = uk;
+        stnt(&G[(long)k * n + q], gk);   // read again only after other streams have passed
+        stnt(&U[(long)k * n + q], uk);
         const T rq = fmaT(-beta, gk, r[q]);
         stnt(&x[q], fmaT(beta, uk, ldnt(&x[q])));   // x is read again only by the next update
         r[q] = rq;
